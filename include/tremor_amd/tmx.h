@@ -1,0 +1,127 @@
+/*
+ * tmx.h - C-ABI of the MI355X template-matching library (libtremor_amd.so).
+ *
+ * Exact z-normalised Euclidean distance / Pearson correlation of every template
+ * against every subsequence (window) of one long float32 waveform, for kNN and
+ * threshold search, on one AMD Instinct MI355X (gfx950).  The answers are the
+ * reference sequential scan's answers: same positions, same FP32 distances.
+ *
+ * Conventions
+ *   - Plain C types, caller-owned inputs (copied), caller-allocated outputs,
+ *     except threshold hits (library-owned until tmx_hits_free).
+ *   - Every call returns 0 on success, > 0 for a recoverable condition,
+ *     < 0 for an error; tmx_last_error() then describes it.  No exceptions
+ *     cross this boundary.
+ *   - Calls on one context are serialised by the caller; a context owns one
+ *     HIP stream on one device.
+ *   - Distances are squared z-normalised Euclidean distances d = 2m(1 - rho)
+ *     (ZnormSearch.hpp:23-31).  The missing-sample sentinel is -100000.0f
+ *     (ZnormSearch.hpp:21): a window whose first sample is missing is never
+ *     reported; missing samples inside a window count as 0.
+ *
+ * Reference interfaces each entry point replaces (files of the reference
+ * repository, tremor/...):
+ *   tmx_load_waveform    SkipSequentialScan::build           algos/sss/SkipSequential.cpp:138-198
+ *                        (window statistics)                 algos/tremor/indexing.cpp:128-153
+ *   tmx_knn              SkipSequentialScan::search_knn      algos/sss/SkipSequential.cpp:296-393
+ *                        Tremor::searchIndex (I, D outputs)  algos/htremor/Tremor.hpp:172
+ *   tmx_threshold        SkipSequentialScan::search_threshold algos/sss/SkipSequential.cpp:223-294
+ *   tmx_distance_batch   l2_dist_znorm_subsequence           distance_computers/ZnormSearch.hpp:103-165
+ *   tmx_window_stats     two-pass window statistics          algos/sss/SkipSequential.cpp:157-179
+ *   tmx_prepare_template znormalize_inplace + query_block_order ZnormSearch.hpp:33-90
+ */
+#ifndef TREMOR_AMD_TMX_H
+#define TREMOR_AMD_TMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tmx_ctx tmx_ctx;
+
+/* Arithmetic of the candidate-generating inner product.  Every form is
+ * followed by the exact verification pass, so all return identical answers. */
+enum tmx_precision {
+    TMX_PREC_FP32 = 0,   /* plain FP32 sliding dot product on the vector ALUs */
+    TMX_PREC_BF16X3 = 1, /* Hankel GEMM on MFMA, split-BF16 (hi*hi + hi*lo + lo*hi), FP32 accumulation */
+    TMX_PREC_FP16 = 2    /* Hankel GEMM on MFMA, FP16 operands, FP32 accumulation */
+};
+
+typedef struct {
+    int32_t device;          /* HIP device ordinal */
+    int32_t precision;       /* enum tmx_precision */
+    int64_t cand_capacity;   /* candidate slots per call; 0 = default (grown on demand) */
+    int32_t seed_target;     /* kNN bound seeding: target seed entries per template; 0 = default */
+    int32_t verbose;         /* 1 = print per-phase timings to stderr */
+} tmx_cfg;
+
+typedef struct {
+    double seconds_total;    /* wall time of the call, device work included */
+    double seconds_seed;     /* kNN bound seeding */
+    double seconds_main;     /* inner-product pass with fused candidate filter */
+    double seconds_verify;   /* exact reference-order recomputation of the candidates */
+    double seconds_select;   /* selection / merge */
+    int64_t comparisons;     /* sum over templates of (n - m + 1) */
+    int64_t candidates;      /* windows recomputed exactly */
+    int32_t retries;         /* candidate-buffer overflows resolved by a tighter bound or a bigger buffer */
+    float max_abs_rho_err;   /* max |rho_fast - rho_exact| over the verified candidates */
+    double kernel_ms_main;   /* device time of the main inner-product launch (HIP events on the context stream) */
+    double flops_main;       /* algorithmic flops of that launch: sum over templates of 2 m (n - m + 1) */
+} tmx_stats;
+
+typedef struct {
+    int64_t count;           /* rows */
+    int32_t *tmpl;           /* template index per row */
+    int64_t *counter;        /* running row number within its template (the CSV "Counter") */
+    uint64_t *pos;           /* window start */
+    float *d;                /* squared z-normalised distance (reference FP32) */
+} tmx_hits;
+
+int tmx_cfg_default(tmx_cfg *cfg);
+int tmx_create(tmx_ctx **out, const tmx_cfg *cfg);
+void tmx_destroy(tmx_ctx *ctx);
+const char *tmx_last_error(const tmx_ctx *ctx);
+int tmx_set_precision(tmx_ctx *ctx, int32_t precision);
+
+/* Copies the waveform to the device and builds its FP64 prefix sums of x and
+ * x^2 (missing samples as 0).  Replaces any waveform loaded before. */
+int tmx_load_waveform(tmx_ctx *ctx, const float *x, uint64_t n);
+uint64_t tmx_waveform_length(const tmx_ctx *ctx);
+
+/* kNN search.  templates: raw (not normalised) samples, template i at
+ * templates[offsets[i] .. offsets[i] + lengths[i]).  Output k slots per
+ * template, best first, ties by position; unfilled slots are (0, FLT_MAX).
+ * merge_offset > 0 keeps answers pairwise >= merge_offset apart. */
+int tmx_knn(tmx_ctx *ctx, const float *templates, const int32_t *lengths, const uint64_t *offsets, int32_t count,
+            int32_t k, int32_t merge_offset, uint64_t *out_pos, float *out_d, tmx_stats *stats);
+
+/* Threshold search: every window with d <= 2m(1 - rho), per template sorted
+ * by position; with merge_offset > 0, a hit closer than merge_offset + 1 to
+ * the last kept hit is dropped.  *out is library-owned: tmx_hits_free. */
+int tmx_threshold(tmx_ctx *ctx, const float *templates, const int32_t *lengths, const uint64_t *offsets,
+                  int32_t count, float rho, int32_t merge_offset, tmx_hits **out, tmx_stats *stats);
+void tmx_hits_free(tmx_hits *hits);
+
+/* Exact reference distances (no early abandoning) of one raw template of
+ * length m against the windows starting at starts[0..count); a window whose
+ * first sample is missing or whose std is <= 1e-12 gets FLT_MAX. */
+int tmx_distance_batch(tmx_ctx *ctx, const float *tmpl, int32_t m, const uint64_t *starts, int32_t count,
+                       float *out_d);
+
+/* Exact reference window statistics (two-pass FP64, stored as float) of the
+ * windows [first, first + count) for length m. */
+int tmx_window_stats(tmx_ctx *ctx, int32_t m, uint64_t first, uint64_t count, float *mean, float *std);
+
+/* Z-normalised template and block order exactly as the reference prepares
+ * them, computed on the device.  order needs m / 32 slots. */
+int tmx_prepare_template(tmx_ctx *ctx, const float *tmpl, int32_t m, float *out_z, int32_t *out_order);
+
+/* Waits for all device work of the context. */
+int tmx_sync(tmx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,59 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every entry point
+declared in include/tremor_amd/tmx.h, and rejects use without a device.
+No compute call is made here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tremor_amd", "tmx.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(tmx_[a-z_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from tremor_amd import capi
+    if not os.path.exists(capi.LIB_PATH):
+        from tremor_amd import build
+        build.build()
+    return ctypes.CDLL(capi.LIB_PATH)
+
+
+def test_exports_every_declared_symbol(lib):
+    names = _declared()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in tmx.h but not exported"
+
+
+def test_binding_covers_header():
+    from tremor_amd import capi
+    assert sorted(capi.EXPORTS) == _declared()
+
+
+def test_cfg_default_and_null_handling(lib):
+    from tremor_amd import capi
+    L = capi.lib()
+    cfg = capi.Cfg()
+    assert L.tmx_cfg_default(ctypes.byref(cfg)) == 0
+    assert cfg.precision == capi.PREC_BF16X3 and cfg.device == 0
+    assert L.tmx_last_error(None) == b"null context"
+    assert L.tmx_sync(None) != 0
+    assert L.tmx_waveform_length(None) == 0
+
+
+def test_invalid_precision_rejected_before_device(lib):
+    from tremor_amd import capi
+    L = capi.lib()
+    cfg = capi.Cfg()
+    L.tmx_cfg_default(ctypes.byref(cfg))
+    cfg.precision = 7
+    h = ctypes.c_void_p()
+    assert L.tmx_create(ctypes.byref(h), ctypes.byref(cfg)) != 0
+    assert not h.value

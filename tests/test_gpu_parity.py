@@ -1,0 +1,188 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+Bar: kNN positions and distances bit-identical to the oracle's restatement
+of the reference sequential scan (SkipSequential.cpp:296-393); threshold
+rows (template, counter, position, distance bits) identical to
+SkipSequential.cpp:223-294; exact statistics, template preparation and
+distances bit-identical.  Every precision form of the candidate pass is
+held to the same bar, because the verification pass recomputes every
+candidate in the reference's arithmetic.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["fp32", "bf16x3", "fp16"]
+
+
+@pytest.fixture(scope="module")
+def ctxs():
+    from tremor_amd.capi import Context
+    out = {p: Context(precision=p) for p in PRECS}
+    yield out
+    for c in out.values():
+        c.close()
+
+
+def _wave(n, seed, events=20):
+    from tremor_amd import synth
+    rng = np.random.default_rng(seed)
+    x = synth.bandpassed_noise(n, rng)
+    onsets, _ = synth.inject_events(x, events, rng)
+    return x, onsets, rng
+
+
+def _log(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def test_prepare_template_bit_exact(ctxs, oracle):
+    ctx = ctxs["fp32"]
+    rng = np.random.default_rng(1)
+    ctx.load_waveform(rng.standard_normal(4096).astype(np.float32))
+    for m in (1, 7, 31, 32, 33, 100, 1000, 4097, 65536):
+        _log("prepare", m=m)
+        t = (rng.standard_normal(m) * 3 + 7).astype(np.float32)
+        z, order = ctx.prepare_template(t)
+        zo = oracle.znormalize(t)
+        assert np.array_equal(z.view(np.uint32), zo.view(np.uint32)), m
+        assert np.array_equal(order, oracle.block_order(zo)), m
+
+
+def test_window_stats_and_distances_bit_exact(ctxs, oracle):
+    from tremor_amd import synth
+    ctx = ctxs["fp32"]
+    w = synth.validate_recipe()
+    x = w.x.copy()
+    x[100] = synth.MISSING
+    x[3000] = synth.MISSING
+    ctx.load_waveform(x)
+    for m in (64, 33, 500):
+        _log("stats", m=m)
+        mean, sd = ctx.window_stats(m, 0, x.size - m + 1)
+        mo, so = oracle.window_stats(x, m)
+        assert np.array_equal(mean.view(np.uint32), mo.view(np.uint32))
+        assert np.array_equal(sd.view(np.uint32), so.view(np.uint32))
+        t = w.templates[1][:m] if m <= 64 else x[1000:1000 + m]
+        starts = np.arange(0, x.size - m + 1, 7, dtype=np.uint64)
+        d = ctx.distance_batch(t, starts)
+        z = oracle.znormalize(t)
+        full = oracle.dist_all(z, x, mo, so)
+        assert np.array_equal(d.view(np.uint32), full[starts.astype(np.int64)].view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_validate_recipe(ctxs, oracle, prec):
+    from tremor_amd import synth
+    ctx = ctxs[prec]
+    w = synth.validate_recipe()
+    ctx.load_waveform(w.x)
+    for merge in (0, 16):
+        _log("validate-knn", prec=prec, merge=merge)
+        pos, d, st = ctx.knn(w.templates, 5, merge)
+        po, do, _ = oracle.scan_knn(w.x, w.templates, 5, merge)
+        assert np.array_equal(pos, po)
+        assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+        _log("validate-thr", prec=prec, merge=merge)
+        r = ctx.threshold(w.templates, 0.4, merge)
+        tt, tp, td = oracle.scan_threshold(w.x, w.templates, 0.4, merge)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_knn_parity_c0_shape(ctxs, oracle, prec):
+    ctx = ctxs[prec]
+    x, onsets, rng = _wave(200_000, 5, events=12)
+    from tremor_amd import synth
+    tm = synth.cut_templates(x, onsets, [256] * 6, rng)
+    ctx.load_waveform(x)
+    for k, merge in ((10, 0), (10, 64), (1, 0)):
+        _log("knn", prec=prec, k=k, merge=merge)
+        pos, d, st = ctx.knn(tm, k, merge)
+        print("  stats", st, flush=True)
+        po, do, _ = oracle.scan_knn(x, tm, k, merge)
+        assert np.array_equal(pos, po)
+        assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_threshold_parity_c0_shape(ctxs, oracle, prec):
+    ctx = ctxs[prec]
+    x, onsets, rng = _wave(200_000, 6, events=15)
+    from tremor_amd import synth
+    tm = synth.cut_templates(x, onsets, [256] * 5, rng)
+    ctx.load_waveform(x)
+    for rho, merge in ((0.8, 0), (0.5, 0), (0.6, 100)):
+        _log("threshold", prec=prec, rho=rho, merge=merge)
+        r = ctx.threshold(tm, rho, merge)
+        print("  stats", r.stats, "rows", r.pos.size, flush=True)
+        tt, tp, td = oracle.scan_threshold(x, tm, rho, merge)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+        # per-template running counter as in the CSV
+        for t in np.unique(tt):
+            assert np.array_equal(r.counter[r.tmpl == t], np.arange((tt == t).sum()))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_uneven_lengths_and_missing(ctxs, oracle, prec):
+    ctx = ctxs[prec]
+    x, onsets, rng = _wave(150_000, 7, events=10)
+    from tremor_amd import synth
+    lengths = [33, 200, 511, 512, 1000, 1500, 2047, 2100]
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    x[rng.integers(0, x.size, size=40)] = synth.MISSING
+    x[int(onsets[0])] = synth.MISSING
+    ctx.load_waveform(x)
+    _log("uneven-knn", prec=prec)
+    pos, d, st = ctx.knn(tm, 7, 0)
+    print("  stats", st, flush=True)
+    po, do, _ = oracle.scan_knn(x, tm, 7, 0)
+    assert np.array_equal(pos, po)
+    assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+    _log("uneven-thr", prec=prec)
+    r = ctx.threshold(tm, 0.7, 10)
+    tt, tp, td = oracle.scan_threshold(x, tm, 0.7, 10)
+    assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+    assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_edges(ctxs, oracle, prec):
+    ctx = ctxs[prec]
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(300).astype(np.float32)
+    x[50:120] = 1.0  # constant stretch: windows excluded (FLT_MAX)
+    ctx.load_waveform(x)
+    tm = [x[0:64].copy(), x[100:300].copy(), rng.standard_normal(300).astype(np.float32),
+          rng.standard_normal(301).astype(np.float32)]
+    _log("edges-knn", prec=prec)
+    pos, d, _ = ctx.knn(tm, 150, 0)
+    po, do, _ = oracle.scan_knn(x, tm, 150, 0)
+    assert np.array_equal(pos, po)
+    assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+    pos, d, _ = ctx.knn(tm, 40, 9)
+    po, do, _ = oracle.scan_knn(x, tm, 40, 9)
+    assert np.array_equal(pos, po)
+    assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+    _log("edges-thr", prec=prec)
+    r = ctx.threshold(tm, -0.5, 0)
+    tt, tp, td = oracle.scan_threshold(x, tm, -0.5, 0)
+    assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+    assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_rho_error_reported_and_small(ctxs):
+    x, onsets, rng = _wave(100_000, 10)
+    from tremor_amd import synth
+    tm = synth.cut_templates(x, onsets, [300, 700], rng)
+    for prec, lim in (("fp32", 1e-4), ("bf16x3", 1e-3), ("fp16", 2e-2)):
+        ctx = ctxs[prec]
+        ctx.load_waveform(x)
+        _log("rho-err", prec=prec)
+        _, _, st = ctx.knn(tm, 5, 0)
+        print("  stats", st, flush=True)
+        assert 0.0 <= st["max_abs_rho_err"] < lim
+        assert st["candidates"] >= 5

@@ -1,0 +1,225 @@
+"""ctypes binding of the C-ABI in include/tremor_amd/tmx.h (libtremor_amd.so).
+
+The library is loaded from this package directory; if it is missing the
+import of this module raises: there is no fallback to any other code path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtremor_amd.so")
+
+PREC_FP32 = 0
+PREC_BF16X3 = 1
+PREC_FP16 = 2
+PRECISIONS = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp16": PREC_FP16}
+
+EXPORTS = [
+    "tmx_cfg_default", "tmx_create", "tmx_destroy", "tmx_last_error", "tmx_set_precision", "tmx_load_waveform",
+    "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch", "tmx_window_stats",
+    "tmx_prepare_template", "tmx_sync",
+]
+
+
+class Cfg(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32), ("cand_capacity", ctypes.c_int64),
+                ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("seconds_total", ctypes.c_double), ("seconds_seed", ctypes.c_double),
+                ("seconds_main", ctypes.c_double), ("seconds_verify", ctypes.c_double),
+                ("seconds_select", ctypes.c_double), ("comparisons", ctypes.c_int64),
+                ("candidates", ctypes.c_int64), ("retries", ctypes.c_int32), ("max_abs_rho_err", ctypes.c_float),
+                ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Hits(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_int64), ("tmpl", ctypes.POINTER(ctypes.c_int32)),
+                ("counter", ctypes.POINTER(ctypes.c_int64)), ("pos", ctypes.POINTER(ctypes.c_uint64)),
+                ("d", ctypes.POINTER(ctypes.c_float))]
+
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run `python -m tremor_amd.build` (no fallback path exists)")
+    L = ctypes.CDLL(path)
+    L.tmx_cfg_default.argtypes = [ctypes.POINTER(Cfg)]
+    L.tmx_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Cfg)]
+    L.tmx_destroy.argtypes = [ctypes.c_void_p]
+    L.tmx_destroy.restype = None
+    L.tmx_last_error.argtypes = [ctypes.c_void_p]
+    L.tmx_last_error.restype = ctypes.c_char_p
+    L.tmx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.tmx_load_waveform.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint64]
+    L.tmx_waveform_length.argtypes = [ctypes.c_void_p]
+    L.tmx_waveform_length.restype = ctypes.c_uint64
+    L.tmx_knn.argtypes = [ctypes.c_void_p, _f32p, _i32p, _u64p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                          _u64p, _f32p, ctypes.POINTER(Stats)]
+    L.tmx_threshold.argtypes = [ctypes.c_void_p, _f32p, _i32p, _u64p, ctypes.c_int32, ctypes.c_float,
+                                ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(Hits)), ctypes.POINTER(Stats)]
+    L.tmx_hits_free.argtypes = [ctypes.POINTER(Hits)]
+    L.tmx_hits_free.restype = None
+    L.tmx_distance_batch.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int32, _u64p, ctypes.c_int32, _f32p]
+    L.tmx_window_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64, _f32p, _f32p]
+    L.tmx_prepare_template.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int32, _f32p, _i32p]
+    L.tmx_sync.argtypes = [ctypes.c_void_p]
+    return L
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+class TmxError(RuntimeError):
+    pass
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def pack_templates(templates):
+    lengths = np.array([len(t) for t in templates], dtype=np.int32)
+    offsets = np.zeros(len(templates), dtype=np.uint64)
+    if len(templates) > 1:
+        offsets[1:] = np.cumsum(lengths[:-1].astype(np.uint64))
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float32) for t in templates]))
+    return flat, lengths, offsets
+
+
+@dataclass
+class ThresholdResult:
+    tmpl: np.ndarray
+    counter: np.ndarray
+    pos: np.ndarray
+    d: np.ndarray
+    stats: dict
+
+
+class Context:
+    """One device context: a resident waveform and search calls on it."""
+
+    def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
+                 seed_target: int = 0, verbose: bool = False):
+        L = lib()
+        cfg = Cfg()
+        L.tmx_cfg_default(ctypes.byref(cfg))
+        cfg.device = device
+        cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        cfg.cand_capacity = cand_capacity
+        cfg.seed_target = seed_target
+        cfg.verbose = 1 if verbose else 0
+        h = ctypes.c_void_p()
+        rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))
+        if rc != 0:
+            raise TmxError(f"tmx_create failed ({rc}): no usable HIP device {device}")
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if self._h:
+            self._L.tmx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise TmxError(self._L.tmx_last_error(self._h).decode())
+
+    def set_precision(self, precision):
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._check(self._L.tmx_set_precision(self._h, p))
+
+    def load_waveform(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        self._check(self._L.tmx_load_waveform(self._h, _p(x, _f32p), x.size))
+
+    @property
+    def n(self) -> int:
+        return int(self._L.tmx_waveform_length(self._h))
+
+    def knn(self, templates, k: int, merge: int = 0):
+        flat, lengths, offsets = pack_templates(templates)
+        T = len(templates)
+        pos = np.zeros((T, k), dtype=np.uint64)
+        d = np.zeros((T, k), dtype=np.float32)
+        st = Stats()
+        self._check(self._L.tmx_knn(self._h, _p(flat, _f32p), _p(lengths, _i32p), _p(offsets, _u64p), T, k, merge,
+                                    _p(pos, _u64p), _p(d, _f32p), ctypes.byref(st)))
+        return pos, d, st.as_dict()
+
+    def threshold(self, templates, rho: float, merge: int = 0) -> ThresholdResult:
+        flat, lengths, offsets = pack_templates(templates)
+        T = len(templates)
+        out = ctypes.POINTER(Hits)()
+        st = Stats()
+        self._check(self._L.tmx_threshold(self._h, _p(flat, _f32p), _p(lengths, _i32p), _p(offsets, _u64p), T,
+                                          float(rho), merge, ctypes.byref(out), ctypes.byref(st)))
+        try:
+            h = out.contents
+            n = int(h.count)
+            if n:
+                res = ThresholdResult(np.ctypeslib.as_array(h.tmpl, (n,)).copy(),
+                                      np.ctypeslib.as_array(h.counter, (n,)).copy(),
+                                      np.ctypeslib.as_array(h.pos, (n,)).copy(),
+                                      np.ctypeslib.as_array(h.d, (n,)).copy(), st.as_dict())
+            else:
+                res = ThresholdResult(np.zeros(0, np.int32), np.zeros(0, np.int64), np.zeros(0, np.uint64),
+                                      np.zeros(0, np.float32), st.as_dict())
+        finally:
+            self._L.tmx_hits_free(out)
+        return res
+
+    def distance_batch(self, template, starts):
+        t = np.ascontiguousarray(template, dtype=np.float32)
+        s = np.ascontiguousarray(starts, dtype=np.uint64)
+        out = np.zeros(s.size, dtype=np.float32)
+        self._check(self._L.tmx_distance_batch(self._h, _p(t, _f32p), t.size, _p(s, _u64p), s.size,
+                                               _p(out, _f32p)))
+        return out
+
+    def window_stats(self, m: int, first: int, count: int):
+        mean = np.zeros(count, dtype=np.float32)
+        sd = np.zeros(count, dtype=np.float32)
+        self._check(self._L.tmx_window_stats(self._h, m, first, count, _p(mean, _f32p), _p(sd, _f32p)))
+        return mean, sd
+
+    def prepare_template(self, template):
+        t = np.ascontiguousarray(template, dtype=np.float32)
+        z = np.zeros(t.size, dtype=np.float32)
+        order = np.zeros(max(t.size // 32, 1), dtype=np.int32)
+        self._check(self._L.tmx_prepare_template(self._h, _p(t, _f32p), t.size, _p(z, _f32p), _p(order, _i32p)))
+        return z, order[: t.size // 32]
+
+    def sync(self):
+        self._check(self._L.tmx_sync(self._h))

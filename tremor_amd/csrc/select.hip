@@ -1,0 +1,257 @@
+// Grouping of candidate windows by template, kNN bound seeding and the
+// canonical kNN selection on the device.
+//
+//   seed bound     the k-th (merge 0) or (2k-1)-th pairwise-separated (merge
+//                  > 0) smallest distance upper bound among sampled windows:
+//                  the role of the reference's bound set pqueue_bsf
+//                  (iSAXPqueue.cpp:471-549, bound size SkipSequential.cpp:299)
+//   kNN select     select_knn_with_exclusion (ZnormSearch.hpp:170-189): sort by
+//                  (d, pos), keep a window unless one kept lies closer than
+//                  merge_offset, stop at k, fill (0, FLT_MAX)
+#include <hipcub/hipcub.hpp>
+
+#include "tmx_internal.hpp"
+
+namespace tmx {
+
+// Predicates of the grouping pass.
+enum GroupPred : int {
+    kPredAll = 0,        // every entry
+    kPredHit = 1,        // exact distance <= the template's bound (threshold hits)
+    kPredFinite = 2,     // value < +inf (seed entries)
+};
+
+// Sort keys: (value bits << 32) | window, or (window << 32) | value bits.
+enum GroupKey : int {
+    kKeyValuePos = 0,
+    kKeyPosValue = 1,
+};
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline bool pred_ok(int pred, const TmplDev *tmpl, uint32_t t, float v)
+{
+    if (pred == kPredHit)
+        return v <= tmpl[t].bound;
+    if (pred == kPredFinite)
+        return v < INFINITY;
+    return true;
+}
+
+__global__ void k_group_count(const Cand *__restrict__ cand, const float *__restrict__ val, uint64_t count,
+                              const TmplDev *__restrict__ tmpl, int pred, uint32_t *__restrict__ counts)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= count)
+        return;
+    const Cand c = cand[i];
+    if (pred_ok(pred, tmpl, c.t, val[i]))
+        atomicAdd(&counts[c.t], 1u);
+}
+
+// Exclusive scan of T counts into offsets[0..T], one workgroup; also resets
+// the scatter cursors.
+__global__ __launch_bounds__(1024) void k_group_offsets(const uint32_t *__restrict__ counts, int T,
+                                                        uint32_t *__restrict__ offsets, uint32_t *__restrict__ cursor)
+{
+    __shared__ uint32_t part[1024];
+    const int per = (T + 1023) / 1024;
+    const int lo = per * threadIdx.x;
+    const int hi = lo + per < T ? lo + per : T;
+    uint32_t s = 0;
+    for (int t = lo; t < hi; t++)
+        s += counts[t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0u;
+    for (int t = lo; t < hi; t++) {
+        offsets[t] = run;
+        cursor[t] = 0u;
+        run += counts[t];
+    }
+    if (threadIdx.x == 1023)
+        offsets[T] = part[1023];
+}
+
+__global__ void k_group_scatter(const Cand *__restrict__ cand, const float *__restrict__ val, uint64_t count,
+                                const TmplDev *__restrict__ tmpl, int pred, int keykind,
+                                const uint32_t *__restrict__ offsets, uint32_t *__restrict__ cursor,
+                                uint64_t *__restrict__ keys)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= count)
+        return;
+    const Cand c = cand[i];
+    const float v = val[i];
+    if (!pred_ok(pred, tmpl, c.t, v))
+        return;
+    const uint32_t slot = offsets[c.t] + atomicAdd(&cursor[c.t], 1u);
+    const uint64_t vb = static_cast<uint64_t>(__float_as_uint(v));
+    keys[slot] = keykind == kKeyValuePos ? ((vb << 32) | c.j) : ((static_cast<uint64_t>(c.j) << 32) | vb);
+}
+
+__device__ inline float key_val(uint64_t k) { return __uint_as_float(static_cast<uint32_t>(k >> 32)); }
+__device__ inline uint32_t key_pos(uint64_t k) { return static_cast<uint32_t>(k); }
+
+// Greedy pass over (value, pos)-sorted keys of one template by one wavefront:
+// keep an entry unless a kept one lies closer than `merge` (strictly), until
+// `want` are kept.  Kept entries go to LDS.  Returns the number kept.
+__device__ int greedy_separated(const uint64_t *keys, uint32_t cnt, int want, int merge, uint64_t *kept)
+{
+    const int lane = threadIdx.x & 63;
+    int nk = 0;
+    for (uint32_t base = 0; base < cnt && nk < want; base += 64) {
+        // 64 keys at a time in registers, broadcast one by one.
+        const uint64_t mine = base + lane < cnt ? keys[base + lane] : ~0ull;
+        const int span = cnt - base < 64u ? static_cast<int>(cnt - base) : 64;
+        for (int l = 0; l < span && nk < want; l++) {
+            const uint64_t k = __shfl(mine, l);
+            const uint32_t p = key_pos(k);
+            bool blocked = false;
+            for (int kb = 0; kb < nk; kb += 64) {
+                const int i = kb + lane;
+                bool near = false;
+                if (i < nk) {
+                    const uint32_t q = key_pos(kept[i]);
+                    const uint32_t gap = q > p ? q - p : p - q;
+                    near = gap < static_cast<uint32_t>(merge);
+                }
+                if (__any(near)) {
+                    blocked = true;
+                    break;
+                }
+            }
+            if (!blocked) {
+                if (lane == 0)
+                    kept[nk] = k;
+                nk++;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+    }
+    return nk;
+}
+
+// Seed bound per template: one wavefront per template.
+__global__ __launch_bounds__(64) void k_seed_bound(const uint64_t *__restrict__ keys,
+                                                   const uint32_t *__restrict__ offsets, TmplDev *tmpl, int want,
+                                                   int merge)
+{
+    extern __shared__ uint64_t kept[];
+    const int t = blockIdx.x;
+    const uint32_t b = offsets[t], e = offsets[t + 1];
+    float U = FLT_MAX;
+    if (merge == 0) {
+        if (e - b >= static_cast<uint32_t>(want))
+            U = key_val(keys[b + want - 1]);
+    } else {
+        const int nk = greedy_separated(keys + b, e - b, want, merge, kept);
+        __syncthreads();
+        if (nk >= want)
+            U = key_val(kept[want - 1]);
+    }
+    if (threadIdx.x == 0)
+        tmpl[t].bound = U < FLT_MAX ? U : FLT_MAX;
+}
+
+// Canonical kNN answer per template from (d, pos)-sorted exact candidates.
+__global__ __launch_bounds__(64) void k_knn_select(const uint64_t *__restrict__ keys,
+                                                   const uint32_t *__restrict__ offsets, int k, int merge,
+                                                   uint64_t *__restrict__ out_pos, float *__restrict__ out_d)
+{
+    extern __shared__ uint64_t kept[];
+    const int t = blockIdx.x;
+    const uint32_t b = offsets[t], e = offsets[t + 1];
+    int nk;
+    if (merge == 0) {
+        nk = static_cast<int>(e - b) < k ? static_cast<int>(e - b) : k;
+        for (int i = threadIdx.x; i < nk; i += 64)
+            kept[i] = keys[b + i];
+    } else {
+        nk = greedy_separated(keys + b, e - b, k, merge, kept);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += 64) {
+        const size_t o = static_cast<size_t>(t) * k + i;
+        if (i < nk) {
+            out_pos[o] = key_pos(kept[i]);
+            out_d[o] = key_val(kept[i]);
+        } else {
+            out_pos[o] = 0;
+            out_d[o] = FLT_MAX;
+        }
+    }
+}
+
+}  // namespace
+
+// Groups the entries satisfying `pred` by template into `keys` (sorted within
+// each template).  offsets has T + 1 slots.  Returns the number grouped via
+// offsets[T] on the device.
+hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count, const TmplDev *tmpl, int T,
+                             int pred, int keykind, uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
+                             uint64_t *keys_tmp, uint64_t *keys, void *sort_tmp, size_t sort_tmp_bytes,
+                             uint64_t key_capacity, hipStream_t stream)
+{
+    hipError_t err = hipMemsetAsync(counts, 0, sizeof(uint32_t) * static_cast<size_t>(T), stream);
+    if (err != hipSuccess)
+        return err;
+    const unsigned grid = static_cast<unsigned>((count + kThreads - 1) / kThreads);
+    if (count > 0)
+        hipLaunchKernelGGL(k_group_count, dim3(grid), dim3(kThreads), 0, stream, cand, val, count, tmpl, pred, counts);
+    hipLaunchKernelGGL(k_group_offsets, dim3(1), dim3(1024), 0, stream, counts, T, offsets, cursor);
+    if (count > 0)
+        hipLaunchKernelGGL(k_group_scatter, dim3(grid), dim3(kThreads), 0, stream, cand, val, count, tmpl, pred,
+                           keykind, offsets, cursor, keys_tmp);
+    err = hipGetLastError();
+    if (err != hipSuccess)
+        return err;
+    if (count == 0)
+        return hipSuccess;
+    size_t bytes = sort_tmp_bytes;
+    return hipcub::DeviceSegmentedRadixSort::SortKeys(sort_tmp, bytes, keys_tmp, keys, static_cast<int>(count),
+                                                     T, offsets, offsets + 1, 0, 64, stream);
+    (void)key_capacity;
+}
+
+size_t group_sort_tmp_bytes(uint64_t count, int T)
+{
+    size_t bytes = 0;
+    (void)hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, bytes, static_cast<const uint64_t *>(nullptr),
+                                               static_cast<uint64_t *>(nullptr), static_cast<int>(count), T,
+                                               static_cast<const uint32_t *>(nullptr),
+                                               static_cast<const uint32_t *>(nullptr), 0, 64);
+    return bytes;
+}
+
+hipError_t launch_seed_bound(const uint64_t *keys, const uint32_t *offsets, TmplDev *tmpl, int T, int want,
+                             int merge, hipStream_t stream)
+{
+    if (T <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_seed_bound, dim3(T), dim3(64), sizeof(uint64_t) * static_cast<size_t>(want), stream, keys,
+                       offsets, tmpl, want, merge);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_select(const uint64_t *keys, const uint32_t *offsets, int T, int k, int merge,
+                             uint64_t *out_pos, float *out_d, hipStream_t stream)
+{
+    if (T <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_knn_select, dim3(T), dim3(64), sizeof(uint64_t) * static_cast<size_t>(k), stream, keys,
+                       offsets, k, merge, out_pos, out_d);
+    return hipGetLastError();
+}
+
+}  // namespace tmx

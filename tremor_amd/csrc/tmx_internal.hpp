@@ -1,0 +1,91 @@
+// Internal declarations shared by the C-ABI layer (tmx.cpp) and the device
+// code (stats.hip, prepare.hip, scan.hip, mfma.hip, verify.hip, select.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tremor_amd/tmx.h"
+
+#include <cfloat>
+#include <cstdint>
+
+namespace tmx {
+
+// Missing-sample sentinel (ZnormSearch.hpp:21).
+constexpr float kMissing = -100000.0f;
+// Early-abandon block of the reference kernel (ZnormSearch.hpp:65).
+constexpr int kBlock = 32;
+
+// Per-template record on the device.  The z-normalised samples live in one
+// packed float array (q_off), the block order in one packed int array
+// (order_off).  Q2 = sum q^2 and S = sum q in FP64 feed the candidate filter.
+struct TmplDev {
+    int64_t q_off;      // offset into the packed z-normalised samples
+    int64_t order_off;  // offset into the packed block orders
+    int64_t raw_off;    // offset into the packed raw samples of this call
+    int32_t m;          // length
+    int32_t kpad;       // length padded for the MFMA operand layout (0 = none)
+    int64_t a_off;      // offset (in 16-byte fragments) of the MFMA operand blocks
+    double Q2;          // sum of q^2
+    double S;           // sum of q
+    float bound;        // candidate cut: threshold distance, or the kNN seed bound
+    int32_t pad0;
+};
+
+// One candidate window of one template.
+struct Cand {
+    uint32_t t;  // template index within the call
+    uint32_t j;  // window start
+};
+
+// Filter mode of the inner-product passes.
+enum FilterMode : int {
+    kModeMain = 0,  // emit every window whose distance lower bound <= bound
+    kModeSeed = 1,  // emit per-thread minima of the distance upper bound
+};
+
+// Relative error bound of the FP64 prefix sums: every prefix value is within
+// kPrefixGamma * (sum of |terms|) of the exact sum (a few thousand sequential
+// and tree additions at most, each 2^-53).
+constexpr double kPrefixGamma = 8192.0 * 1.1102230246251565e-16;
+
+struct Waveform {
+    const float *x;     // raw samples, n + tail padding (missing kept)
+    const double *p1;   // prefix sums of x0 (missing as 0), n + 1 values
+    const double *p2;   // prefix sums of x0^2, n + 1 values
+    uint64_t n;
+};
+
+// ---- launchers (all asynchronous on `stream`) ----------------------------
+
+// FP64 inclusive prefix sums P1, P2 of x0 and x0^2 (P[0] = 0).
+hipError_t launch_prefix(const float *x, uint64_t n, double *p1, double *p2, double *work, size_t work_len,
+                         hipStream_t stream);
+size_t prefix_work_len(uint64_t n);
+
+// Reference template preparation: z-norm (ZnormSearch.hpp:33-62), block
+// order (:72-90), plus Q2 and S.  One workgroup per template.
+hipError_t launch_prepare(const float *raw, TmplDev *tmpl, int count, float *qz, int32_t *order,
+                          hipStream_t stream);
+
+// Plain FP32 sliding-dot pass with the fused candidate filter.
+hipError_t launch_scan_fp32(const Waveform &w, const TmplDev *tmpl, const float *qz, int count,
+                            const int32_t *group_begin, int groups, int mode, int block_stride, Cand *cand,
+                            float *cand_val, unsigned long long *cand_count, uint64_t cand_cap,
+                            hipStream_t stream);
+constexpr int kScanWindows = 4096;  // windows per workgroup of the FP32 scan
+
+// Exact recomputation of candidates in the reference's order.
+hipError_t launch_verify(const float *x, uint64_t n, const TmplDev *tmpl, const float *qz, const int32_t *order,
+                         const Cand *cand, uint64_t count, float *d_out, hipStream_t stream);
+
+// Exact reference stats of windows [first, first + count) for length m.
+hipError_t launch_window_stats(const float *x, int m, uint64_t first, uint64_t count, float *mean, float *sd,
+                               hipStream_t stream);
+
+// Exact distances of one prepared template against the given windows.
+hipError_t launch_dist_batch(const float *x, uint64_t n, const TmplDev *tmpl, const float *qz,
+                             const int32_t *order, const uint64_t *starts, int count, float *out,
+                             hipStream_t stream);
+
+}  // namespace tmx
