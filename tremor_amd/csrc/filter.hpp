@@ -37,8 +37,9 @@ __device__ inline DistInterval dist_interval(const Waveform &w, uint64_t j, int 
     constexpr double u = 5.9604644775390625e-08;   // 2^-24
     constexpr double u64 = 1.1102230246251565e-16;  // 2^-53
     const double md = static_cast<double>(m);
-    const double p1a = w.p1[j], p1b = w.p1[j + m];
-    const double p2a = w.p2[j], p2b = w.p2[j + m];
+    const double2 pa = w.p[j], pb = w.p[j + m];
+    const double p1a = pa.x, p1b = pb.x;
+    const double p2a = pa.y, p2b = pb.y;
     const double S1 = p1b - p1a;
     const double S2 = p2b - p2a;
     // |P1[i] - exact| <= gamma * sum_{k<i} |x_k| <= gamma * sqrt(i * P2[i]).

@@ -33,7 +33,7 @@
 // lane evaluates its most promising window rigorously and the wavefront
 // emits the minimum upper bound.
 //
-// Workgroup: 4 wavefronts x 2 column tiles x 1024 windows = 8192 windows;
+// Workgroup: 4 wavefronts x 4 column tiles x 1024 windows = 16384 windows;
 // templates of a group (sorted by length) are processed two at a time.
 #include "filter.hpp"
 
@@ -47,18 +47,30 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kNM = 2;                        // templates per pass
-constexpr int kNN = 2;                        // 32-column tiles per wavefront
+constexpr int kNN = 4;                        // 32-column tiles per wavefront
 constexpr int kTileWin = 1024;                // windows per 32x32 output tile
 constexpr int kNW = kWaves * kNN * kTileWin;  // windows per workgroup
-constexpr int kKC = 1024;                     // K samples staged per chunk
+constexpr int kKC = 2048;                     // K samples staged per chunk
 constexpr int kGroup = 8;                     // templates per workgroup
 constexpr int kStage = kNW + kKC;             // staged samples per chunk
 
-// LDS swizzle in 16-bit elements: 8 elements of padding after every 32, so
-// the 16-byte reads of 8 consecutive lanes (columns 64 bytes apart) hit
-// distinct banks.  8-element groups never straddle a 32-element row.
-__device__ inline int lds_at(int e) { return e + ((e >> 5) << 3); }
-constexpr int kLdsLen = kStage + (kStage >> 2);
+// LDS layout of the staged tile (16-bit elements): rows of 32 elements (one
+// column of the Hankel view), 8-element chunks permuted within a row by
+// (row / 2) mod 4.  The 16-byte reads of 8 consecutive lanes (consecutive
+// rows) then land in 8 distinct 16-byte bank groups, without padding.
+__device__ inline int swz(int e) { return e ^ (((e >> 6) & 3) << 3); }
+
+// Epilogue scratch, aliased over the tile once a template pair's K loop is
+// done: per wavefront the window statistics of one 1024-window tile (padded
+// rows of 33) and a queue of windows that passed the level-1 test, decided
+// by the rigorous FP64 filter once the accumulators are no longer live.
+constexpr int kStatLen = kTileWin + kTileWin / 32;
+constexpr int kQ = 384;  // queue entries per wavefront (>= kNM * 64 for the seed mode)
+constexpr int kTileBytesBf16 = 2 * kStage * 2;
+constexpr int kTileBytesF16 = kStage * 2;
+constexpr int kEpiBytes = kWaves * kStatLen * 8 + kWaves * kQ * 12 + kWaves * 16 * 64 * 4;
+static_assert(kQ >= kNM * 64, "seed entries need one queue slot per lane and template");
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 struct alignas(16) Frag {
     uint32_t w[4];
@@ -147,36 +159,51 @@ template <int PREC>
 __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const TmplDev *__restrict__ tmpl,
                                                            const int32_t *__restrict__ group_begin,
                                                            const Frag *__restrict__ afrag, int mode,
-                                                           int block_stride, Cand *__restrict__ cand,
+                                                           int block_stride, int groups, uint32_t nblocks,
+                                                           Cand *__restrict__ cand,
                                                            float *__restrict__ cand_val,
                                                            unsigned long long *__restrict__ cand_count, uint64_t cap)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t lhi[kLdsLen];
-    __shared__ __attribute__((aligned(16))) uint16_t llo[PREC == TMX_PREC_FP16 ? 8 : kLdsLen];
-    __shared__ float s_red[kThreads / 64];
-    __shared__ float s_dt[kWaves][16][64];  // dot products of a tile with filter survivors
+    constexpr int kLds = cmax(PREC == TMX_PREC_FP16 ? kTileBytesF16 : kTileBytesBf16, kEpiBytes);
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kLds];
+    __shared__ float s_red[kWaves];
+    uint16_t *const lhi = reinterpret_cast<uint16_t *>(lds);
+    uint16_t *const llo = lhi + kStage;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int col = lane & 31, h = lane >> 5;
-    const uint64_t blk = static_cast<uint64_t>(blockIdx.x) * static_cast<uint64_t>(block_stride);
+    float2 *const stats = reinterpret_cast<float2 *>(lds) + wave * kStatLen;
+    uint32_t *const qj = reinterpret_cast<uint32_t *>(lds + kWaves * kStatLen * 8) + wave * kQ;
+    float *const qdt = reinterpret_cast<float *>(lds + kWaves * kStatLen * 8 + kWaves * kQ * 4) + wave * kQ;
+    int32_t *const qi = reinterpret_cast<int32_t *>(lds + kWaves * kStatLen * 8 + kWaves * kQ * 8) + wave * kQ;
+    float *const sdt = reinterpret_cast<float *>(lds + kWaves * kStatLen * 8 + kWaves * kQ * 12) + wave * 16 * 64;
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
+    // on each XCD, consecutive workgroups take every template group of one
+    // waveform block, so the block's samples and prefix sums are shared in
+    // that XCD's L2 instead of being fetched once per group.
+    const uint32_t xcd = blockIdx.x & 7u, k8 = blockIdx.x >> 3;
+    const int g = static_cast<int>(k8 % static_cast<uint32_t>(groups));
+    const uint32_t bidx = (k8 / static_cast<uint32_t>(groups)) * 8u + xcd;
+    if (bidx >= nblocks)
+        return;
+    const uint64_t blk = static_cast<uint64_t>(bidx) * static_cast<uint64_t>(block_stride);
     const uint64_t J0 = blk * kNW;
     if (J0 >= w.n)
         return;
-    const int g = blockIdx.y;
     const int tb = group_begin[g], te = group_begin[g + 1];
     const uint64_t jend = J0 + kNW < w.n ? J0 + kNW : w.n;
-    const float c = static_cast<float>((w.p1[jend] - w.p1[J0]) / static_cast<double>(jend - J0));
+    const float c = static_cast<float>((w.p[jend].x - w.p[J0].x) / static_cast<double>(jend - J0));
 
     // Largest K of the group (templates sorted by length: the last one).
     const int kmax = tmpl[te - 1].kpad;
 
-    // FP16: power-of-two scale so the staged block fits FP16.
+    // FP16: power-of-two scale so the staged block fits FP16.  The waveform
+    // buffer carries zero padding past n, so tile reads need no guard.
     float scale = 1.0f, inv_scale = 1.0f;
     if (PREC == TMX_PREC_FP16) {
         float mx = 0.0f;
-        for (uint64_t e = tid; e < static_cast<uint64_t>(kNW + kmax); e += kThreads) {
-            const uint64_t p = J0 + e;
-            float v = p < w.n ? w.x[p] : 0.0f;
+        for (int e = tid; e < kNW + kmax; e += kThreads) {
+            float v = w.x[J0 + e];
             v = v == kMissing ? 0.0f : v;
             mx = fmaxf(mx, fabsf(v - c));
         }
@@ -199,12 +226,12 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
     for (int tp = tb; tp < te; tp += kNM) {
         const int nt = te - tp < kNM ? te - tp : kNM;
         int ks[kNM];
-        int64_t aoff[kNM];
+        const Frag *abase[kNM];
 #pragma unroll
         for (int i = 0; i < kNM; i++) {
             const int ti = tp + (i < nt ? i : 0);
             ks[i] = i < nt ? tmpl[ti].kpad / 16 : 0;
-            aoff[i] = tmpl[ti].a_off;
+            abase[i] = afrag + tmpl[ti].a_off + lane;
         }
         const int kt = (ks[0] > ks[kNM - 1] ? ks[0] : ks[kNM - 1]) * 16;
         f32x16 acc[kNM][kNN];
@@ -222,12 +249,13 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
             // Stage x~ = (x0 - c) [* 2^-e] for [J0 + k0, J0 + k0 + kNW + kc), 8 samples per item.
             for (int e8 = tid; e8 < (kNW + kc) / 8; e8 += kThreads) {
                 const int e = e8 * 8;
+                const float4 *src = reinterpret_cast<const float4 *>(w.x + J0 + k0 + e);
+                const float4 v0 = src[0], v1 = src[1];
+                const float xv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
                 uint16_t hv[8], lv[8];
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
-                    const uint64_t p = J0 + static_cast<uint64_t>(k0 + e + u);
-                    float v = p < w.n ? w.x[p] : 0.0f;
-                    v = v == kMissing ? 0.0f : v;
+                    const float v = xv[u] == kMissing ? 0.0f : xv[u];
                     const float xt = v - c;
                     if (PREC == TMX_PREC_FP16) {
                         hv[u] = f16_bits(xt * inv_scale);
@@ -240,35 +268,49 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
 #pragma unroll
                 for (int q = 0; q < 4; q++)
                     fh.w[q] = static_cast<uint32_t>(hv[2 * q]) | (static_cast<uint32_t>(hv[2 * q + 1]) << 16);
-                *reinterpret_cast<Frag *>(&lhi[lds_at(e)]) = fh;
+                *reinterpret_cast<Frag *>(&lhi[swz(e)]) = fh;
                 if (PREC != TMX_PREC_FP16) {
                     Frag fl;
 #pragma unroll
                     for (int q = 0; q < 4; q++)
                         fl.w[q] = static_cast<uint32_t>(lv[2 * q]) | (static_cast<uint32_t>(lv[2 * q + 1]) << 16);
-                    *reinterpret_cast<Frag *>(&llo[lds_at(e)]) = fl;
+                    *reinterpret_cast<Frag *>(&llo[swz(e)]) = fl;
                 }
             }
             __syncthreads();
+            // Template operands are prefetched one k-step ahead (registers), so
+            // the global-load latency overlaps the previous k-step's MFMAs.
+            Frag nh[kNM], nl[kNM];
+            auto load_a = [&](int kappa) {
+#pragma unroll
+                for (int i = 0; i < kNM; i++) {
+                    if (kappa < ks[i]) {
+                        const Frag *src = abase[i] + static_cast<int64_t>(kappa) * 128;
+                        nh[i] = src[0];
+                        if (PREC != TMX_PREC_FP16)
+                            nl[i] = src[64];
+                    } else {
+                        nh[i] = Frag{{0u, 0u, 0u, 0u}};
+                        nl[i] = Frag{{0u, 0u, 0u, 0u}};
+                    }
+                }
+            };
+            load_a(k0 >> 4);
             for (int kk = 0; kk < kc; kk += 16) {
                 const int kappa = (k0 + kk) >> 4;
                 Frag ah[kNM], al[kNM];
 #pragma unroll
                 for (int i = 0; i < kNM; i++) {
-                    if (kappa < ks[i]) {
-                        const Frag *src = afrag + aoff[i] + static_cast<int64_t>(kappa) * 128;
-                        ah[i] = src[lane];
-                        al[i] = src[64 + lane];
-                    } else {
-                        ah[i] = Frag{{0u, 0u, 0u, 0u}};
-                        al[i] = Frag{{0u, 0u, 0u, 0u}};
-                    }
+                    ah[i] = nh[i];
+                    al[i] = nl[i];
                 }
+                if (kk + 16 < kc)
+                    load_a(kappa + 1);
 #pragma unroll
                 for (int jn = 0; jn < kNN; jn++) {
                     const int cc = (wave * kNN + jn) * 32 + col;  // column within the block
-                    const int e = 32 * cc + kk + 8 * h;
-                    const Frag bh = *reinterpret_cast<const Frag *>(&lhi[lds_at(e)]);
+                    const int e = swz(32 * cc + kk + 8 * h);
+                    const Frag bh = *reinterpret_cast<const Frag *>(&lhi[e]);
                     if (PREC == TMX_PREC_FP16) {
                         f16x8 bv;
                         __builtin_memcpy(&bv, &bh, 16);
@@ -279,7 +321,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                             acc[i][jn] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc[i][jn], 0, 0, 0);
                         }
                     } else {
-                        const Frag bl = *reinterpret_cast<const Frag *>(&llo[lds_at(e)]);
+                        const Frag bl = *reinterpret_cast<const Frag *>(&llo[e]);
                         bf16x8 bvh, bvl;
                         __builtin_memcpy(&bvh, &bh, 16);
                         __builtin_memcpy(&bvl, &bl, 16);
@@ -297,7 +339,11 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
             }
         }
 
-        // ---- epilogue -------------------------------------------------------
+        // ---- epilogue (the tile area of LDS is free from here on) -------------
+        // Level 1, while the accumulators are live: FP32 only.  Windows that
+        // pass go to the wavefront's queue (or, if it is full, straight to
+        // the candidate list: extra candidates cost verification time only).
+        int qn = 0;  // wavefront-uniform queue length
 #pragma unroll
         for (int i = 0; i < kNM; i++) {
             if (i >= nt)
@@ -307,23 +353,21 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
             if (static_cast<uint64_t>(m) > w.n)
                 continue;
             const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
-            const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
-            const double bound = static_cast<double>(tmpl[t].bound);
-            const double eta = eta_mfma(PREC, tmpl[t].kpad);
+            const double Q2 = tmpl[t].Q2;
             const double md = static_cast<double>(m);
             const double inv_m = 1.0 / md;
+            const double eta = eta_mfma(PREC, tmpl[t].kpad);
             // FP16 floor terms: staged |x~| * 2^-ex below 2^-14 and |q| * 2^7
             // below 2^-14 may be flushed.  x side: sum |q| 2^-14 2^ex <=
             // 2^-14 2^ex sqrt(m Q2) (absolute); q side: sum 2^-21 |x~| <=
             // 2^-21 sqrt(m) sqrt(sum x~^2) (relative to the window).
             const double abs_unit = PREC == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;  // 2^-21
-            const double ea = eta + abs_unit * sqrt(md / Q2) * 1.01;
-            const double eref = eref_cap(m);
-            const float gmin = static_cast<float>((Q2 + md - bound - eref) * 0.5 * (1.0 - 1e-3));
+            const float gmin =
+                static_cast<float>((Q2 + md - static_cast<double>(tmpl[t].bound) - eref_cap(m)) * 0.5 * (1.0 - 1e-3));
             const float eta_sq = static_cast<float>(eta * sqrt(Q2) * (1.0 + 1e-3));
             const float sqrt_m = static_cast<float>(sqrt(md));
             const float rsqrt_m = static_cast<float>(1.0 / sqrt(md)) * (1.0f + 1e-6f);
-            const float absS = static_cast<float>(fabs(S) * (1.0 + 1e-3));
+            const float absS = static_cast<float>(fabs(tmpl[t].S) * (1.0 + 1e-3));
             const float eabs_q = static_cast<float>(abs_unit * sqrt(md) * 1.01);  // * sqrt(sum x~^2)
             const float eabs_x = PREC == TMX_PREC_FP16
                                      ? static_cast<float>(6.103515625e-05 * static_cast<double>(scale) * 128.0 *
@@ -334,90 +378,156 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
             float best_dt = 0.0f;
 #pragma unroll
             for (int jn = 0; jn < kNN; jn++) {
-                const uint64_t jcol = J0 + 32 * static_cast<uint64_t>((wave * kNN + jn) * 32 + col);
-                uint32_t surv = 0;
+                const uint64_t tile0 = J0 + 32ull * static_cast<uint64_t>((wave * kNN + jn) * 32);
+                // Window statistics of the tile, coalesced: (m sigma^2, mu) as
+                // floats; -1 marks windows the level-1 test cannot judge (stats
+                // swamped by prefix rounding, or |mu| / sigma > 1e3), NaN marks
+                // windows past the last one.
+                __syncthreads();
+#pragma unroll 1
+                for (int it0 = 0; it0 < 16; it0 += 4) {
+                    double2 pa[4], pb[4];
 #pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const uint64_t j = tile0 + static_cast<uint64_t>((it0 + u) * 64 + lane);
+                        const uint64_t jj = j < W ? j : 0;
+                        pa[u] = w.p[jj];
+                        pb[u] = w.p[jj + m];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int wl = (it0 + u) * 64 + lane;
+                        const uint64_t j = tile0 + static_cast<uint64_t>(wl);
+                        float2 sv = make_float2(NAN, 0.0f);
+                        if (j < W) {
+                            const double S1 = pb[u].x - pa[u].x, S2 = pb[u].y - pa[u].y;
+                            const double V = S2 - S1 * S1 * inv_m;
+                            const double eVc = kPrefixGamma * (pb[u].y + pa[u].y) * 4.0 + 1e-15 * fabs(S2);
+                            const double mu = S1 * inv_m;
+                            const bool judge = V > 1e3 * eVc && V > md * 1e-20 && !(mu * mu * md > 1e6 * V);
+                            sv = make_float2(judge ? static_cast<float>(V) : -1.0f, static_cast<float>(mu));
+                        }
+                        stats[wl + (wl >> 5)] = sv;
+                    }
+                }
+                __syncthreads();
+                // The tile's dot products go to LDS so that the test below
+                // runs as a loop (few live registers) and this tile's
+                // accumulators die here.
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    sdt[r * 64 + lane] = acc[i][jn][r] * scale;
+                __syncthreads();
+#pragma unroll 1
                 for (int r = 0; r < 16; r++) {
-                    const uint64_t j = jcol + static_cast<uint64_t>(8 * (r >> 2) + 4 * h + (r & 3));
-                    if (j >= W)
-                        continue;
-                    const float dt = acc[i][jn][r] * scale;
-                    const double p1a = w.p1[j], p1b = w.p1[j + m];
-                    const double p2a = w.p2[j], p2b = w.p2[j + m];
-                    const double S1 = p1b - p1a, S2 = p2b - p2a;
-                    const double V = S2 - S1 * S1 * inv_m;
-                    const double eVc = kPrefixGamma * (p2b + p2a) * 4.0 + 1e-15 * fabs(S2);
-                    const bool shaky = !(V > 1e3 * eVc) || !(V > md * 1e-20);
-                    const double mu = S1 * inv_m;
-                    // |mu| / sigma > 1e3: beyond the cap of the level-1 E_ref bound.
-                    const bool offset = mu * mu * md > 1e6 * V;
-                    const float Vf = static_cast<float>(V);
+                    const int wl = 32 * col + 8 * (r >> 2) + 4 * h + (r & 3);
+                    const float2 sv = stats[wl + (wl >> 5)];
+                    const float Vf = sv.x;
+                    const bool valid = Vf == Vf;  // not past the last window
+                    const float dt = sdt[r * 64 + lane];
                     if (mode == kModeMain) {
-                        bool survive = shaky || offset || gmin <= 0.0f;
-                        if (!survive) {
-                            const float D = static_cast<float>(mu) - c;
+                        bool survive = valid && (Vf < 0.0f || gmin <= 0.0f);
+                        if (valid && !survive) {
+                            const float D = sv.y - c;
                             const float sV = Vf * __frsqrt_rn(Vf);
                             const float rhs = gmin * sV * rsqrt_m;
                             const float b = eta_sq * (sV + sqrt_m * fabsf(D)) + fabsf(D) * absS +
                                             eabs_q * (sV + sqrt_m * fabsf(D)) + eabs_x;
                             survive = dt + b * 1.001f + 1e-30f >= rhs * 0.999f;
                         }
-                        surv |= survive ? (1u << r) : 0u;
-                    } else if (!shaky && !offset && Vf > 0.0f) {
+                        const uint64_t mask = __ballot(survive);
+                        if (mask != 0ull) {
+                            const uint32_t j = static_cast<uint32_t>(tile0 + static_cast<uint64_t>(wl));
+                            const int idx = qn + __popcll(mask & ((1ull << lane) - 1ull));
+                            if (survive) {
+                                if (idx < kQ) {
+                                    qj[idx] = j;
+                                    qdt[idx] = dt;
+                                    qi[idx] = i;
+                                } else if (w.x[j] != kMissing) {
+                                    // Queue full: a candidate without the rigorous test.
+                                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
+                                    if (slot < cap) {
+                                        cand[slot].t = static_cast<uint32_t>(t);
+                                        cand[slot].j = j;
+                                        cand_val[slot] = NAN;
+                                    }
+                                }
+                            }
+                            qn += __popcll(mask);
+                        }
+                    } else if (valid && Vf > 0.0f) {
                         const float gapprox = dt * __frsqrt_rn(Vf);
                         if (gapprox > best_g) {
                             best_g = gapprox;
-                            best_j = static_cast<uint32_t>(j);
+                            best_j = static_cast<uint32_t>(tile0 + static_cast<uint64_t>(wl));
                             best_dt = dt;
-                        }
-                    }
-                }
-                if (mode == kModeMain && __any(surv != 0u)) {
-                    // Rare: park the tile's dot products in LDS and walk the
-                    // survivors with one instance of the rigorous filter.
-#pragma unroll
-                    for (int r = 0; r < 16; r++)
-                        s_dt[wave][r][lane] = acc[i][jn][r] * scale;
-                    while (surv != 0u) {
-                        const int r = __builtin_ctz(surv);
-                        surv &= surv - 1u;
-                        const uint64_t j = jcol + static_cast<uint64_t>(8 * (r >> 2) + 4 * h + (r & 3));
-                        if (w.x[j] == kMissing)
-                            continue;
-                        const float dt = s_dt[wave][r][lane];
-                        DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(dt), ea);
-                        if (PREC == TMX_PREC_FP16 && iv.lo > -INFINITY) {
-                            const double V = (w.p2[j + m] - w.p2[j]) -
-                                             (w.p1[j + m] - w.p1[j]) * (w.p1[j + m] - w.p1[j]) * inv_m;
-                            const double sig = sqrt(fmax(V, 1e-300) * inv_m);
-                            iv.lo -= 2.0 * static_cast<double>(eabs_x) / (sig * (1.0 - 1e-3)) * 1.01;
-                        }
-                        if (iv.lo <= bound) {
-                            const unsigned long long slot = atomicAdd(cand_count, 1ULL);
-                            if (slot < cap) {
-                                cand[slot].t = static_cast<uint32_t>(t);
-                                cand[slot].j = static_cast<uint32_t>(j);
-                                cand_val[slot] = static_cast<float>(iv.est);
-                            }
                         }
                     }
                 }
             }
             if (mode == kModeSeed) {
+                qj[i * 64 + lane] = best_j;
+                qdt[i * 64 + lane] = best_dt;
+            }
+        }
+        __syncthreads();
+
+        // Level 2: the rigorous FP64 interval of filter.hpp, accumulators dead.
+        if (mode == kModeMain) {
+            const int qe = qn < kQ ? qn : kQ;
+            for (int e = lane; e < qe; e += 64) {
+                const int t = tp + qi[e];
+                const uint64_t j = qj[e];
+                if (w.x[j] == kMissing)
+                    continue;
+                const int m = tmpl[t].m;
+                const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
+                const double md = static_cast<double>(m);
+                const double abs_unit = PREC == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;
+                const double ea = eta_mfma(PREC, tmpl[t].kpad) + abs_unit * sqrt(md / Q2) * 1.01;
+                DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(qdt[e]), ea);
+                if (PREC == TMX_PREC_FP16 && iv.lo > -INFINITY) {
+                    const double S1 = w.p[j + m].x - w.p[j].x;
+                    const double V = (w.p[j + m].y - w.p[j].y) - S1 * S1 / md;
+                    const double sig = sqrt(fmax(V, 1e-300) / md);
+                    const double ex = 6.103515625e-05 * static_cast<double>(scale) * 128.0 * sqrt(md * Q2) * 1.01;
+                    iv.lo -= 2.0 * ex / (sig * (1.0 - 1e-3)) * 1.01;
+                }
+                if (iv.lo <= static_cast<double>(tmpl[t].bound)) {
+                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
+                    if (slot < cap) {
+                        cand[slot].t = static_cast<uint32_t>(t);
+                        cand[slot].j = static_cast<uint32_t>(j);
+                        cand_val[slot] = static_cast<float>(iv.est);
+                    }
+                }
+            }
+        } else {
+            for (int i = 0; i < nt; i++) {
+                const int t = tp + i;
+                const int m = tmpl[t].m;
                 float ub = INFINITY;
-                if (best_j != 0xffffffffu && w.x[best_j] != kMissing) {
-                    DistInterval iv = dist_interval(w, best_j, m, Q2, S, c, static_cast<double>(best_dt), ea);
+                const uint32_t bj0 = qj[i * 64 + lane];
+                if (static_cast<uint64_t>(m) <= w.n && bj0 != 0xffffffffu && w.x[bj0] != kMissing) {
+                    const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
+                    const double md = static_cast<double>(m);
+                    const double abs_unit = PREC == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;
+                    const double ea = eta_mfma(PREC, tmpl[t].kpad) + abs_unit * sqrt(md / Q2) * 1.01;
+                    DistInterval iv =
+                        dist_interval(w, bj0, m, Q2, S, c, static_cast<double>(qdt[i * 64 + lane]), ea);
                     if (PREC == TMX_PREC_FP16 && iv.hi < INFINITY) {
-                        const double p1a = w.p1[best_j], p1b = w.p1[best_j + m];
-                        const double p2a = w.p2[best_j], p2b = w.p2[best_j + m];
-                        const double V = (p2b - p2a) - (p1b - p1a) * (p1b - p1a) * inv_m;
-                        const double sig = sqrt(fmax(V, 1e-300) * inv_m);
-                        iv.hi += 2.0 * static_cast<double>(eabs_x) / (sig * (1.0 - 1e-3)) * 1.01;
+                        const double S1 = w.p[bj0 + m].x - w.p[bj0].x;
+                        const double V = (w.p[bj0 + m].y - w.p[bj0].y) - S1 * S1 / md;
+                        const double sig = sqrt(fmax(V, 1e-300) / md);
+                        const double ex =
+                            6.103515625e-05 * static_cast<double>(scale) * 128.0 * sqrt(md * Q2) * 1.01;
+                        iv.hi += 2.0 * ex / (sig * (1.0 - 1e-3)) * 1.01;
                     }
                     if (iv.hi < 3.0e38)
                         ub = static_cast<float>(iv.hi) * (1.0f + 1e-6f);
                 }
-                uint32_t bj = best_j;
+                uint32_t bj = bj0;
                 for (int off = 32; off > 0; off >>= 1) {
                     const float ov = __shfl_xor(ub, off);
                     const uint32_t oj = __shfl_xor(bj, off);
@@ -468,14 +578,19 @@ hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, c
         return hipSuccess;
     const uint64_t blocks = (w.n + kNW - 1) / kNW;
     const uint64_t launched = (blocks + block_stride - 1) / block_stride;
-    const dim3 grid(static_cast<unsigned>(launched), static_cast<unsigned>(groups));
+    const uint64_t total = ((launched + 7) / 8) * 8 * static_cast<uint64_t>(groups);
+    if (total >= (1ull << 32))
+        return hipErrorInvalidConfiguration;
+    const dim3 grid(static_cast<unsigned>(total));
     const Frag *a = static_cast<const Frag *>(a_frag);
     if (precision == TMX_PREC_FP16)
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_FP16>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a, mode,
-                           block_stride, cand, cand_val, cand_count, cand_cap);
+                           block_stride, groups, static_cast<uint32_t>(launched), cand, cand_val, cand_count,
+                           cand_cap);
     else
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_BF16X3>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a,
-                           mode, block_stride, cand, cand_val, cand_count, cand_cap);
+                           mode, block_stride, groups, static_cast<uint32_t>(launched), cand, cand_val, cand_count,
+                           cand_cap);
     return hipGetLastError();
 }
 

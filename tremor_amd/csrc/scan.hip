@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_fp32(Waveform w, const TmplDe
     if (j0 >= w.n)
         return;
     const uint64_t jend = j0 + kWB < w.n ? j0 + kWB : w.n;
-    const float c = static_cast<float>((w.p1[jend] - w.p1[j0]) / static_cast<double>(jend - j0));
+    const float c = static_cast<float>((w.p[jend].x - w.p[j0].x) / static_cast<double>(jend - j0));
     const int tb = group_begin[g], te = group_begin[g + 1];
     for (int t = tb; t < te; t++) {
         const int m = tmpl[t].m;

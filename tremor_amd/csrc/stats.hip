@@ -1,15 +1,16 @@
 // FP64 prefix sums of the waveform (x0 = x with missing samples as 0) and of
-// x0^2.  These are the resident statistics from which the inner-product
-// passes derive every window's mean and standard deviation for any template
-// length (with a rigorous error bound), replacing the reference's per-length
-// two-pass statistics loop (SkipSequential.cpp:157-190, indexing.cpp:128-153)
-// on the filtering side.  The exact two-pass statistics are recomputed only
-// for candidate windows (verify.hip).
+// x0^2, interleaved as pairs.  These are the resident statistics from which
+// the inner-product passes derive every window's mean and standard deviation
+// for any template length (with a rigorous error bound), replacing the
+// reference's per-length two-pass statistics loop (SkipSequential.cpp:157-190,
+// indexing.cpp:128-153) on the filtering side.  The exact two-pass statistics
+// are recomputed only for candidate windows (verify.hip).
 //
-// Three launches, HBM-bound: chunk sums, a scan of the chunk sums in one
-// workgroup, and the chunk-local scans with the chunk offsets added.  Every
-// output is a sum of its terms through at most ~600 sequential or tree
-// additions, well inside kPrefixGamma.
+// Three launches, HBM-bound (4 bytes read, 16 bytes written per sample): chunk
+// sums, a scan of the chunk sums in one workgroup, and the chunk-local scans
+// with the chunk offsets added.  Every output is a sum of its terms through
+// at most ~600 sequential or tree additions, well inside kPrefixGamma.  The
+// waveform buffer carries zero padding past n, so chunk reads need no guard.
 #include "tmx_internal.hpp"
 
 namespace tmx {
@@ -20,15 +21,21 @@ constexpr int kPItems = 16;
 constexpr int kPChunk = kPThreads * kPItems;  // samples per chunk
 constexpr int kTopThreads = 1024;
 
-__device__ inline double x0_at(const float *x, uint64_t i, uint64_t n)
+// The thread's 16 consecutive samples (missing as 0), 64-byte aligned loads.
+__device__ inline void load16(const float *x, uint64_t base, double *v)
 {
-    if (i >= n)
-        return 0.0;
-    const float v = x[i];
-    return v == kMissing ? 0.0 : static_cast<double>(v);
+    const float4 *src = reinterpret_cast<const float4 *>(x + base);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float4 f = src[q];
+        const float e[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            v[4 * q + u] = e[u] == kMissing ? 0.0 : static_cast<double>(e[u]);
+    }
 }
 
-// Inclusive Hillis-Steele scan of one double per thread across the workgroup.
+// Inclusive Hillis-Steele scan of two doubles per thread across the workgroup.
 template <int N>
 __device__ inline void block_scan2(double &a, double &b, double *sa, double *sb)
 {
@@ -53,17 +60,18 @@ __device__ inline void block_scan2(double &a, double &b, double *sa, double *sb)
     b = sb[tid];
 }
 
-__global__ __launch_bounds__(kPThreads) void k_chunk_sums(const float *__restrict__ x, uint64_t n,
-                                                          double *__restrict__ s1, double *__restrict__ s2)
+__global__ __launch_bounds__(kPThreads) void k_chunk_sums(const float *__restrict__ x, double *__restrict__ s1,
+                                                          double *__restrict__ s2)
 {
     __shared__ double sa[kPThreads], sb[kPThreads];
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kPChunk + static_cast<uint64_t>(threadIdx.x) * kPItems;
+    double v[kPItems];
+    load16(x, base, v);
     double a = 0.0, b = 0.0;
 #pragma unroll
     for (int e = 0; e < kPItems; e++) {
-        const double v = x0_at(x, base + e, n);
-        a += v;
-        b = fma(v, v, b);
+        a += v[e];
+        b = fma(v[e], v[e], b);
     }
     block_scan2<kPThreads>(a, b, sa, sb);
     if (threadIdx.x == kPThreads - 1) {
@@ -85,16 +93,9 @@ __global__ __launch_bounds__(kTopThreads) void k_chunk_scan(double *__restrict__
         a += s1[c];
         b += s2[c];
     }
-    double ia = a, ib = b;
-    block_scan2<kTopThreads>(ia, ib, sa, sb);
-    double ra = ia - a, rb = ib - b;  // exclusive prefix of this thread's range
-    if (threadIdx.x > 0) {
-        ra = sa[threadIdx.x - 1];
-        rb = sb[threadIdx.x - 1];
-    } else {
-        ra = 0.0;
-        rb = 0.0;
-    }
+    block_scan2<kTopThreads>(a, b, sa, sb);
+    double ra = threadIdx.x > 0 ? sa[threadIdx.x - 1] : 0.0;
+    double rb = threadIdx.x > 0 ? sb[threadIdx.x - 1] : 0.0;
     for (uint64_t c = lo; c < hi; c++) {
         const double va = s1[c], vb = s2[c];
         s1[c] = ra;
@@ -104,44 +105,51 @@ __global__ __launch_bounds__(kTopThreads) void k_chunk_scan(double *__restrict__
     }
 }
 
+constexpr int kRound = 64 * kPItems;  // pairs staged per round (64 threads' worth)
+
 __global__ __launch_bounds__(kPThreads) void k_chunk_final(const float *__restrict__ x, uint64_t n,
                                                            const double *__restrict__ off1,
-                                                           const double *__restrict__ off2, double *__restrict__ p1,
-                                                           double *__restrict__ p2)
+                                                           const double *__restrict__ off2, double2 *__restrict__ p)
 {
     __shared__ double sa[kPThreads], sb[kPThreads];
-    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kPChunk + static_cast<uint64_t>(threadIdx.x) * kPItems;
+    __shared__ double2 stage[kRound];
+    const int tid = threadIdx.x;
+    const uint64_t cbase = static_cast<uint64_t>(blockIdx.x) * kPChunk;
+    const uint64_t base = cbase + static_cast<uint64_t>(tid) * kPItems;
     double v[kPItems];
+    load16(x, base, v);
     double a = 0.0, b = 0.0;
 #pragma unroll
     for (int e = 0; e < kPItems; e++) {
-        v[e] = x0_at(x, base + e, n);
         a += v[e];
         b = fma(v[e], v[e], b);
     }
-    double ia = a, ib = b;
-    block_scan2<kPThreads>(ia, ib, sa, sb);
-    double ra = off1[blockIdx.x] + (ia - a);
-    double rb = off2[blockIdx.x] + (ib - b);
-    if (threadIdx.x > 0) {
-        ra = off1[blockIdx.x] + sa[threadIdx.x - 1];
-        rb = off2[blockIdx.x] + sb[threadIdx.x - 1];
-    } else {
-        ra = off1[blockIdx.x];
-        rb = off2[blockIdx.x];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        p1[0] = 0.0;
-        p2[0] = 0.0;
-    }
+    block_scan2<kPThreads>(a, b, sa, sb);
+    double ra = off1[blockIdx.x] + (tid > 0 ? sa[tid - 1] : 0.0);
+    double rb = off2[blockIdx.x] + (tid > 0 ? sb[tid - 1] : 0.0);
+    if (blockIdx.x == 0 && tid == 0)
+        p[0] = make_double2(0.0, 0.0);
+    double2 out[kPItems];
 #pragma unroll
     for (int e = 0; e < kPItems; e++) {
         ra += v[e];
         rb = fma(v[e], v[e], rb);
-        const uint64_t i = base + e;
-        if (i < n) {
-            p1[i + 1] = ra;
-            p2[i + 1] = rb;
+        out[e] = make_double2(ra, rb);
+    }
+    // Transpose through LDS in rounds of 64 threads so that the stores of
+    // consecutive threads are consecutive pairs.
+    for (int q = 0; q < kPThreads / 64; q++) {
+        __syncthreads();
+        if ((tid >> 6) == q) {
+#pragma unroll
+            for (int e = 0; e < kPItems; e++)
+                stage[(tid & 63) * kPItems + e] = out[e];
+        }
+        __syncthreads();
+        for (int k = tid; k < kRound; k += kPThreads) {
+            const uint64_t i = cbase + static_cast<uint64_t>(q) * kRound + static_cast<uint64_t>(k);
+            if (i < n)
+                p[i + 1] = stage[k];
         }
     }
 }
@@ -154,21 +162,16 @@ size_t prefix_work_len(uint64_t n)
     return static_cast<size_t>(2 * chunks + 2);
 }
 
-hipError_t launch_prefix(const float *x, uint64_t n, double *p1, double *p2, double *work, size_t work_len,
-                         hipStream_t stream)
+hipError_t launch_prefix(const float *x, uint64_t n, double2 *p, double *work, size_t work_len, hipStream_t stream)
 {
     const uint64_t chunks = (n + kPChunk - 1) / kPChunk;
-    if (chunks == 0)
-        return hipMemsetAsync(p1, 0, sizeof(double), stream) == hipSuccess
-                   ? hipMemsetAsync(p2, 0, sizeof(double), stream)
-                   : hipErrorInvalidValue;
-    if (work_len < 2 * chunks)
+    if (chunks == 0 || work_len < 2 * chunks)
         return hipErrorInvalidValue;
     double *s1 = work, *s2 = work + chunks;
-    hipLaunchKernelGGL(k_chunk_sums, dim3(static_cast<unsigned>(chunks)), dim3(kPThreads), 0, stream, x, n, s1, s2);
+    hipLaunchKernelGGL(k_chunk_sums, dim3(static_cast<unsigned>(chunks)), dim3(kPThreads), 0, stream, x, s1, s2);
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(kTopThreads), 0, stream, s1, s2, chunks);
     hipLaunchKernelGGL(k_chunk_final, dim3(static_cast<unsigned>(chunks)), dim3(kPThreads), 0, stream, x, n, s1, s2,
-                       p1, p2);
+                       p);
     return hipGetLastError();
 }
 

@@ -104,7 +104,7 @@ struct tmx_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     // resident waveform
-    DevMem x, p1, p2, prefix_work;
+    DevMem x, p, prefix_work;
     uint64_t n = 0;
     // per-call buffers
     DevMem raw, qz, order, tmpl, groups, afrag;
@@ -247,7 +247,7 @@ int ensure_cand(tmx_ctx *ctx, uint64_t cap, int T)
 // count through *count (may exceed the capacity).
 int run_pass(tmx_ctx *ctx, const CallPlan &plan, int mode, int block_stride, uint64_t *count)
 {
-    Waveform w{ctx->x.as<float>(), ctx->p1.as<double>(), ctx->p2.as<double>(), ctx->n};
+    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
     TMX_HIP(launch_zero_u64(ctx->cand_count.as<unsigned long long>(), ctx->stream), "reset counter");
     const int groups = static_cast<int>(plan.group_begin.size()) - 1;
     TMX_HIP(hipEventRecord(ctx->ev0, ctx->stream), "event");
@@ -431,14 +431,13 @@ int tmx_load_waveform(tmx_ctx *ctx, const float *x, uint64_t n)
     const size_t pad = 1 << 20;
     ctx->n = 0;
     TMX_HIP(ctx->x.reserve(sizeof(float) * (n + pad)), "hipMalloc waveform");
-    TMX_HIP(ctx->p1.reserve(sizeof(double) * (n + 1)), "hipMalloc prefix sums");
-    TMX_HIP(ctx->p2.reserve(sizeof(double) * (n + 1)), "hipMalloc prefix sums");
+    TMX_HIP(ctx->p.reserve(sizeof(double2) * (n + 1)), "hipMalloc prefix sums");
     const size_t wl = prefix_work_len(n);
     TMX_HIP(ctx->prefix_work.reserve(sizeof(double) * wl), "hipMalloc prefix scratch");
     TMX_HIP(hipMemcpyAsync(ctx->x.p, x, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream), "copy waveform");
     TMX_HIP(hipMemsetAsync(ctx->x.as<float>() + n, 0, sizeof(float) * pad, ctx->stream), "clear padding");
-    TMX_HIP(launch_prefix(ctx->x.as<float>(), n, ctx->p1.as<double>(), ctx->p2.as<double>(),
-                          ctx->prefix_work.as<double>(), wl, ctx->stream),
+    TMX_HIP(launch_prefix(ctx->x.as<float>(), n, ctx->p.as<double2>(), ctx->prefix_work.as<double>(), wl,
+                          ctx->stream),
             "prefix sums");
     TMX_HIP(hipStreamSynchronize(ctx->stream), "load waveform");
     ctx->n = n;
@@ -478,7 +477,7 @@ int tmx_knn(tmx_ctx *ctx, const float *templates, const int32_t *lengths, const 
     // sample of blocks, then the k-th (2k-1-th separated) smallest.
     const int want = merge_offset > 0 ? 2 * k - 1 : k;
     const uint64_t blocks = waveform_blocks(ctx);
-    const uint64_t target = ctx->cfg.seed_target > 0 ? static_cast<uint64_t>(ctx->cfg.seed_target) : 4096;
+    const uint64_t target = ctx->cfg.seed_target > 0 ? static_cast<uint64_t>(ctx->cfg.seed_target) : 1024;
     const uint64_t per_block = ctx->cfg.precision == TMX_PREC_FP32 ? 4 : 4;
     uint64_t stride = (blocks * per_block) / target;
     if (stride < 1)

@@ -51,16 +51,14 @@ constexpr double kPrefixGamma = 8192.0 * 1.1102230246251565e-16;
 
 struct Waveform {
     const float *x;     // raw samples, n + tail padding (missing kept)
-    const double *p1;   // prefix sums of x0 (missing as 0), n + 1 values
-    const double *p2;   // prefix sums of x0^2, n + 1 values
+    const double2 *p;   // prefix sums (x0, x0^2) interleaved, missing as 0, n + 1 pairs
     uint64_t n;
 };
 
 // ---- launchers (all asynchronous on `stream`) ----------------------------
 
-// FP64 inclusive prefix sums P1, P2 of x0 and x0^2 (P[0] = 0).
-hipError_t launch_prefix(const float *x, uint64_t n, double *p1, double *p2, double *work, size_t work_len,
-                         hipStream_t stream);
+// FP64 inclusive prefix sums (P1, P2) of x0 and x0^2, interleaved (P[0] = 0).
+hipError_t launch_prefix(const float *x, uint64_t n, double2 *p, double *work, size_t work_len, hipStream_t stream);
 size_t prefix_work_len(uint64_t n);
 
 // Reference template preparation: z-norm (ZnormSearch.hpp:33-62), block
