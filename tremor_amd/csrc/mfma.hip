@@ -66,9 +66,10 @@ __device__ inline int swz(int e) { return e ^ (((e >> 6) & 3) << 3); }
 // by the rigorous FP64 filter once the accumulators are no longer live.
 constexpr int kStatLen = kTileWin + kTileWin / 32;
 constexpr int kQ = 384;  // queue entries per wavefront (>= kNM * 64 for the seed mode)
+constexpr int kStatBatch = 2;  // tile-statistics loads in flight per lane
 constexpr int kTileBytesBf16 = 2 * kStage * 2;
 constexpr int kTileBytesF16 = kStage * 2;
-constexpr int kEpiBytes = kWaves * kStatLen * 8 + kWaves * kQ * 12 + kWaves * 16 * 64 * 4;
+constexpr int kEpiBytes = kWaves * kStatLen * 4 + kWaves * kQ * 12 + kWaves * 16 * 64 * 4;
 static_assert(kQ >= kNM * 64, "seed entries need one queue slot per lane and template");
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
@@ -155,11 +156,21 @@ __device__ inline double eta_mfma(int precision, int kpad)
 
 // ---- main kernel ------------------------------------------------------------
 
+// Orders LDS traffic among the lanes of one wavefront (its private scratch).
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int PREC>
 __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const TmplDev *__restrict__ tmpl,
                                                            const int32_t *__restrict__ group_begin,
                                                            const Frag *__restrict__ afrag, int mode,
                                                            int block_stride, int groups, uint32_t nblocks,
+                                                           const float2 *__restrict__ rb,
+                                                           const float4 *__restrict__ binfo, int rb_len,
                                                            Cand *__restrict__ cand,
                                                            float *__restrict__ cand_val,
                                                            unsigned long long *__restrict__ cand_count, uint64_t cap)
@@ -172,11 +183,11 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int col = lane & 31, h = lane >> 5;
-    float2 *const stats = reinterpret_cast<float2 *>(lds) + wave * kStatLen;
-    uint32_t *const qj = reinterpret_cast<uint32_t *>(lds + kWaves * kStatLen * 8) + wave * kQ;
-    float *const qdt = reinterpret_cast<float *>(lds + kWaves * kStatLen * 8 + kWaves * kQ * 4) + wave * kQ;
-    int32_t *const qi = reinterpret_cast<int32_t *>(lds + kWaves * kStatLen * 8 + kWaves * kQ * 8) + wave * kQ;
-    float *const sdt = reinterpret_cast<float *>(lds + kWaves * kStatLen * 8 + kWaves * kQ * 12) + wave * 16 * 64;
+    float *const stats = reinterpret_cast<float *>(lds) + wave * kStatLen;
+    uint32_t *const qj = reinterpret_cast<uint32_t *>(lds + kWaves * kStatLen * 4) + wave * kQ;
+    float *const qdt = reinterpret_cast<float *>(lds + kWaves * kStatLen * 4 + kWaves * kQ * 4) + wave * kQ;
+    int32_t *const qi = reinterpret_cast<int32_t *>(lds + kWaves * kStatLen * 4 + kWaves * kQ * 8) + wave * kQ;
+    float *const sdt = reinterpret_cast<float *>(lds + kWaves * kStatLen * 4 + kWaves * kQ * 12) + wave * 16 * 64;
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
     // on each XCD, consecutive workgroups take every template group of one
     // waveform block, so the block's samples and prefix sums are shared in
@@ -191,8 +202,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
     if (J0 >= w.n)
         return;
     const int tb = group_begin[g], te = group_begin[g + 1];
-    const uint64_t jend = J0 + kNW < w.n ? J0 + kNW : w.n;
-    const float c = static_cast<float>((w.p[jend].x - w.p[J0].x) / static_cast<double>(jend - J0));
+    const float4 bi = binfo[blk];  // block centre and error bounds of the rebased prefix pairs
+    const float c = bi.x;
+    const float2 *const rbb = rb + blk * static_cast<uint64_t>(rb_len);
 
     // Largest K of the group (templates sorted by length: the last one).
     const int kmax = tmpl[te - 1].kpad;
@@ -343,7 +355,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
         // Level 1, while the accumulators are live: FP32 only.  Windows that
         // pass go to the wavefront's queue (or, if it is full, straight to
         // the candidate list: extra candidates cost verification time only).
-        int qn = 0;  // wavefront-uniform queue length
+        int qn = 0;         // wavefront-uniform queue length
+        uint32_t ovf = 0;   // tiles (i * kNN + jn) whose survivors overflowed the queue
+        __syncthreads();  // the epilogue scratch overlays the tile other wavefronts were reading
 #pragma unroll
         for (int i = 0; i < kNM; i++) {
             if (i >= nt)
@@ -366,6 +380,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                 static_cast<float>((Q2 + md - static_cast<double>(tmpl[t].bound) - eref_cap(m)) * 0.5 * (1.0 - 1e-3));
             const float eta_sq = static_cast<float>(eta * sqrt(Q2) * (1.0 + 1e-3));
             const float sqrt_m = static_cast<float>(sqrt(md));
+            const float mf = static_cast<float>(m);
+            const float inv_mf = 1.0f / mf;
             const float rsqrt_m = static_cast<float>(1.0 / sqrt(md)) * (1.0f + 1e-6f);
             const float absS = static_cast<float>(fabs(tmpl[t].S) * (1.0 + 1e-3));
             const float eabs_q = static_cast<float>(abs_unit * sqrt(md) * 1.01);  // * sqrt(sum x~^2)
@@ -379,85 +395,85 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
 #pragma unroll
             for (int jn = 0; jn < kNN; jn++) {
                 const uint64_t tile0 = J0 + 32ull * static_cast<uint64_t>((wave * kNN + jn) * 32);
-                // Window statistics of the tile, coalesced: (m sigma^2, mu) as
-                // floats; -1 marks windows the level-1 test cannot judge (stats
-                // swamped by prefix rounding, or |mu| / sigma > 1e3), NaN marks
-                // windows past the last one.
-                __syncthreads();
+                // Per window of the tile, coalesced, from the block-rebased
+                // prefix pairs: main mode, the threshold the dot product must
+                // reach to survive level 1 (-inf: cannot be judged - stats
+                // swamped by rounding or |mu| / sigma > 1e3 - so it survives;
+                // +inf: past the last window); seed mode, 1 / sqrt(m sigma^2)
+                // (or -1).
+                wave_sync();
 #pragma unroll 1
-                for (int it0 = 0; it0 < 16; it0 += 4) {
-                    double2 pa[4], pb[4];
+                for (int it0 = 0; it0 < 16; it0 += kStatBatch) {
+                    float2 ra[kStatBatch], rz[kStatBatch];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const uint64_t j = tile0 + static_cast<uint64_t>((it0 + u) * 64 + lane);
-                        const uint64_t jj = j < W ? j : 0;
-                        pa[u] = w.p[jj];
-                        pb[u] = w.p[jj + m];
+                    for (int u = 0; u < kStatBatch; u++) {
+                        const int i0 = static_cast<int>(tile0 - J0) + (it0 + u) * 64 + lane;
+                        ra[u] = rbb[i0];
+                        rz[u] = rbb[i0 + m];
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
+                    for (int u = 0; u < kStatBatch; u++) {
                         const int wl = (it0 + u) * 64 + lane;
                         const uint64_t j = tile0 + static_cast<uint64_t>(wl);
-                        float2 sv = make_float2(NAN, 0.0f);
+                        float thr = INFINITY;
                         if (j < W) {
-                            const double S1 = pb[u].x - pa[u].x, S2 = pb[u].y - pa[u].y;
-                            const double V = S2 - S1 * S1 * inv_m;
-                            const double eVc = kPrefixGamma * (pb[u].y + pa[u].y) * 4.0 + 1e-15 * fabs(S2);
-                            const double mu = S1 * inv_m;
-                            const bool judge = V > 1e3 * eVc && V > md * 1e-20 && !(mu * mu * md > 1e6 * V);
-                            sv = make_float2(judge ? static_cast<float>(V) : -1.0f, static_cast<float>(mu));
+                            // Window sums of x0 - c and (x0 - c)^2 with their error bounds.
+                            const float S1 = rz[u].x - ra[u].x, S2 = rz[u].y - ra[u].y;
+                            const float eS1 = 2.0f * bi.y + 1.2e-7f * (fabsf(ra[u].x) + fabsf(rz[u].x) + fabsf(S1));
+                            const float eS2 = 2.0f * bi.z + 1.2e-7f * (fabsf(ra[u].y) + fabsf(rz[u].y) + fabsf(S2));
+                            const float V = S2 - S1 * S1 * inv_mf;
+                            const float eV = eS2 + (2.0f * fabsf(S1) * eS1 + eS1 * eS1) * inv_mf +
+                                             2.4e-7f * (fabsf(S2) + S1 * S1 * inv_mf);
+                            const float aD = fabsf(S1 * inv_mf) + eS1 * inv_mf;  // >= |mu - c|
+                            const float amu = fabsf(c) + aD;
+                            const bool judge = V > 1e3f * eV && V > mf * 1e-20f && !(amu * amu * mf > 1e6f * V);
+                            if (mode == kModeMain) {
+                                if (!judge || gmin <= 0.0f) {
+                                    thr = -INFINITY;
+                                } else {
+                                    const float sV = V * __frsqrt_rn(V);
+                                    const float b = eta_sq * (sV + sqrt_m * aD) + aD * absS +
+                                                    eabs_q * (sV + sqrt_m * aD) + eabs_x;
+                                    thr = gmin * sV * rsqrt_m * 0.999f - b * 1.001f - 1e-30f;
+                                }
+                            } else {
+                                thr = judge ? __frsqrt_rn(V) : -1.0f;
+                            }
                         }
-                        stats[wl + (wl >> 5)] = sv;
+                        stats[wl + (wl >> 5)] = thr;
                     }
                 }
-                __syncthreads();
-                // The tile's dot products go to LDS so that the test below
-                // runs as a loop (few live registers) and this tile's
-                // accumulators die here.
+                // The tile's dot products go to LDS (this tile's accumulators
+                // die here) and the level-1 test runs as a loop.
 #pragma unroll
                 for (int r = 0; r < 16; r++)
                     sdt[r * 64 + lane] = acc[i][jn][r] * scale;
-                __syncthreads();
-#pragma unroll 1
+                wave_sync();
+#pragma unroll 2
                 for (int r = 0; r < 16; r++) {
                     const int wl = 32 * col + 8 * (r >> 2) + 4 * h + (r & 3);
-                    const float2 sv = stats[wl + (wl >> 5)];
-                    const float Vf = sv.x;
-                    const bool valid = Vf == Vf;  // not past the last window
+                    const float thr = stats[wl + (wl >> 5)];
                     const float dt = sdt[r * 64 + lane];
                     if (mode == kModeMain) {
-                        bool survive = valid && (Vf < 0.0f || gmin <= 0.0f);
-                        if (valid && !survive) {
-                            const float D = sv.y - c;
-                            const float sV = Vf * __frsqrt_rn(Vf);
-                            const float rhs = gmin * sV * rsqrt_m;
-                            const float b = eta_sq * (sV + sqrt_m * fabsf(D)) + fabsf(D) * absS +
-                                            eabs_q * (sV + sqrt_m * fabsf(D)) + eabs_x;
-                            survive = dt + b * 1.001f + 1e-30f >= rhs * 0.999f;
-                        }
+                        const bool survive = dt >= thr;
                         const uint64_t mask = __ballot(survive);
                         if (mask != 0ull) {
-                            const uint32_t j = static_cast<uint32_t>(tile0 + static_cast<uint64_t>(wl));
+                            // Survivors go to the queue; a tile whose survivors do
+                            // not fit is marked and, once the accumulators are
+                            // dead, all its windows become candidates (extra
+                            // candidates cost verification time only).
                             const int idx = qn + __popcll(mask & ((1ull << lane) - 1ull));
-                            if (survive) {
-                                if (idx < kQ) {
-                                    qj[idx] = j;
-                                    qdt[idx] = dt;
-                                    qi[idx] = i;
-                                } else if (w.x[j] != kMissing) {
-                                    // Queue full: a candidate without the rigorous test.
-                                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
-                                    if (slot < cap) {
-                                        cand[slot].t = static_cast<uint32_t>(t);
-                                        cand[slot].j = j;
-                                        cand_val[slot] = NAN;
-                                    }
-                                }
+                            if (survive && idx < kQ) {
+                                qj[idx] = static_cast<uint32_t>(tile0 + static_cast<uint64_t>(wl));
+                                qdt[idx] = dt;
+                                qi[idx] = i * kNN + jn;
                             }
                             qn += __popcll(mask);
+                            if (qn > kQ)
+                                ovf |= 1u << (i * kNN + jn);
                         }
-                    } else if (valid && Vf > 0.0f) {
-                        const float gapprox = dt * __frsqrt_rn(Vf);
+                    } else if (thr > 0.0f && thr < INFINITY) {
+                        const float gapprox = dt * thr;
                         if (gapprox > best_g) {
                             best_g = gapprox;
                             best_j = static_cast<uint32_t>(tile0 + static_cast<uint64_t>(wl));
@@ -475,9 +491,31 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
 
         // Level 2: the rigorous FP64 interval of filter.hpp, accumulators dead.
         if (mode == kModeMain) {
+            // Overflowed tiles: every valid window is a candidate.
+            for (int b = 0; b < kNM * kNN; b++) {
+                if (!((ovf >> b) & 1u))
+                    continue;
+                const int t = tp + b / kNN;
+                const int m = tmpl[t].m;
+                const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
+                const uint64_t tile0 = J0 + 32ull * static_cast<uint64_t>((wave * kNN + b % kNN) * 32);
+                for (int wl = lane; wl < kTileWin; wl += 64) {
+                    const uint64_t j = tile0 + static_cast<uint64_t>(wl);
+                    if (j >= W || w.x[j] == kMissing)
+                        continue;
+                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
+                    if (slot < cap) {
+                        cand[slot].t = static_cast<uint32_t>(t);
+                        cand[slot].j = static_cast<uint32_t>(j);
+                        cand_val[slot] = NAN;
+                    }
+                }
+            }
             const int qe = qn < kQ ? qn : kQ;
             for (int e = lane; e < qe; e += 64) {
-                const int t = tp + qi[e];
+                if ((ovf >> qi[e]) & 1u)
+                    continue;
+                const int t = tp + qi[e] / kNN;
                 const uint64_t j = qj[e];
                 if (w.x[j] == kMissing)
                     continue;
@@ -570,8 +608,9 @@ hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, const float *qz, int
 }
 
 hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, const int32_t *group_begin, int groups,
-                            int precision, const void *a_frag, int mode, int block_stride, Cand *cand, float *cand_val,
-                            unsigned long long *cand_count, uint64_t cand_cap, hipStream_t stream)
+                            int precision, const void *a_frag, const float2 *rb, const float4 *binfo, int rb_len,
+                            int mode, int block_stride, Cand *cand, float *cand_val, unsigned long long *cand_count,
+                            uint64_t cand_cap, hipStream_t stream)
 {
     (void)count;
     if (groups <= 0 || w.n == 0)
@@ -585,12 +624,12 @@ hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, c
     const Frag *a = static_cast<const Frag *>(a_frag);
     if (precision == TMX_PREC_FP16)
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_FP16>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a, mode,
-                           block_stride, groups, static_cast<uint32_t>(launched), cand, cand_val, cand_count,
-                           cand_cap);
+                           block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, cand, cand_val,
+                           cand_count, cand_cap);
     else
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_BF16X3>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a,
-                           mode, block_stride, groups, static_cast<uint32_t>(launched), cand, cand_val, cand_count,
-                           cand_cap);
+                           mode, block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, cand,
+                           cand_val, cand_count, cand_cap);
     return hipGetLastError();
 }
 

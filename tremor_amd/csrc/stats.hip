@@ -154,7 +154,67 @@ __global__ __launch_bounds__(kPThreads) void k_chunk_final(const float *__restri
     }
 }
 
+// Block-rebased FP32 prefix pairs for the level-1 filter of the matrix-core
+// pass: for waveform block b (first window J0 = b * block), centre c_b = mean
+// of x0 over the block's windows, and for i in [0, len)
+//   rb[b * len + i] = ( sum_{J0 <= k < J0 + i} (x0_k - c_b),
+//                       sum_{J0 <= k < J0 + i} (x0_k - c_b)^2 )
+// from the FP64 prefix sums, rounded to float.  binfo[b] = (c_b, E1, E2, 0):
+// E1, E2 bound the absolute error of every entry's FP64 value (prefix-sum
+// error, kPrefixGamma, plus the rounding of the rebasing), so that the FP32
+// consumer can bound a window's statistics: entry error <= E + 2^-24 |entry|.
+__global__ void k_block_prefix(const double2 *__restrict__ p, uint64_t n, uint64_t koff, int block, int len,
+                               float4 *__restrict__ binfo, float2 *__restrict__ rb)
+{
+    constexpr double u64 = 1.1102230246251565e-16;
+    const uint64_t b = blockIdx.y;
+    const uint64_t J0 = b * static_cast<uint64_t>(block);
+    const uint64_t jend = J0 + block < n ? J0 + block : n;
+    const double2 p0 = p[J0];
+    const float c = static_cast<float>((p[jend].x - p0.x) / static_cast<double>(jend - J0));
+    const double cd = static_cast<double>(c);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint64_t klast = J0 + static_cast<uint64_t>(len) < n ? J0 + static_cast<uint64_t>(len) : n;
+        const double pmax = p[klast].y;  // P2 is non-decreasing: bounds every P2 of the block
+        const double L = static_cast<double>(klast - J0);
+        const double eP2 = kPrefixGamma * pmax;
+        const double eP1 = kPrefixGamma * sqrt(static_cast<double>(koff + klast) * pmax);  // absolute index
+        const double s1max = sqrt(L * pmax);
+        const double E1 = 2.0 * eP1 + 4.0 * u64 * (s1max + fabs(cd) * L);
+        const double E2 = 2.0 * eP2 + 4.0 * fabs(cd) * eP1 + 4.0 * u64 * (pmax + 2.0 * fabs(cd) * s1max + cd * cd * L);
+        binfo[b] = make_float4(c, static_cast<float>(E1 * 1.01 + 1e-30), static_cast<float>(E2 * 1.01 + 1e-30), 0.0f);
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) {
+        const uint64_t k = J0 + static_cast<uint64_t>(i) < n ? J0 + static_cast<uint64_t>(i) : n;
+        const double2 pk = p[k];
+        const double cnt = static_cast<double>(k - J0);
+        const double s1 = pk.x - p0.x;
+        const double s2 = pk.y - p0.y;
+        const double t1 = s1 - cd * cnt;
+        const double t2 = s2 - 2.0 * cd * s1 + cd * cd * cnt;
+        rb[b * static_cast<uint64_t>(len) + static_cast<uint64_t>(i)] =
+            make_float2(static_cast<float>(t1), static_cast<float>(t2));
+    }
+}
+
 }  // namespace
+
+hipError_t launch_block_prefix(const double2 *p, uint64_t n, int block, int len, float4 *binfo, float2 *rb,
+                               hipStream_t stream)
+{
+    const uint64_t blocks = (n + block - 1) / block;
+    if (blocks == 0)
+        return hipErrorInvalidValue;
+    const unsigned gx = static_cast<unsigned>((len + 255) / 256 < 64 ? (len + 255) / 256 : 64);
+    // grid.y is limited to 65535: larger waveforms take several launches.
+    for (uint64_t b0 = 0; b0 < blocks; b0 += 65535) {
+        const uint64_t nb = blocks - b0 < 65535 ? blocks - b0 : 65535;
+        hipLaunchKernelGGL(k_block_prefix, dim3(gx, static_cast<unsigned>(nb)), dim3(256), 0, stream,
+                           p + b0 * block, n - b0 * block, b0 * block, block, len, binfo + b0,
+                           rb + b0 * static_cast<uint64_t>(len));
+    }
+    return hipGetLastError();
+}
 
 size_t prefix_work_len(uint64_t n)
 {

@@ -42,8 +42,9 @@ hipError_t launch_knn_select(const uint64_t *keys, const uint32_t *offsets, int 
 hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, const float *qz, int precision, void *a_frag,
                             hipStream_t stream);
 hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, const int32_t *group_begin, int groups,
-                            int precision, const void *a_frag, int mode, int block_stride, Cand *cand, float *cand_val,
-                            unsigned long long *cand_count, uint64_t cand_cap, hipStream_t stream);
+                            int precision, const void *a_frag, const float2 *rb, const float4 *binfo, int rb_len,
+                            int mode, int block_stride, Cand *cand, float *cand_val, unsigned long long *cand_count,
+                            uint64_t cand_cap, hipStream_t stream);
 int mfma_group_size();
 uint64_t mfma_block_windows();
 int64_t mfma_frag_count(int kpad);
@@ -106,6 +107,10 @@ struct tmx_ctx {
     // resident waveform
     DevMem x, p, prefix_work;
     uint64_t n = 0;
+    // block-rebased FP32 prefix pairs for the matrix-core pass (built on demand)
+    DevMem rb, binfo;
+    int rb_len = 0;
+    uint64_t rb_n = 0;
     // per-call buffers
     DevMem raw, qz, order, tmpl, groups, afrag;
     DevMem cand, cand_val, cand_d, cand_count;
@@ -243,10 +248,37 @@ int ensure_cand(tmx_ctx *ctx, uint64_t cap, int T)
     return 0;
 }
 
+// Block-rebased prefix pairs long enough for the plan's longest template.
+int ensure_block_prefix(tmx_ctx *ctx, const CallPlan &plan)
+{
+    int kmax = 0;
+    for (const auto &t : plan.tmpl)
+        kmax = std::max(kmax, t.kpad);
+    const int nw = static_cast<int>(mfma_block_windows());
+    const int need = ((nw + kmax + 1023) / 1024) * 1024;
+    if (ctx->rb_n == ctx->n && ctx->rb_len >= need)
+        return 0;
+    const uint64_t blocks = (ctx->n + nw - 1) / nw;
+    ctx->rb_n = 0;
+    TMX_HIP(ctx->rb.reserve(sizeof(float2) * blocks * static_cast<uint64_t>(need)), "hipMalloc block prefix");
+    TMX_HIP(ctx->binfo.reserve(sizeof(float4) * blocks), "hipMalloc block prefix");
+    TMX_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->n, nw, need, ctx->binfo.as<float4>(), ctx->rb.as<float2>(),
+                                ctx->stream),
+            "block prefix");
+    ctx->rb_len = need;
+    ctx->rb_n = ctx->n;
+    return 0;
+}
+
 // One inner-product pass (seed or main) over every group; returns the entry
 // count through *count (may exceed the capacity).
 int run_pass(tmx_ctx *ctx, const CallPlan &plan, int mode, int block_stride, uint64_t *count)
 {
+    if (ctx->cfg.precision != TMX_PREC_FP32) {
+        const int rc = ensure_block_prefix(ctx, plan);
+        if (rc)
+            return rc;
+    }
     Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
     TMX_HIP(launch_zero_u64(ctx->cand_count.as<unsigned long long>(), ctx->stream), "reset counter");
     const int groups = static_cast<int>(plan.group_begin.size()) - 1;
@@ -258,9 +290,9 @@ int run_pass(tmx_ctx *ctx, const CallPlan &plan, int mode, int block_stride, uin
                 "FP32 scan");
     } else {
         TMX_HIP(launch_scan_mfma(w, ctx->tmpl.as<TmplDev>(), plan.T, ctx->groups.as<int32_t>(), groups,
-                                 ctx->cfg.precision, ctx->afrag.p, mode, block_stride, ctx->cand.as<Cand>(),
-                                 ctx->cand_val.as<float>(), ctx->cand_count.as<unsigned long long>(), ctx->cand_cap,
-                                 ctx->stream),
+                                 ctx->cfg.precision, ctx->afrag.p, ctx->rb.as<float2>(), ctx->binfo.as<float4>(),
+                                 ctx->rb_len, mode, block_stride, ctx->cand.as<Cand>(), ctx->cand_val.as<float>(),
+                                 ctx->cand_count.as<unsigned long long>(), ctx->cand_cap, ctx->stream),
                 "MFMA scan");
     }
     TMX_HIP(hipEventRecord(ctx->ev1, ctx->stream), "event");
@@ -430,6 +462,7 @@ int tmx_load_waveform(tmx_ctx *ctx, const float *x, uint64_t n)
     // Tail padding: inner-product tiles read past the last window.
     const size_t pad = 1 << 20;
     ctx->n = 0;
+    ctx->rb_n = 0;
     TMX_HIP(ctx->x.reserve(sizeof(float) * (n + pad)), "hipMalloc waveform");
     TMX_HIP(ctx->p.reserve(sizeof(double2) * (n + 1)), "hipMalloc prefix sums");
     const size_t wl = prefix_work_len(n);

@@ -61,6 +61,12 @@ struct Waveform {
 hipError_t launch_prefix(const float *x, uint64_t n, double2 *p, double *work, size_t work_len, hipStream_t stream);
 size_t prefix_work_len(uint64_t n);
 
+// Block-rebased FP32 prefix pairs (and per-block centre and error bounds) for
+// the level-1 filter of the matrix-core pass; `block` windows per block,
+// `len` entries per block.
+hipError_t launch_block_prefix(const double2 *p, uint64_t n, int block, int len, float4 *binfo, float2 *rb,
+                               hipStream_t stream);
+
 // Reference template preparation: z-norm (ZnormSearch.hpp:33-62), block
 // order (:72-90), plus Q2 and S.  One workgroup per template.
 hipError_t launch_prepare(const float *raw, TmplDev *tmpl, int count, float *qz, int32_t *order,
