@@ -40,7 +40,9 @@ def _args():
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--scan-steps", type=int, default=1, help="steps of the plain FP32 scan (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
-    ap.add_argument("--check", action="store_true", help="compare the two GPU forms' answers")
+    ap.add_argument("--config", default="c1", choices=["c1", "c4"],
+                    help="c1: batched uneven templates (the headline); c4: long templates, FFT vs GEMM forms")
+    ap.add_argument("--fft", default="auto", choices=["auto", "off", "always"])
     return ap.parse_args()
 
 
@@ -68,6 +70,45 @@ def cpu_baseline(x, templates, k, budget_s):
                       f"(window statistics built beforehand, as the reference separates index and query time)"}
 
 
+def bench_long(a):
+    """Long-template regime (c4 shape): n = 2^28, 64 templates m ~ logU[8192,
+    65536], kNN k = 10; the FFT form, the split-BF16 GEMM form and the cost
+    model's choice, all with exact verification."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    n = a.n if a.n != 10**8 else 2**28
+    w = synth.make("c4", n=n, templates=a.templates if a.templates != 256 else 64)
+    k = 10
+    ctx = Context(precision=a.prec, fft="always")
+    ctx.load_waveform(w.x)
+    out = {"config": {"workload": f"long templates: n={w.x.size}, {len(w.templates)} templates "
+                                  f"m~logU[8192,65536], kNN k={k}"}}
+    res = {}
+    for mode, steps in (("always", a.steps), ("off", 1), ("auto", a.steps)):
+        ctx.set_fft_mode(mode)
+        pos, d, st = ctx.knn(w.templates, k, 0)  # warmup (builds spectra / operands)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pos, d, st = ctx.knn(w.templates, k, 0)
+        el = (time.perf_counter() - t0) / steps
+        res[mode] = {"ms_per_step": 1e3 * el, "comparisons_per_s": st["comparisons"] / el,
+                     "kernel_ms": st["kernel_ms_main"], "fft_templates": st["fft_templates"],
+                     "fft_block": st["fft_block"], "candidates": st["candidates"],
+                     "max_abs_rho_err": st["max_abs_rho_err"], "answers": (pos.copy(), d.copy())}
+        print(f"[bench] c4 {mode}: {1e3 * el:.1f} ms/step, main {st['kernel_ms_main']:.1f} ms, "
+              f"fft templates {st['fft_templates']}", file=sys.stderr, flush=True)
+    same = all(np.array_equal(res[m]["answers"][0], res["off"]["answers"][0]) and
+               np.array_equal(res[m]["answers"][1].view(np.uint32), res["off"]["answers"][1].view(np.uint32))
+               for m in res)
+    for m in res:
+        del res[m]["answers"]
+    out.update({"metric": METRIC, "unit": "comparisons/s", "n_gpus": 1, "forms": res,
+                "same_answers_all_forms": bool(same), "steps": a.steps, "higher_is_better": True,
+                "value": res["auto"]["comparisons_per_s"], "dtype": a.prec + " / fp32 FFT + exact verify",
+                "data": "synthetic"})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -75,6 +116,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    if a.config == "c4":
+        return bench_long(a)
     from tremor_amd import synth
     from tremor_amd.capi import Context
 
@@ -83,7 +126,7 @@ def main():
     t_gen = time.time() - t_gen
     print(f"[bench] generated n={w.x.size} T={len(w.templates)} in {t_gen:.1f}s", file=sys.stderr, flush=True)
 
-    ctx = Context(precision=a.prec)
+    ctx = Context(precision=a.prec, fft=a.fft)
     t0 = time.time()
     ctx.load_waveform(w.x)
     load_s = time.time() - t0

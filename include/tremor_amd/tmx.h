@@ -49,12 +49,20 @@ enum tmx_precision {
     TMX_PREC_FP16 = 2    /* Hankel GEMM on MFMA, FP16 operands, FP32 accumulation */
 };
 
+/* Which templates take the FFT (MASS overlap-save) form of the candidate pass. */
+enum tmx_fft_mode {
+    TMX_FFT_AUTO = 0,    /* cost model per template (long templates), as the reference's adaptive refinement */
+    TMX_FFT_OFF = 1,     /* every template on the inner-product form chosen by `precision` */
+    TMX_FFT_ALWAYS = 2   /* every template on the FFT form */
+};
+
 typedef struct {
     int32_t device;          /* HIP device ordinal */
     int32_t precision;       /* enum tmx_precision */
     int64_t cand_capacity;   /* candidate slots per call; 0 = default (grown on demand) */
     int32_t seed_target;     /* kNN bound seeding: target seed entries per template; 0 = default */
     int32_t verbose;         /* 1 = print per-phase timings to stderr */
+    int32_t fft_mode;        /* enum tmx_fft_mode; ignored for TMX_PREC_FP32 (plain scan) */
 } tmx_cfg;
 
 typedef struct {
@@ -67,8 +75,10 @@ typedef struct {
     int64_t candidates;      /* windows recomputed exactly */
     int32_t retries;         /* candidate-buffer overflows resolved by a tighter bound or a bigger buffer */
     float max_abs_rho_err;   /* max |rho_fast - rho_exact| over the verified candidates */
-    double kernel_ms_main;   /* device time of the main inner-product launch (HIP events on the context stream) */
-    double flops_main;       /* algorithmic flops of that launch: sum over templates of 2 m (n - m + 1) */
+    double kernel_ms_main;   /* device time of the main candidate pass (HIP events on the context stream) */
+    double flops_main;       /* algorithmic flops of that pass: sum over templates of 2 m (n - m + 1) */
+    int32_t fft_templates;   /* templates that took the FFT form */
+    int32_t fft_block;       /* FFT length B of those templates (0: none) */
 } tmx_stats;
 
 typedef struct {
@@ -84,6 +94,7 @@ int tmx_create(tmx_ctx **out, const tmx_cfg *cfg);
 void tmx_destroy(tmx_ctx *ctx);
 const char *tmx_last_error(const tmx_ctx *ctx);
 int tmx_set_precision(tmx_ctx *ctx, int32_t precision);
+int tmx_set_fft_mode(tmx_ctx *ctx, int32_t mode);
 
 /* Copies the waveform to the device and builds its FP64 prefix sums of x and
  * x^2 (missing samples as 0).  Replaces any waveform loaded before. */

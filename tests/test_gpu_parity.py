@@ -186,3 +186,66 @@ def test_rho_error_reported_and_small(ctxs):
         print("  stats", st, flush=True)
         assert 0.0 <= st["max_abs_rho_err"] < lim
         assert st["candidates"] >= 5
+
+
+@pytest.fixture(scope="module")
+def fft_ctx():
+    from tremor_amd.capi import Context
+    c = Context(precision="bf16x3", fft="always")
+    yield c
+    c.close()
+
+
+def test_fft_form_validate_recipe(fft_ctx, oracle):
+    from tremor_amd import synth
+    w = synth.validate_recipe()
+    fft_ctx.load_waveform(w.x)
+    for merge in (0, 16):
+        _log("fft-validate", merge=merge)
+        pos, d, st = fft_ctx.knn(w.templates, 5, merge)
+        assert st["fft_templates"] == len(w.templates)
+        po, do, _ = oracle.scan_knn(w.x, w.templates, 5, merge)
+        assert np.array_equal(pos, po)
+        assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+        r = fft_ctx.threshold(w.templates, 0.4, merge)
+        tt, tp, td = oracle.scan_threshold(w.x, w.templates, 0.4, merge)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_fft_form_uneven_long_templates(fft_ctx, oracle):
+    from tremor_amd import synth
+    x, onsets, rng = _wave(400_000, 11, events=30)
+    lengths = [100, 700, 3000, 9000, 20000]
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    x[rng.integers(0, x.size, size=20)] = synth.MISSING
+    fft_ctx.load_waveform(x)
+    _log("fft-uneven-knn")
+    pos, d, st = fft_ctx.knn(tm, 10, 0)
+    print("  stats", st, flush=True)
+    po, do, _ = oracle.scan_knn(x, tm, 10, 0)
+    assert np.array_equal(pos, po)
+    assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+    assert st["fft_block"] >= 4 * 20000
+    assert st["max_abs_rho_err"] < 1e-2
+    _log("fft-uneven-thr")
+    r = fft_ctx.threshold(tm, 0.5, 50)
+    tt, tp, td = oracle.scan_threshold(x, tm, 0.5, 50)
+    assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+    assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_cost_model_routes_long_templates(oracle):
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(300_000, 12, events=20)
+    tm = synth.cut_templates(x, onsets, [256, 512, 30000], rng)
+    with Context(precision="bf16x3", fft="auto") as ctx:
+        ctx.load_waveform(x)
+        _log("auto-route")
+        pos, d, st = ctx.knn(tm, 5, 0)
+        print("  stats", st, flush=True)
+        assert st["fft_templates"] == 1
+        po, do, _ = oracle.scan_knn(x, tm, 5, 0)
+        assert np.array_equal(pos, po)
+        assert np.array_equal(d.view(np.uint32), do.view(np.uint32))

@@ -17,9 +17,11 @@ PREC_FP32 = 0
 PREC_BF16X3 = 1
 PREC_FP16 = 2
 PRECISIONS = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp16": PREC_FP16}
+FFT_MODES = {"auto": 0, "off": 1, "always": 2}
 
 EXPORTS = [
-    "tmx_cfg_default", "tmx_create", "tmx_destroy", "tmx_last_error", "tmx_set_precision", "tmx_load_waveform",
+    "tmx_cfg_default", "tmx_create", "tmx_destroy", "tmx_last_error", "tmx_set_precision", "tmx_set_fft_mode",
+    "tmx_load_waveform",
     "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch", "tmx_window_stats",
     "tmx_prepare_template", "tmx_sync",
 ]
@@ -27,7 +29,7 @@ EXPORTS = [
 
 class Cfg(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32), ("cand_capacity", ctypes.c_int64),
-                ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+                ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32), ("fft_mode", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -35,7 +37,8 @@ class Stats(ctypes.Structure):
                 ("seconds_main", ctypes.c_double), ("seconds_verify", ctypes.c_double),
                 ("seconds_select", ctypes.c_double), ("comparisons", ctypes.c_int64),
                 ("candidates", ctypes.c_int64), ("retries", ctypes.c_int32), ("max_abs_rho_err", ctypes.c_float),
-                ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double)]
+                ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double),
+                ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -63,6 +66,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.tmx_last_error.argtypes = [ctypes.c_void_p]
     L.tmx_last_error.restype = ctypes.c_char_p
     L.tmx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.tmx_set_fft_mode.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.tmx_load_waveform.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint64]
     L.tmx_waveform_length.argtypes = [ctypes.c_void_p]
     L.tmx_waveform_length.restype = ctypes.c_uint64
@@ -119,7 +123,7 @@ class Context:
     """One device context: a resident waveform and search calls on it."""
 
     def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
-                 seed_target: int = 0, verbose: bool = False):
+                 seed_target: int = 0, verbose: bool = False, fft: str = "auto"):
         L = lib()
         cfg = Cfg()
         L.tmx_cfg_default(ctypes.byref(cfg))
@@ -128,6 +132,7 @@ class Context:
         cfg.cand_capacity = cand_capacity
         cfg.seed_target = seed_target
         cfg.verbose = 1 if verbose else 0
+        cfg.fft_mode = FFT_MODES[fft]
         h = ctypes.c_void_p()
         rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:
@@ -159,6 +164,9 @@ class Context:
     def set_precision(self, precision):
         p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self._check(self._L.tmx_set_precision(self._h, p))
+
+    def set_fft_mode(self, mode: str):
+        self._check(self._L.tmx_set_fft_mode(self._h, FFT_MODES[mode]))
 
     def load_waveform(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)

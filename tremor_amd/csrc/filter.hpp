@@ -3,7 +3,7 @@
 // For window j of template t (length m) an inner-product pass delivers
 //   dt = sum_i q_i * (x0[j+i] - c)   (c: the pass's centring constant)
 // computed in reduced or FP32 arithmetic with |dt - dt_exact| <= eta *
-// sqrt(Q2) * sqrt(sum_i (x0[j+i] - c)^2).  From the FP64 prefix sums this
+// sqrt(Q2) * sqrt(sum_i (x0[j+i] - c)^2) + eabs.  From the FP64 prefix sums this
 // function derives an interval [lo, hi] that contains the reference's FP32
 // distance d_ref (ZnormSearch.hpp:103-165 with the statistics of
 // SkipSequential.cpp:157-179):
@@ -32,7 +32,7 @@ struct DistInterval {
 };
 
 __device__ inline DistInterval dist_interval(const Waveform &w, uint64_t j, int m, double Q2, double S, float c,
-                                             double dt, double eta)
+                                             double dt, double eta, double eabs = 0.0)
 {
     constexpr double u = 5.9604644775390625e-08;   // 2^-24
     constexpr double u64 = 1.1102230246251565e-16;  // 2^-53
@@ -65,7 +65,7 @@ __device__ inline DistInterval dist_interval(const Waveform &w, uint64_t j, int 
     const double emu = eS1 / md + 2.0 * u64 * fabs(mu);
     const double dmc = fabs(mu - static_cast<double>(c)) + emu;
     const double St2 = (V + eV) + md * dmc * dmc;  // >= sum (x0 - c)^2 over the window
-    const double Edot = eta * sqrt(Q2) * sqrt(St2) * (1.0 + 1e-6);
+    const double Edot = eta * sqrt(Q2) * sqrt(St2) * (1.0 + 1e-6) + eabs;
     const double N = dt - (mu - static_cast<double>(c)) * S;
     const double eN = Edot + emu * fabs(S) + 4.0 * u64 * (fabs(dt) + fabs((mu - c) * S));
     const double nhi = N + eN, nlo = N - eN;

@@ -369,7 +369,6 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
             const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
             const double Q2 = tmpl[t].Q2;
             const double md = static_cast<double>(m);
-            const double inv_m = 1.0 / md;
             const double eta = eta_mfma(PREC, tmpl[t].kpad);
             // FP16 floor terms: staged |x~| * 2^-ex below 2^-14 and |q| * 2^7
             // below 2^-14 may be flushed.  x side: sum |q| 2^-14 2^ex <=

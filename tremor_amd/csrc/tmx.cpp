@@ -24,6 +24,8 @@
 #include <string>
 #include <vector>
 
+#include <hipfft/hipfft.h>
+
 #include "filter.hpp"
 #include "tmx_internal.hpp"
 
@@ -49,6 +51,24 @@ int mfma_group_size();
 uint64_t mfma_block_windows();
 int64_t mfma_frag_count(int kpad);
 int mfma_kpad(int m);
+// fft.hip
+struct FftPair {
+    uint32_t t;
+    uint32_t p;
+};
+hipError_t launch_piece_input(const float *x, uint64_t n, int step, int B, uint64_t p0, int cnt, const float4 *pinfo,
+                              float *out, double *norm2, hipStream_t stream);
+hipError_t launch_piece_norm(const double *norm2, float2 *pnorm, uint64_t P, hipStream_t stream);
+hipError_t launch_spec_max(const float2 *spec, int rows, int bins, float *out, int stride_out, hipStream_t stream);
+hipError_t launch_tmpl_input(const TmplDev *tmpl, const int32_t *ids, int count, int B, const float *qz, float *out,
+                             hipStream_t stream);
+hipError_t launch_product(const FftPair *pairs, int npairs, const float2 *spectra, const float2 *qspec,
+                          const int32_t *slot, int bins, float2 *Y, hipStream_t stream);
+hipError_t launch_fft_filter(const Waveform &w, const TmplDev *tmpl, const FftPair *pairs, int npairs, const float *y,
+                             int B, int step, const float2 *rb, int rb_len, const float4 *pinfo, const float2 *pnorm,
+                             const float *qmax, const int32_t *slot, double kfft, int mode, Cand *cand,
+                             float *cand_val, unsigned long long *cand_count, uint64_t cap, hipStream_t stream);
+double fft_error_factor(int B);
 // misc kernels
 hipError_t launch_zero_u64(unsigned long long *p, hipStream_t stream);
 hipError_t launch_rho_err(const Cand *cand, const float *est, const float *exact, uint64_t count,
@@ -119,6 +139,24 @@ struct tmx_ctx {
     uint64_t cand_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_pass_ms = 0.0f;
+    // FFT (overlap-save) form: one class (B, Lmax) of cached piece spectra
+    struct Fft {
+        int B = 0, Lmax = 0, step = 0, bins = 0;
+        uint64_t P = 0, n = 0;
+        DevMem spectra, pinfo, pnorm, rb, norm2, work_real, Y, y, pairs, qspec, qmax, qin, ids, slot;
+        hipfftHandle fwd = 0, inv = 0;
+        int fwd_batch = 0, inv_batch = 0;
+        void release_plans()
+        {
+            if (fwd)
+                (void)hipfftDestroy(fwd);
+            if (inv)
+                (void)hipfftDestroy(inv);
+            fwd = inv = 0;
+            fwd_batch = inv_batch = 0;
+        }
+        ~Fft() { release_plans(); }
+    } fft;
 
     int fail(const std::string &msg, int code = -1)
     {
@@ -151,7 +189,41 @@ struct CallPlan {
     int64_t afrag_len = 0;          // 16-byte fragments
     int64_t comparisons = 0;
     double flops = 0.0;             // sum of 2 m (n - m + 1)
+    int Tg = 0;                     // templates [0, Tg) take the inner-product form, [Tg, T) the FFT form
+    int fft_L = 0;                  // longest FFT template
 };
+
+// Cost model (per template, seconds per window), in the role of the
+// reference's adaptive choice between scan and FFT (Tremor.cpp:2265-2286,
+// fft_fallback.hpp:55-60): the matrix-core form costs in proportion to the
+// padded template length plus a fixed epilogue, the FFT form in proportion
+// to B log2 B per piece of (B - Lmax + 1) windows.  Constants measured on
+// MI355X (matrix-core loop and epilogue of the split-BF16 / FP16 passes,
+// batched hipFFT c2r with the product and filter kernels); FFT iff it is
+// estimated below 0.9 x the inner-product form, as in the reference.
+double gemm_seconds_per_window(int precision, int kpad)
+{
+    const double per_k = precision == TMX_PREC_FP16 ? 2.2e-15 : 4.6e-15;
+    return per_k * kpad + 2.1e-12;
+}
+
+int fft_block_for(int L)
+{
+    int B = 1 << 15;
+    while (B < 4 * L)
+        B *= 2;
+    return B;
+}
+
+double fft_seconds_per_window(int L)
+{
+    const int B = fft_block_for(L);
+    double lg = 0.0;
+    while ((1 << static_cast<int>(lg)) < B)
+        lg += 1.0;
+    const double per_piece = 0.45e-12 * B * lg + 1.6e-12 * B;
+    return per_piece / static_cast<double>(B - L + 1);
+}
 
 int build_plan(tmx_ctx *ctx, const float *templates, const int32_t *lengths, const uint64_t *offsets, int count,
                CallPlan &plan, std::vector<float> &raw_packed)
@@ -193,13 +265,32 @@ int build_plan(tmx_ctx *ctx, const float *templates, const int32_t *lengths, con
     plan.raw_len = static_cast<int64_t>(raw_packed.size());
     plan.order_len = ooff;
     plan.afrag_len = aoff;
+    // Path choice: templates are sorted by length and the FFT form wins for
+    // long templates, so the FFT set is a suffix of the device order.
+    plan.Tg = count;
+    if (ctx->cfg.precision != TMX_PREC_FP32 && ctx->cfg.fft_mode != TMX_FFT_OFF) {
+        if (ctx->cfg.fft_mode == TMX_FFT_ALWAYS) {
+            plan.Tg = 0;
+        } else {
+            while (plan.Tg > 0) {
+                const TmplDev &t = plan.tmpl[plan.Tg - 1];
+                if (!(fft_seconds_per_window(t.m) < 0.9 * gemm_seconds_per_window(ctx->cfg.precision, t.kpad)))
+                    break;
+                plan.Tg--;
+            }
+        }
+    }
+    for (int d = plan.Tg; d < count; d++)
+        plan.fft_L = std::max(plan.fft_L, plan.tmpl[d].m);
     const int G = ctx->cfg.precision == TMX_PREC_FP32 ? 1 : mfma_group_size();
     plan.group_begin.clear();
-    for (int d = 0; d < count; d += G)
+    for (int d = 0; d < plan.Tg; d += G)
         plan.group_begin.push_back(d);
-    plan.group_begin.push_back(count);
+    plan.group_begin.push_back(plan.Tg);
     return 0;
 }
+
+int fft_prepare(tmx_ctx *ctx, const CallPlan &plan);
 
 // Uploads templates, prepares them on the device, packs MFMA operands.
 int upload_and_prepare(tmx_ctx *ctx, CallPlan &plan, const std::vector<float> &raw_packed)
@@ -223,11 +314,11 @@ int upload_and_prepare(tmx_ctx *ctx, CallPlan &plan, const std::vector<float> &r
             "prepare kernel");
     if (ctx->cfg.precision != TMX_PREC_FP32) {
         TMX_HIP(ctx->afrag.reserve(16 * static_cast<size_t>(plan.afrag_len + 64)), "hipMalloc MFMA operands");
-        TMX_HIP(launch_mfma_pack(ctx->tmpl.as<TmplDev>(), T, ctx->qz.as<float>(), ctx->cfg.precision, ctx->afrag.p,
-                                 ctx->stream),
+        TMX_HIP(launch_mfma_pack(ctx->tmpl.as<TmplDev>(), plan.Tg, ctx->qz.as<float>(), ctx->cfg.precision,
+                                 ctx->afrag.p, ctx->stream),
                 "MFMA operand packing");
     }
-    return 0;
+    return fft_prepare(ctx, plan);
 }
 
 int ensure_cand(tmx_ctx *ctx, uint64_t cap, int T)
@@ -252,8 +343,8 @@ int ensure_cand(tmx_ctx *ctx, uint64_t cap, int T)
 int ensure_block_prefix(tmx_ctx *ctx, const CallPlan &plan)
 {
     int kmax = 0;
-    for (const auto &t : plan.tmpl)
-        kmax = std::max(kmax, t.kpad);
+    for (int d = 0; d < plan.Tg; d++)
+        kmax = std::max(kmax, plan.tmpl[d].kpad);
     const int nw = static_cast<int>(mfma_block_windows());
     const int need = ((nw + kmax + 1023) / 1024) * 1024;
     if (ctx->rb_n == ctx->n && ctx->rb_len >= need)
@@ -270,11 +361,175 @@ int ensure_block_prefix(tmx_ctx *ctx, const CallPlan &plan)
     return 0;
 }
 
+// ---- FFT form ---------------------------------------------------------------
+
+constexpr int kFftPieceBatch = 64;  // pieces / templates per forward transform launch
+
+// Piece spectra of the current waveform for templates up to L samples:
+// B = 2^15 doubled until >= 4 L (fft_fallback.cpp:124-126), Lmax = B / 4,
+// step = B - Lmax + 1; per piece the centre, the rebased prefix pairs of the
+// level-1 filter, ||x~||_2 and ||X||_inf.
+int ensure_fft_class(tmx_ctx *ctx, int L)
+{
+    auto &f = ctx->fft;
+    const int B = fft_block_for(L);
+    if (f.n == ctx->n && f.B == B)
+        return 0;
+    f.n = 0;
+    if (f.B != B)
+        f.release_plans();
+    f.B = B;
+    f.Lmax = B / 4;
+    f.step = B - f.Lmax + 1;
+    f.bins = B / 2 + 1;
+    f.P = (ctx->n + f.step - 1) / f.step;
+    const uint64_t Palloc = ((f.P + kFftPieceBatch - 1) / kFftPieceBatch) * kFftPieceBatch;
+    TMX_HIP(f.spectra.reserve(sizeof(float2) * Palloc * f.bins), "hipMalloc FFT spectra");
+    TMX_HIP(f.pinfo.reserve(sizeof(float4) * f.P), "hipMalloc FFT pieces");
+    TMX_HIP(f.pnorm.reserve(sizeof(float2) * f.P), "hipMalloc FFT pieces");
+    TMX_HIP(f.norm2.reserve(sizeof(double) * f.P), "hipMalloc FFT pieces");
+    TMX_HIP(f.rb.reserve(sizeof(float2) * f.P * static_cast<uint64_t>(B + 1)), "hipMalloc FFT prefix pairs");
+    TMX_HIP(f.work_real.reserve(sizeof(float) * static_cast<size_t>(kFftPieceBatch) * B), "hipMalloc FFT input");
+    if (!f.fwd) {
+        int nn = B;
+        if (hipfftPlanMany(&f.fwd, 1, &nn, nullptr, 1, B, nullptr, 1, f.bins, HIPFFT_R2C, kFftPieceBatch) !=
+            HIPFFT_SUCCESS)
+            return ctx->fail("hipFFT forward plan failed", -2);
+        f.fwd_batch = kFftPieceBatch;
+    }
+    hipfftSetStream(f.fwd, ctx->stream);
+    TMX_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->n, f.step, B + 1, f.pinfo.as<float4>(), f.rb.as<float2>(),
+                                ctx->stream),
+            "FFT prefix pairs");
+    TMX_HIP(hipMemsetAsync(f.norm2.p, 0, sizeof(double) * f.P, ctx->stream), "clear norms");
+    for (uint64_t p0 = 0; p0 < f.P; p0 += kFftPieceBatch) {
+        const int cnt = static_cast<int>(std::min<uint64_t>(kFftPieceBatch, f.P - p0));
+        TMX_HIP(launch_piece_input(ctx->x.as<float>(), ctx->n, f.step, B, p0, cnt, f.pinfo.as<float4>(),
+                                   f.work_real.as<float>(), f.norm2.as<double>(), ctx->stream),
+                "FFT piece input");
+        if (hipfftExecR2C(f.fwd, f.work_real.as<hipfftReal>(),
+                          reinterpret_cast<hipfftComplex *>(f.spectra.as<float2>() + p0 * f.bins)) != HIPFFT_SUCCESS)
+            return ctx->fail("hipFFT r2c of the pieces failed", -2);
+    }
+    TMX_HIP(launch_piece_norm(f.norm2.as<double>(), f.pnorm.as<float2>(), f.P, ctx->stream), "FFT piece norms");
+    TMX_HIP(launch_spec_max(f.spectra.as<float2>(), static_cast<int>(f.P), f.bins,
+                            reinterpret_cast<float *>(f.pnorm.as<float2>()) + 1, 2, ctx->stream),
+            "FFT spectral maxima");
+    TMX_HIP(hipStreamSynchronize(ctx->stream), "FFT spectra");
+    f.n = ctx->n;
+    return 0;
+}
+
+// Spectra of the call's FFT templates [Tg, T) (reversed, zero padded).
+int fft_prepare(tmx_ctx *ctx, const CallPlan &plan)
+{
+    const int F = plan.T - plan.Tg;
+    if (F <= 0)
+        return 0;
+    int rc = ensure_fft_class(ctx, plan.fft_L);
+    if (rc)
+        return rc;
+    auto &f = ctx->fft;
+    const int Falloc = ((F + kFftPieceBatch - 1) / kFftPieceBatch) * kFftPieceBatch;
+    TMX_HIP(f.qspec.reserve(sizeof(float2) * static_cast<size_t>(Falloc) * f.bins), "hipMalloc template spectra");
+    TMX_HIP(f.qmax.reserve(sizeof(float) * F), "hipMalloc template spectra");
+    TMX_HIP(f.qin.reserve(sizeof(float) * static_cast<size_t>(kFftPieceBatch) * f.B), "hipMalloc template input");
+    std::vector<int32_t> ids(F), slot(plan.T, -1);
+    for (int i = 0; i < F; i++) {
+        ids[i] = plan.Tg + i;
+        slot[plan.Tg + i] = i;
+    }
+    TMX_HIP(f.ids.reserve(sizeof(int32_t) * F), "hipMalloc FFT ids");
+    TMX_HIP(f.slot.reserve(sizeof(int32_t) * plan.T), "hipMalloc FFT ids");
+    TMX_HIP(hipMemcpyAsync(f.ids.p, ids.data(), sizeof(int32_t) * F, hipMemcpyHostToDevice, ctx->stream), "copy ids");
+    TMX_HIP(hipMemcpyAsync(f.slot.p, slot.data(), sizeof(int32_t) * plan.T, hipMemcpyHostToDevice, ctx->stream),
+            "copy slots");
+    hipfftSetStream(f.fwd, ctx->stream);
+    for (int i0 = 0; i0 < F; i0 += kFftPieceBatch) {
+        const int cnt = std::min(kFftPieceBatch, F - i0);
+        TMX_HIP(launch_tmpl_input(ctx->tmpl.as<TmplDev>(), f.ids.as<int32_t>() + i0, cnt, f.B, ctx->qz.as<float>(),
+                                  f.qin.as<float>(), ctx->stream),
+                "FFT template input");
+        if (hipfftExecR2C(f.fwd, f.qin.as<hipfftReal>(),
+                          reinterpret_cast<hipfftComplex *>(f.qspec.as<float2>() + static_cast<size_t>(i0) * f.bins)) !=
+            HIPFFT_SUCCESS)
+            return ctx->fail("hipFFT r2c of the templates failed", -2);
+    }
+    TMX_HIP(launch_spec_max(f.qspec.as<float2>(), F, f.bins, f.qmax.as<float>(), 1, ctx->stream),
+            "template spectral maxima");
+    return 0;
+}
+
+// The FFT form of a pass over every `stride`-th piece.
+int run_fft_pass(tmx_ctx *ctx, const CallPlan &plan, int mode, int stride)
+{
+    const int F = plan.T - plan.Tg;
+    if (F <= 0)
+        return 0;
+    auto &f = ctx->fft;
+    std::vector<FftPair> pairs;
+    for (uint64_t p = 0; p < f.P; p += static_cast<uint64_t>(stride))
+        for (int i = 0; i < F; i++) {
+            const int t = plan.Tg + i;
+            const uint64_t m = static_cast<uint64_t>(plan.tmpl[t].m);
+            if (m <= ctx->n && p * f.step < ctx->n - m + 1)
+                pairs.push_back(FftPair{static_cast<uint32_t>(t), static_cast<uint32_t>(p)});
+        }
+    if (pairs.empty())
+        return 0;
+    const int NB = static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, (512ll << 20) / (12ll * f.B))));
+    if (!f.inv || f.inv_batch != NB) {
+        if (f.inv)
+            (void)hipfftDestroy(f.inv);
+        f.inv = 0;
+        int nn = f.B;
+        if (hipfftPlanMany(&f.inv, 1, &nn, nullptr, 1, f.bins, nullptr, 1, f.B, HIPFFT_C2R, NB) != HIPFFT_SUCCESS)
+            return ctx->fail("hipFFT inverse plan failed", -2);
+        f.inv_batch = NB;
+    }
+    hipfftSetStream(f.inv, ctx->stream);
+    TMX_HIP(f.Y.reserve(sizeof(float2) * static_cast<size_t>(NB) * f.bins), "hipMalloc FFT products");
+    TMX_HIP(f.y.reserve(sizeof(float) * static_cast<size_t>(NB) * f.B), "hipMalloc FFT outputs");
+    TMX_HIP(f.pairs.reserve(sizeof(FftPair) * pairs.size()), "hipMalloc FFT pairs");
+    TMX_HIP(hipMemcpyAsync(f.pairs.p, pairs.data(), sizeof(FftPair) * pairs.size(), hipMemcpyHostToDevice,
+                           ctx->stream),
+            "copy FFT pairs");
+    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
+    const double kfft = fft_error_factor(f.B);
+    for (size_t b0 = 0; b0 < pairs.size(); b0 += NB) {
+        const int cnt = static_cast<int>(std::min<size_t>(NB, pairs.size() - b0));
+        const FftPair *pp = f.pairs.as<FftPair>() + b0;
+        TMX_HIP(launch_product(pp, cnt, f.spectra.as<float2>(), f.qspec.as<float2>(), f.slot.as<int32_t>(), f.bins,
+                               f.Y.as<float2>(), ctx->stream),
+                "FFT product");
+        if (hipfftExecC2R(f.inv, reinterpret_cast<hipfftComplex *>(f.Y.as<float2>()), f.y.as<hipfftReal>()) !=
+            HIPFFT_SUCCESS)
+            return ctx->fail("hipFFT c2r failed", -2);
+        TMX_HIP(launch_fft_filter(w, ctx->tmpl.as<TmplDev>(), pp, cnt, f.y.as<float>(), f.B, f.step,
+                                  f.rb.as<float2>(), f.B + 1, f.pinfo.as<float4>(), f.pnorm.as<float2>(),
+                                  f.qmax.as<float>(), f.slot.as<int32_t>(), kfft, mode, ctx->cand.as<Cand>(),
+                                  ctx->cand_val.as<float>(), ctx->cand_count.as<unsigned long long>(), ctx->cand_cap,
+                                  ctx->stream),
+                "FFT filter");
+    }
+    return 0;
+}
+
+int fft_seed_stride(tmx_ctx *ctx, uint64_t target)
+{
+    const auto &f = ctx->fft;
+    if (f.P == 0)
+        return 1;
+    const uint64_t chunks = (static_cast<uint64_t>(f.step) + 4095) / 4096;
+    const uint64_t s = (f.P * chunks) / std::max<uint64_t>(1, target);
+    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(s, 1u << 30)));
+}
+
 // One inner-product pass (seed or main) over every group; returns the entry
 // count through *count (may exceed the capacity).
-int run_pass(tmx_ctx *ctx, const CallPlan &plan, int mode, int block_stride, uint64_t *count)
+int run_pass(tmx_ctx *ctx, const CallPlan &plan, int mode, int block_stride, uint64_t *count, int fft_stride = 1)
 {
-    if (ctx->cfg.precision != TMX_PREC_FP32) {
+    if (ctx->cfg.precision != TMX_PREC_FP32 && plan.Tg > 0) {
         const int rc = ensure_block_prefix(ctx, plan);
         if (rc)
             return rc;
@@ -294,6 +549,11 @@ int run_pass(tmx_ctx *ctx, const CallPlan &plan, int mode, int block_stride, uin
                                  ctx->rb_len, mode, block_stride, ctx->cand.as<Cand>(), ctx->cand_val.as<float>(),
                                  ctx->cand_count.as<unsigned long long>(), ctx->cand_cap, ctx->stream),
                 "MFMA scan");
+    }
+    {
+        const int rc = run_fft_pass(ctx, plan, mode, fft_stride);
+        if (rc)
+            return rc;
     }
     TMX_HIP(hipEventRecord(ctx->ev1, ctx->stream), "event");
     unsigned long long c = 0;
@@ -389,6 +649,7 @@ int tmx_cfg_default(tmx_cfg *cfg)
     cfg->cand_capacity = 0;
     cfg->seed_target = 0;
     cfg->verbose = 0;
+    cfg->fft_mode = TMX_FFT_AUTO;
     return 0;
 }
 
@@ -402,7 +663,8 @@ int tmx_create(tmx_ctx **out, const tmx_cfg *cfg)
         ctx->cfg = *cfg;
     else
         tmx_cfg_default(&ctx->cfg);
-    if (ctx->cfg.precision < TMX_PREC_FP32 || ctx->cfg.precision > TMX_PREC_FP16) {
+    if (ctx->cfg.precision < TMX_PREC_FP32 || ctx->cfg.precision > TMX_PREC_FP16 ||
+        ctx->cfg.fft_mode < TMX_FFT_AUTO || ctx->cfg.fft_mode > TMX_FFT_ALWAYS) {
         delete ctx;
         return -1;
     }
@@ -440,6 +702,16 @@ void tmx_destroy(tmx_ctx *ctx)
 
 const char *tmx_last_error(const tmx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int tmx_set_fft_mode(tmx_ctx *ctx, int32_t mode)
+{
+    if (!ctx)
+        return -1;
+    if (mode < TMX_FFT_AUTO || mode > TMX_FFT_ALWAYS)
+        return ctx->fail("unknown FFT mode");
+    ctx->cfg.fft_mode = mode;
+    return 0;
+}
+
 int tmx_set_precision(tmx_ctx *ctx, int32_t precision)
 {
     if (!ctx)
@@ -463,6 +735,7 @@ int tmx_load_waveform(tmx_ctx *ctx, const float *x, uint64_t n)
     const size_t pad = 1 << 20;
     ctx->n = 0;
     ctx->rb_n = 0;
+    ctx->fft.n = 0;
     TMX_HIP(ctx->x.reserve(sizeof(float) * (n + pad)), "hipMalloc waveform");
     TMX_HIP(ctx->p.reserve(sizeof(double2) * (n + 1)), "hipMalloc prefix sums");
     const size_t wl = prefix_work_len(n);
@@ -517,7 +790,8 @@ int tmx_knn(tmx_ctx *ctx, const float *templates, const int32_t *lengths, const 
         stride = 1;
     double t1 = now_s();
     uint64_t nseed = 0;
-    if ((rc = run_pass(ctx, plan, kModeSeed, static_cast<int>(std::min<uint64_t>(stride, 1u << 30)), &nseed)))
+    if ((rc = run_pass(ctx, plan, kModeSeed, static_cast<int>(std::min<uint64_t>(stride, 1u << 30)), &nseed,
+                       fft_seed_stride(ctx, target))))
         return rc;
     if (nseed > ctx->cand_cap)
         nseed = ctx->cand_cap;  // any subset of seed entries still gives a valid bound
@@ -535,6 +809,8 @@ int tmx_knn(tmx_ctx *ctx, const float *templates, const int32_t *lengths, const 
         return rc;
     st.kernel_ms_main = ctx->last_pass_ms;
     st.flops_main = plan.flops;
+    st.fft_templates = plan.T - plan.Tg;
+    st.fft_block = plan.T > plan.Tg ? ctx->fft.B : 0;
     double t3 = now_s();
     st.seconds_main = t3 - t2;
     st.candidates = static_cast<int64_t>(ncand);
@@ -612,6 +888,8 @@ int tmx_threshold(tmx_ctx *ctx, const float *templates, const int32_t *lengths, 
         return rc;
     st.kernel_ms_main = ctx->last_pass_ms;
     st.flops_main = plan.flops;
+    st.fft_templates = plan.T - plan.Tg;
+    st.fft_block = plan.T > plan.Tg ? ctx->fft.B : 0;
     const double t3 = now_s();
     st.seconds_main = t3 - t2;
     st.candidates = static_cast<int64_t>(ncand);
