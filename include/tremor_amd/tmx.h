@@ -12,8 +12,9 @@
  *   - Every call returns 0 on success, > 0 for a recoverable condition,
  *     < 0 for an error; tmx_last_error() then describes it.  No exceptions
  *     cross this boundary.
- *   - Calls on one context are serialised by the caller; a context owns one
- *     HIP stream on one device.
+ *   - Synchronous calls on one context are serialised by the context; they
+ *     run on the context's own HIP stream on one device.  The serving loop
+ *     (below) adds one stream per lane.
  *   - Distances are squared z-normalised Euclidean distances d = 2m(1 - rho)
  *     (ZnormSearch.hpp:23-31).  The missing-sample sentinel is -100000.0f
  *     (ZnormSearch.hpp:21): a window whose first sample is missing is never
@@ -129,8 +130,77 @@ int tmx_window_stats(tmx_ctx *ctx, int32_t m, uint64_t first, uint64_t count, fl
  * them, computed on the device.  order needs m / 32 slots. */
 int tmx_prepare_template(tmx_ctx *ctx, const float *tmpl, int32_t m, float *out_z, int32_t *out_order);
 
-/* Waits for all device work of the context. */
+/* Waits for all device work of the context's synchronous calls. */
 int tmx_sync(tmx_ctx *ctx);
+
+/* ---- Serving loop ---------------------------------------------------------
+ * No counterpart in the reference (its searches are batch jobs); the boundary
+ * follows its search entry points (tmx_knn / tmx_threshold above) per request.
+ *
+ * The waveform and its statistics stay resident; requests (one template each,
+ * kNN or threshold, any length) are submitted from any thread and queue until
+ * a lane is free.  A free lane takes the waiting requests in arrival order, as
+ * many as fit the batch size and the memory budget, and runs them as one
+ * batch on its own HIP stream; batches of different lanes overlap on the
+ * device (continuous batching).  A request's answer does not depend on the
+ * batch it ran in: every answer is the exact reference answer.
+ * While the loop runs, tmx_load_waveform is refused; the synchronous calls
+ * remain available (they run on their own stream). */
+
+enum tmx_request_kind {
+    TMX_REQ_KNN = 0,
+    TMX_REQ_THRESHOLD = 1
+};
+
+typedef struct {
+    int32_t lanes;           /* batches in flight at once, one stream each (default 2, at most 8) */
+    int32_t max_batch;       /* requests per batch (default 64) */
+    int64_t mem_budget;      /* HBM bytes admitted requests may hold; 0 = half the free memory at start */
+    int32_t max_template;    /* longest template accepted (default 65536) */
+    int32_t reserved;
+} tmx_server_cfg;
+
+typedef struct {
+    int32_t kind;            /* enum tmx_request_kind */
+    int32_t m;               /* template length */
+    const float *tmpl;       /* raw template samples, copied by tmx_submit */
+    int32_t k;               /* kNN: answers wanted, 1..4096 */
+    float rho;               /* threshold: correlation threshold */
+    int32_t merge_offset;    /* as in tmx_knn / tmx_threshold */
+    int32_t reserved;
+} tmx_request;
+
+typedef struct {
+    int32_t status;          /* 0 answered; < 0 failed (error says why) */
+    int32_t kind;
+    int64_t count;           /* kNN: k slots (unfilled (0, FLT_MAX)); threshold: hit rows by position */
+    uint64_t *pos;           /* window starts (library-owned until tmx_result_free) */
+    float *d;                /* squared z-normalised distances (reference FP32) */
+    double submit_s;         /* steady-clock seconds at submission */
+    double start_s;          /* its batch started */
+    double done_s;           /* its answer was ready */
+    int64_t batch;           /* batch sequence number */
+    int32_t batch_size;      /* requests in that batch */
+    int32_t lane;            /* lane that ran it */
+    char error[160];
+} tmx_result;
+
+int tmx_server_cfg_default(tmx_server_cfg *cfg);
+/* Starts the lanes (worker threads).  The derived structures the requests
+ * will need are built on first use. */
+int tmx_server_start(tmx_ctx *ctx, const tmx_server_cfg *cfg);
+/* Answers every request submitted so far, then stops the lanes.  Answers not
+ * yet collected stay available to tmx_poll / tmx_wait. */
+int tmx_server_stop(tmx_ctx *ctx);
+/* Thread-safe.  Returns a ticket > 0, or < 0 if the request is invalid or
+ * the loop is not running. */
+int64_t tmx_submit(tmx_ctx *ctx, const tmx_request *req);
+/* 0: answered, *out filled and the ticket retired; 1: pending; < 0: unknown
+ * ticket.  Thread-safe. */
+int tmx_poll(tmx_ctx *ctx, int64_t ticket, tmx_result *out);
+/* As tmx_poll, waiting up to timeout_s seconds (< 0: no limit); 1 on timeout. */
+int tmx_wait(tmx_ctx *ctx, int64_t ticket, double timeout_s, tmx_result *out);
+void tmx_result_free(tmx_result *res);
 
 #ifdef __cplusplus
 }

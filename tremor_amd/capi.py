@@ -24,7 +24,12 @@ EXPORTS = [
     "tmx_load_waveform",
     "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch", "tmx_window_stats",
     "tmx_prepare_template", "tmx_sync",
+    "tmx_server_cfg_default", "tmx_server_start", "tmx_server_stop", "tmx_submit", "tmx_poll", "tmx_wait",
+    "tmx_result_free",
 ]
+
+REQ_KNN = 0
+REQ_THRESHOLD = 1
 
 
 class Cfg(ctypes.Structure):
@@ -48,6 +53,25 @@ class Hits(ctypes.Structure):
     _fields_ = [("count", ctypes.c_int64), ("tmpl", ctypes.POINTER(ctypes.c_int32)),
                 ("counter", ctypes.POINTER(ctypes.c_int64)), ("pos", ctypes.POINTER(ctypes.c_uint64)),
                 ("d", ctypes.POINTER(ctypes.c_float))]
+
+
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("lanes", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("mem_budget", ctypes.c_int64),
+                ("max_template", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("m", ctypes.c_int32), ("tmpl", ctypes.POINTER(ctypes.c_float)),
+                ("k", ctypes.c_int32), ("rho", ctypes.c_float), ("merge_offset", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("kind", ctypes.c_int32), ("count", ctypes.c_int64),
+                ("pos", ctypes.POINTER(ctypes.c_uint64)), ("d", ctypes.POINTER(ctypes.c_float)),
+                ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("batch", ctypes.c_int64), ("batch_size", ctypes.c_int32), ("lane", ctypes.c_int32),
+                ("error", ctypes.c_char * 160)]
 
 
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -80,6 +104,15 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.tmx_window_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64, _f32p, _f32p]
     L.tmx_prepare_template.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int32, _f32p, _i32p]
     L.tmx_sync.argtypes = [ctypes.c_void_p]
+    L.tmx_server_cfg_default.argtypes = [ctypes.POINTER(ServerCfg)]
+    L.tmx_server_start.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerCfg)]
+    L.tmx_server_stop.argtypes = [ctypes.c_void_p]
+    L.tmx_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request)]
+    L.tmx_submit.restype = ctypes.c_int64
+    L.tmx_poll.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(Result)]
+    L.tmx_wait.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(Result)]
+    L.tmx_result_free.argtypes = [ctypes.POINTER(Result)]
+    L.tmx_result_free.restype = None
     return L
 
 
@@ -231,3 +264,55 @@ class Context:
 
     def sync(self):
         self._check(self._L.tmx_sync(self._h))
+
+    # ---- serving loop ----
+
+    def serve_start(self, lanes: int = 2, max_batch: int = 64, mem_budget: int = 0, max_template: int = 65536):
+        c = ServerCfg()
+        self._L.tmx_server_cfg_default(ctypes.byref(c))
+        c.lanes, c.max_batch, c.mem_budget, c.max_template = lanes, max_batch, mem_budget, max_template
+        self._check(self._L.tmx_server_start(self._h, ctypes.byref(c)))
+
+    def serve_stop(self):
+        self._check(self._L.tmx_server_stop(self._h))
+
+    def submit(self, template, kind: str = "knn", k: int = 10, rho: float = 0.8, merge: int = 0) -> int:
+        """Queues one request; returns its ticket.  kind: 'knn' or 'threshold'."""
+        t = np.ascontiguousarray(template, dtype=np.float32)
+        r = Request()
+        r.kind = REQ_KNN if kind == "knn" else REQ_THRESHOLD
+        r.m = t.size
+        r.tmpl = _p(t, _f32p)
+        r.k = k
+        r.rho = rho
+        r.merge_offset = merge
+        ticket = self._L.tmx_submit(self._h, ctypes.byref(r))
+        if ticket <= 0:
+            raise TmxError(self._L.tmx_last_error(self._h).decode())
+        return int(ticket)
+
+    def _collect(self, rc, res):
+        if rc < 0:
+            raise TmxError("unknown ticket")
+        if rc == 1:
+            return None
+        try:
+            n = int(res.count)
+            out = {"status": int(res.status), "error": res.error.decode(errors="replace"),
+                   "pos": np.ctypeslib.as_array(res.pos, (n,)).copy() if n else np.zeros(0, np.uint64),
+                   "d": np.ctypeslib.as_array(res.d, (n,)).copy() if n else np.zeros(0, np.float32),
+                   "submit_s": res.submit_s, "start_s": res.start_s, "done_s": res.done_s,
+                   "batch": int(res.batch), "batch_size": int(res.batch_size), "lane": int(res.lane)}
+        finally:
+            self._L.tmx_result_free(ctypes.byref(res))
+        return out
+
+    def poll(self, ticket: int):
+        """The answer as a dict, or None while pending."""
+        res = Result()
+        return self._collect(self._L.tmx_poll(self._h, ticket, ctypes.byref(res)), res)
+
+    def wait(self, ticket: int, timeout: float = -1.0):
+        """Waits for the answer (timeout < 0: no limit); None on timeout."""
+        res = Result()
+        return self._collect(self._L.tmx_wait(self._h, ticket, float(timeout), ctypes.byref(res)), res)

@@ -1,0 +1,230 @@
+// Context internals shared by the synchronous C-ABI (tmx.cpp) and the
+// serving loop (serve.cpp).
+//
+// A context holds the resident waveform and the structures derived from it
+// (FP64 prefix sums, block-rebased FP32 prefix pairs, FFT piece spectra) and
+// any number of lanes.  A lane is one HIP stream with its own per-batch work
+// buffers; it runs one batch of templates at a time.  The synchronous calls
+// use the context's direct lane, the serving loop one lane per worker thread,
+// so batches of different lanes overlap on the device.
+//
+// Resident structures are guarded by a reader/writer lock: a batch holds it
+// shared from start to end, a (re)build of a derived structure holds it
+// exclusively, so nothing a running batch reads is ever reallocated.
+#pragma once
+
+#include <hipfft/hipfft.h>
+
+#include <chrono>
+#include <cstddef>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
+#include <string>
+#include <vector>
+
+#include "tmx_internal.hpp"
+
+namespace tmx {
+
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    ~DevMem() { release(); }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t reserve(size_t want)
+    {
+        if (want <= bytes)
+            return hipSuccess;
+        release();
+        const size_t b = want < 256 ? 256 : want;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess)
+            bytes = b;
+        else
+            p = nullptr;
+        return e;
+    }
+    template <class T>
+    T *as() const
+    {
+        return static_cast<T *>(p);
+    }
+};
+
+inline double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// One stream and its per-batch work buffers (grown on demand, reused).
+struct Lane {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_pass_ms = 0.0f;
+    std::string err;
+    // templates of the batch
+    DevMem raw, qz, order, tmpl, groups, afrag;
+    // candidates, grouping, results
+    DevMem cand, cand_val, cand_d, cand_count;
+    DevMem counts, offsets, cursor, keys_tmp, keys, sort_tmp;
+    DevMem out_pos, out_d, errbits;
+    uint64_t cand_cap = 0;
+    // FFT form: template spectra, products, inverse transforms
+    DevMem Y, y, pairs, qspec, qmax, qin, ids, slot;
+    hipfftHandle tfwd = 0, inv = 0;
+    int tfwd_B = 0, inv_B = 0, inv_batch = 0;
+
+    Lane() = default;
+    Lane(const Lane &) = delete;
+    Lane &operator=(const Lane &) = delete;
+    ~Lane() { destroy(); }
+    hipError_t init(int dev)
+    {
+        device = dev;
+        hipError_t e = hipSetDevice(dev);
+        if (e == hipSuccess)
+            e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+        if (e == hipSuccess)
+            e = hipEventCreate(&ev0);
+        if (e == hipSuccess)
+            e = hipEventCreate(&ev1);
+        return e;
+    }
+    void destroy()
+    {
+        if (stream) {
+            (void)hipSetDevice(device);
+            (void)hipStreamSynchronize(stream);
+        }
+        if (tfwd)
+            (void)hipfftDestroy(tfwd);
+        if (inv)
+            (void)hipfftDestroy(inv);
+        tfwd = inv = 0;
+        if (ev0)
+            (void)hipEventDestroy(ev0);
+        if (ev1)
+            (void)hipEventDestroy(ev1);
+        ev0 = ev1 = nullptr;
+        if (stream)
+            (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+    int fail(const std::string &msg, int code = -1)
+    {
+        err = msg;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char *what)
+    {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return -2;
+    }
+};
+
+// Piece spectra of the resident waveform for one FFT length B (overlap-save:
+// pieces of B samples every step = B - Lmax + 1, Lmax = B / 4).
+struct FftClass {
+    int B = 0, Lmax = 0, step = 0, bins = 0;
+    uint64_t P = 0;
+    DevMem spectra, pinfo, pnorm, rb, norm2, work_real;
+    hipfftHandle fwd = 0;
+    FftClass() = default;
+    FftClass(const FftClass &) = delete;
+    FftClass &operator=(const FftClass &) = delete;
+    ~FftClass()
+    {
+        if (fwd)
+            (void)hipfftDestroy(fwd);
+    }
+};
+
+// One template of a batch.
+struct Query {
+    const float *data;  // raw samples
+    int32_t m;
+    int32_t k;          // > 0: kNN with k answers; 0: threshold
+    int32_t merge;      // merge offset
+    float rho;          // threshold correlation (threshold templates)
+};
+
+// Its answer: kNN, k slots (unfilled (0, FLT_MAX)); threshold, the hits by
+// position after the merge.
+struct QueryOut {
+    std::vector<uint64_t> pos;
+    std::vector<float> d;
+};
+
+struct Server;
+struct ServerDelete {
+    void operator()(Server *s) const;  // serve.cpp
+};
+
+}  // namespace tmx
+
+struct tmx_ctx {
+    tmx_cfg cfg{};
+    std::mutex err_mu;
+    std::string err;
+    std::mutex call_mu;            // one synchronous call at a time
+    std::shared_mutex res_mu;      // resident structures (see above)
+    std::mutex gate_mu;            // taken before res_mu: a waiting writer holds it, so readers cannot starve it
+    tmx::Lane direct;              // lane of the synchronous calls
+    // resident waveform
+    tmx::DevMem x, p, prefix_work;
+    uint64_t n = 0;
+    // block-rebased FP32 prefix pairs of the matrix-core pass
+    tmx::DevMem rb, binfo;
+    int rb_len = 0;
+    uint64_t rb_n = 0;
+    // FFT piece spectra, one class per block length
+    std::vector<std::unique_ptr<tmx::FftClass>> fft;
+    // serving loop (null when not running)
+    std::unique_ptr<tmx::Server, tmx::ServerDelete> server;
+
+    int fail(const std::string &msg, int code = -1)
+    {
+        std::lock_guard<std::mutex> lk(err_mu);
+        err = msg;
+        return code;
+    }
+};
+
+namespace tmx {
+
+// Shared (reader) hold of the resident structures, through the gate.
+inline std::shared_lock<std::shared_mutex> read_resident(tmx_ctx *ctx)
+{
+    std::lock_guard<std::mutex> gate(ctx->gate_mu);
+    return std::shared_lock<std::shared_mutex>(ctx->res_mu);
+}
+
+// Exclusive (writer) hold: the gate stays held until the writer is done.
+struct WriteResident {
+    std::lock_guard<std::mutex> gate;
+    std::unique_lock<std::shared_mutex> lock;
+    explicit WriteResident(tmx_ctx *ctx) : gate(ctx->gate_mu), lock(ctx->res_mu) {}
+};
+
+// The whole search for one batch of templates on lane L: preparation, kNN
+// bound seeding, candidate pass, exact verification, selection.  out gets
+// one entry per query, in query order.  Thread-safe across lanes.
+int run_batch(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count, std::vector<QueryOut> &out,
+              tmx_stats *st);
+
+// Stops the serving loop (if running): every submitted request is finished.
+void server_shutdown(tmx_ctx *ctx);
+bool server_running(const tmx_ctx *ctx);
+
+}  // namespace tmx

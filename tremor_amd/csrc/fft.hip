@@ -23,11 +23,6 @@
 
 namespace tmx {
 
-struct FftPair {
-    uint32_t t;  // template (call index)
-    uint32_t p;  // piece
-};
-
 namespace {
 
 constexpr int kThreads = 256;
@@ -93,6 +88,13 @@ __global__ void k_spec_max(const float2 *__restrict__ spec, int bins, float *__r
     }
     if (threadIdx.x == 0)
         out[static_cast<uint64_t>(blockIdx.x) * stride_out] = red[0] * (1.0f + 1e-5f);
+}
+
+__global__ void k_zero_f64(double *__restrict__ p, uint64_t count)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < count)
+        p[i] = 0.0;
 }
 
 // ||x~_p||_2 from the accumulated squares (rounded up).
@@ -253,6 +255,15 @@ hipError_t launch_piece_input(const float *x, uint64_t n, int step, int B, uint6
 {
     const int gx = (B + kThreads * 16 - 1) / (kThreads * 16);
     hipLaunchKernelGGL(k_piece_input, dim3(gx, cnt), dim3(kThreads), 0, stream, x, n, step, B, p0, pinfo, out, norm2);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_f64(double *p, uint64_t count, hipStream_t stream)
+{
+    if (count == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_zero_f64, dim3(static_cast<unsigned>((count + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       stream, p, count);
     return hipGetLastError();
 }
 

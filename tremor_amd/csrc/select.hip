@@ -19,12 +19,14 @@ enum GroupPred : int {
     kPredAll = 0,        // every entry
     kPredHit = 1,        // exact distance <= the template's bound (threshold hits)
     kPredFinite = 2,     // value < +inf (seed entries)
+    kPredQuery = 3,      // per template: every entry for kNN, hits for threshold
 };
 
 // Sort keys: (value bits << 32) | window, or (window << 32) | value bits.
 enum GroupKey : int {
     kKeyValuePos = 0,
     kKeyPosValue = 1,
+    kKeyQuery = 2,       // per template: (value, pos) for kNN, (pos, value) for threshold
 };
 
 namespace {
@@ -37,6 +39,8 @@ __device__ inline bool pred_ok(int pred, const TmplDev *tmpl, uint32_t t, float 
         return v <= tmpl[t].bound;
     if (pred == kPredFinite)
         return v < INFINITY;
+    if (pred == kPredQuery)
+        return tmpl[t].k > 0 || v <= tmpl[t].bound;
     return true;
 }
 
@@ -95,7 +99,15 @@ __global__ void k_group_scatter(const Cand *__restrict__ cand, const float *__re
         return;
     const uint32_t slot = offsets[c.t] + atomicAdd(&cursor[c.t], 1u);
     const uint64_t vb = static_cast<uint64_t>(__float_as_uint(v));
-    keys[slot] = keykind == kKeyValuePos ? ((vb << 32) | c.j) : ((static_cast<uint64_t>(c.j) << 32) | vb);
+    const bool value_first = keykind == kKeyValuePos || (keykind == kKeyQuery && tmpl[c.t].k > 0);
+    keys[slot] = value_first ? ((vb << 32) | c.j) : ((static_cast<uint64_t>(c.j) << 32) | vb);
+}
+
+__global__ void k_zero_counts(uint32_t *__restrict__ counts, int T)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < T)
+        counts[i] = 0u;
 }
 
 __device__ inline float key_val(uint64_t k) { return __uint_as_float(static_cast<uint32_t>(k >> 32)); }
@@ -142,13 +154,17 @@ __device__ int greedy_separated(const uint64_t *keys, uint32_t cnt, int want, in
     return nk;
 }
 
-// Seed bound per template: one wavefront per template.
+// Seed bound per kNN template (k > 0): one wavefront per template; the
+// bound set holds k entries (merge 0) or 2k - 1 (SkipSequential.cpp:299).
 __global__ __launch_bounds__(64) void k_seed_bound(const uint64_t *__restrict__ keys,
-                                                   const uint32_t *__restrict__ offsets, TmplDev *tmpl, int want,
-                                                   int merge)
+                                                   const uint32_t *__restrict__ offsets, TmplDev *tmpl)
 {
     extern __shared__ uint64_t kept[];
     const int t = blockIdx.x;
+    const int k = tmpl[t].k, merge = tmpl[t].merge;
+    if (k <= 0)
+        return;
+    const int want = merge > 0 ? 2 * k - 1 : k;
     const uint32_t b = offsets[t], e = offsets[t + 1];
     float U = FLT_MAX;
     if (merge == 0) {
@@ -164,13 +180,18 @@ __global__ __launch_bounds__(64) void k_seed_bound(const uint64_t *__restrict__ 
         tmpl[t].bound = U < FLT_MAX ? U : FLT_MAX;
 }
 
-// Canonical kNN answer per template from (d, pos)-sorted exact candidates.
+// Canonical kNN answer per kNN template from (d, pos)-sorted exact
+// candidates, into k slots from the template's output offset.
 __global__ __launch_bounds__(64) void k_knn_select(const uint64_t *__restrict__ keys,
-                                                   const uint32_t *__restrict__ offsets, int k, int merge,
-                                                   uint64_t *__restrict__ out_pos, float *__restrict__ out_d)
+                                                   const uint32_t *__restrict__ offsets,
+                                                   const TmplDev *__restrict__ tmpl, uint64_t *__restrict__ out_pos,
+                                                   float *__restrict__ out_d)
 {
     extern __shared__ uint64_t kept[];
     const int t = blockIdx.x;
+    const int k = tmpl[t].k, merge = tmpl[t].merge;
+    if (k <= 0)
+        return;
     const uint32_t b = offsets[t], e = offsets[t + 1];
     int nk;
     if (merge == 0) {
@@ -182,7 +203,7 @@ __global__ __launch_bounds__(64) void k_knn_select(const uint64_t *__restrict__ 
     }
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += 64) {
-        const size_t o = static_cast<size_t>(t) * k + i;
+        const size_t o = static_cast<size_t>(tmpl[t].out_off) + i;
         if (i < nk) {
             out_pos[o] = key_pos(kept[i]);
             out_d[o] = key_val(kept[i]);
@@ -203,9 +224,9 @@ hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count,
                              uint64_t *keys_tmp, uint64_t *keys, void *sort_tmp, size_t sort_tmp_bytes,
                              uint64_t key_capacity, hipStream_t stream)
 {
-    hipError_t err = hipMemsetAsync(counts, 0, sizeof(uint32_t) * static_cast<size_t>(T), stream);
-    if (err != hipSuccess)
-        return err;
+    // Counters cleared by a kernel, not a memset node: the sequence stays
+    // capturable in a HIP graph.
+    hipLaunchKernelGGL(k_zero_counts, dim3((T + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, counts, T);
     const unsigned grid = static_cast<unsigned>((count + kThreads - 1) / kThreads);
     if (count > 0)
         hipLaunchKernelGGL(k_group_count, dim3(grid), dim3(kThreads), 0, stream, cand, val, count, tmpl, pred, counts);
@@ -213,7 +234,7 @@ hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count,
     if (count > 0)
         hipLaunchKernelGGL(k_group_scatter, dim3(grid), dim3(kThreads), 0, stream, cand, val, count, tmpl, pred,
                            keykind, offsets, cursor, keys_tmp);
-    err = hipGetLastError();
+    const hipError_t err = hipGetLastError();
     if (err != hipSuccess)
         return err;
     if (count == 0)
@@ -234,23 +255,25 @@ size_t group_sort_tmp_bytes(uint64_t count, int T)
     return bytes;
 }
 
-hipError_t launch_seed_bound(const uint64_t *keys, const uint32_t *offsets, TmplDev *tmpl, int T, int want,
-                             int merge, hipStream_t stream)
+// max_want: the largest bound-set size (2k - 1 or k) over the templates.
+hipError_t launch_seed_bound(const uint64_t *keys, const uint32_t *offsets, TmplDev *tmpl, int T, int max_want,
+                             hipStream_t stream)
 {
-    if (T <= 0)
+    if (T <= 0 || max_want <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_seed_bound, dim3(T), dim3(64), sizeof(uint64_t) * static_cast<size_t>(want), stream, keys,
-                       offsets, tmpl, want, merge);
+    hipLaunchKernelGGL(k_seed_bound, dim3(T), dim3(64), sizeof(uint64_t) * static_cast<size_t>(max_want), stream,
+                       keys, offsets, tmpl);
     return hipGetLastError();
 }
 
-hipError_t launch_knn_select(const uint64_t *keys, const uint32_t *offsets, int T, int k, int merge,
+// max_k: the largest k over the templates.
+hipError_t launch_knn_select(const uint64_t *keys, const uint32_t *offsets, const TmplDev *tmpl, int T, int max_k,
                              uint64_t *out_pos, float *out_d, hipStream_t stream)
 {
-    if (T <= 0)
+    if (T <= 0 || max_k <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_knn_select, dim3(T), dim3(64), sizeof(uint64_t) * static_cast<size_t>(k), stream, keys,
-                       offsets, k, merge, out_pos, out_d);
+    hipLaunchKernelGGL(k_knn_select, dim3(T), dim3(64), sizeof(uint64_t) * static_cast<size_t>(max_k), stream, keys,
+                       offsets, tmpl, out_pos, out_d);
     return hipGetLastError();
 }
 

@@ -29,7 +29,16 @@ struct TmplDev {
     double Q2;          // sum of q^2
     double S;           // sum of q
     float bound;        // candidate cut: threshold distance, or the kNN seed bound
+    int32_t k;          // kNN answers wanted (0: threshold template)
+    int32_t merge;      // exclusion zone of the answers (merge offset)
     int32_t pad0;
+    int64_t out_off;    // first kNN output slot of this template
+};
+
+// One (template, FFT piece) pair of the FFT form.
+struct FftPair {
+    uint32_t t;  // template (call index)
+    uint32_t p;  // piece
 };
 
 // One candidate window of one template.
