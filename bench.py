@@ -40,9 +40,16 @@ def _args():
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--scan-steps", type=int, default=1, help="steps of the plain FP32 scan (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
-    ap.add_argument("--config", default="c1", choices=["c1", "c4"],
-                    help="c1: batched uneven templates (the headline); c4: long templates, FFT vs GEMM forms")
+    ap.add_argument("--config", default="c1", choices=["c1", "c3", "c4"],
+                    help="c1: batched uneven templates (the headline); c3: serving loop latency; "
+                         "c4: long templates, FFT vs GEMM forms")
     ap.add_argument("--fft", default="auto", choices=["auto", "off", "always"])
+    ap.add_argument("--rate", type=float, default=200.0, help="c3: Poisson arrival rate, requests/s")
+    ap.add_argument("--duration", type=float, default=10.0, help="c3: seconds of arrivals (batched loop)")
+    ap.add_argument("--single-duration", type=float, default=3.0,
+                    help="c3: seconds of the same stream replayed one request per launch")
+    ap.add_argument("--lanes", type=int, default=2)
+    ap.add_argument("--max-batch", type=int, default=64)
     return ap.parse_args()
 
 
@@ -109,6 +116,97 @@ def bench_long(a):
     print(json.dumps(out), flush=True)
 
 
+def _serve(ctx, stream, lanes, max_batch, label):
+    """Replays the stream's arrival times against the serving loop; returns
+    latency percentiles and throughput from the library's own timestamps."""
+    import threading
+    ctx.serve_start(lanes=lanes, max_batch=max_batch)
+    tickets = [None] * len(stream)
+    t0 = time.monotonic()
+
+    def submitter():
+        for i, r in enumerate(stream):
+            dt = t0 + r["t"] - time.monotonic()
+            if dt > 0:
+                time.sleep(dt)
+            tickets[i] = ctx.submit(r["template"], r["kind"], r["k"], r["rho"], 0)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    res = []
+    last_print = time.monotonic()
+    for i in range(len(stream)):
+        while tickets[i] is None:
+            time.sleep(0.0005)
+        res.append(ctx.wait(tickets[i], timeout=600.0))
+        if time.monotonic() - last_print > 20:
+            print(f"[bench] {label}: {i + 1}/{len(stream)} answered", file=sys.stderr, flush=True)
+            last_print = time.monotonic()
+    th.join()
+    ctx.serve_stop()
+    bad = [r for r in res if r is None or r["status"] != 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} requests failed: {bad[0] and bad[0]['error']}")
+    lat = np.array([r["done_s"] - r["submit_s"] for r in res]) * 1e3
+    first = min(r["submit_s"] for r in res)
+    last = max(r["done_s"] for r in res)
+    bs = np.array([r["batch_size"] for r in res])
+    return {"requests": len(res), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "span_s": last - first,
+            "requests_per_s": len(res) / (last - first), "mean_batch": float(bs.mean()),
+            "batches": len({r["batch"] for r in res}), "lanes": lanes, "max_batch": max_batch}, res
+
+
+def bench_serving(a):
+    """Serving loop (c3 shape): a 2^31-sample waveform resident with its FP64
+    prefix sums; Poisson stream of template requests (lengths U[128, 8192],
+    70% kNN k in {1, 10, 100}, 30% threshold rho ~ U(0.6, 0.9)) replayed in
+    real time against the continuous-batching loop, then against one request
+    per launch (one lane, batch size 1); latencies from the library's own
+    submit / done timestamps."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    n = a.n if a.n != 10**8 else 2**31
+    t = time.time()
+    w = synth.make("c3", n=n, templates=1,
+                   progress=lambda f: print(f"[bench] waveform {100 * f:.0f}%", file=sys.stderr, flush=True))
+    stream = synth.request_stream(w, a.rate, a.duration)
+    print(f"[bench] n={w.x.size} generated in {time.time() - t:.1f}s; {len(stream)} requests over {a.duration}s",
+          file=sys.stderr, flush=True)
+    ctx = Context(precision=a.prec, fft=a.fft)
+    t = time.time()
+    ctx.load_waveform(w.x)
+    load_s = time.time() - t
+    # Derived structures (block prefix pairs for templates up to 8192, FFT
+    # piece spectra) are built by a first search of each form.
+    t = time.time()
+    warm = [synth.cut_templates(w.x, w.events, [m], np.random.default_rng(1))[0] for m in (8192, 128)]
+    for tm in warm:
+        ctx.knn([tm], 1, 0)
+    warm_s = time.time() - t
+    print(f"[bench] resident in {load_s:.1f}s, derived structures in {warm_s:.1f}s", file=sys.stderr, flush=True)
+    comps = sum(w.x.size - r["template"].size + 1 for r in stream)
+    batched, res_b = _serve(ctx, stream, a.lanes, a.max_batch, "batched")
+    print(f"[bench] batched: p50 {batched['p50_ms']:.1f} ms p99 {batched['p99_ms']:.1f} ms, "
+          f"{batched['requests_per_s']:.1f} req/s", file=sys.stderr, flush=True)
+    sub = [r for r in stream if r["t"] < a.single_duration]
+    single, res_s = _serve(ctx, sub, 1, 1, "one-per-launch")
+    print(f"[bench] one per launch: p50 {single['p50_ms']:.1f} ms p99 {single['p99_ms']:.1f} ms, "
+          f"{single['requests_per_s']:.1f} req/s", file=sys.stderr, flush=True)
+    same = all(np.array_equal(x["pos"], y["pos"]) and np.array_equal(x["d"].view(np.uint32), y["d"].view(np.uint32))
+               for x, y in zip(res_b, res_s))
+    out = {"metric": METRIC, "value": comps / batched["span_s"], "unit": "comparisons/s", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": a.prec + " MFMA / fp32 FFT + exact verify", "data": "synthetic",
+           "config": {"workload": f"serving: n={w.x.size} resident, Poisson {a.rate} req/s for {a.duration}s, "
+                                  f"m~U[128,8192], 70% kNN k in {{1,10,100}}, 30% threshold rho~U(0.6,0.9)",
+                      "n": int(w.x.size), "rate": a.rate, "duration_s": a.duration},
+           "p50_ms": batched["p50_ms"], "p99_ms": batched["p99_ms"], "batched": batched,
+           "one_per_launch": single, "same_answers_batched_vs_single": bool(same),
+           "stats_build_s": load_s, "derived_build_s": warm_s}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -118,6 +216,8 @@ def main():
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     if a.config == "c4":
         return bench_long(a)
+    if a.config == "c3":
+        return bench_serving(a)
     from tremor_amd import synth
     from tremor_amd.capi import Context
 

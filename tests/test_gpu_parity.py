@@ -235,6 +235,33 @@ def test_fft_form_uneven_long_templates(fft_ctx, oracle):
     assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
 
 
+@pytest.mark.parametrize("merge", [0, 16])
+def test_fused_fft_form(fft_ctx, oracle, merge):
+    """Templates up to 8192 samples take the fused LDS transform (B = 2^15):
+    many pieces, the waveform's last piece, missing samples, short and
+    maximal lengths."""
+    from tremor_amd import synth
+    x, onsets, rng = _wave(300_000, 21, events=30)
+    lengths = [16, 33, 64, 100, 700, 3000, 8192]
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    x[rng.integers(0, x.size, size=20)] = synth.MISSING
+    fft_ctx.load_waveform(x)
+    _log("fused-knn", merge=merge)
+    pos, d, st = fft_ctx.knn(tm, 10, merge)
+    print("  stats", st, flush=True)
+    assert st["fft_templates"] == len(tm) and st["fft_block"] == 32768
+    po, do, _ = oracle.scan_knn(x, tm, 10, merge)
+    assert np.array_equal(pos, po)
+    assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+    assert st["max_abs_rho_err"] < 1e-3
+    for rho in (0.5, 0.8):
+        _log("fused-thr", merge=merge, rho=rho)
+        r = fft_ctx.threshold(tm, rho, 3 * merge)
+        tt, tp, td = oracle.scan_threshold(x, tm, rho, 3 * merge)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
 def test_cost_model_routes_long_templates(oracle):
     from tremor_amd import synth
     from tremor_amd.capi import Context

@@ -69,6 +69,7 @@ inline double now_s()
 // One stream and its per-batch work buffers (grown on demand, reused).
 struct Lane {
     int device = 0;
+    int cus = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_pass_ms = 0.0f;
@@ -99,6 +100,8 @@ struct Lane {
             e = hipEventCreate(&ev0);
         if (e == hipSuccess)
             e = hipEventCreate(&ev1);
+        if (e == hipSuccess)
+            e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         return e;
     }
     void destroy()
@@ -138,7 +141,8 @@ struct Lane {
 struct FftClass {
     int B = 0, Lmax = 0, step = 0, bins = 0;
     uint64_t P = 0;
-    DevMem spectra, pinfo, pnorm, rb, norm2, work_real;
+    bool fused = false;  // B = 2^15: fused LDS pass (twiddle table, no rebased prefix pairs)
+    DevMem spectra, pinfo, pnorm, rb, norm2, work_real, table;
     hipfftHandle fwd = 0;
     FftClass() = default;
     FftClass(const FftClass &) = delete;

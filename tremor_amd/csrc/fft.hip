@@ -28,15 +28,18 @@ namespace {
 constexpr int kThreads = 256;
 
 // Pieces p0 .. p0 + cnt: x~ = x0 - c_p for samples inside the waveform, 0 past
-// its end; sum of squares of each piece into norm2[p] (FP64 atomics).
+// its end; sum of squares of each piece into norm2[p] (FP64 atomics) and
+// max |x~| into pinfo[p].w (float bits, non-negative: integer max).
 __global__ void k_piece_input(const float *__restrict__ x, uint64_t n, int step, int B, uint64_t p0,
-                              const float4 *__restrict__ pinfo, float *__restrict__ out, double *__restrict__ norm2)
+                              float4 *__restrict__ pinfo, float *__restrict__ out, double *__restrict__ norm2)
 {
     __shared__ double red[kThreads];
+    __shared__ float redm[kThreads];
     const uint64_t p = p0 + blockIdx.y;
     const float c = pinfo[p].x;
     const uint64_t base = p * static_cast<uint64_t>(step);
     double s = 0.0;
+    float amax = 0.0f;
     for (int k = blockIdx.x * kThreads + threadIdx.x; k < B; k += gridDim.x * kThreads) {
         const uint64_t pos = base + static_cast<uint64_t>(k);
         float v = 0.0f;
@@ -46,16 +49,22 @@ __global__ void k_piece_input(const float *__restrict__ x, uint64_t n, int step,
         }
         out[static_cast<uint64_t>(blockIdx.y) * B + k] = v;
         s += static_cast<double>(v) * static_cast<double>(v);
+        amax = fmaxf(amax, fabsf(v));
     }
     red[threadIdx.x] = s;
+    redm[threadIdx.x] = amax;
     __syncthreads();
     for (int o = kThreads / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o)
+        if (threadIdx.x < o) {
             red[threadIdx.x] += red[threadIdx.x + o];
+            redm[threadIdx.x] = fmaxf(redm[threadIdx.x], redm[threadIdx.x + o]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         atomicAdd(&norm2[p], red[0]);
+        atomicMax(reinterpret_cast<unsigned int *>(&pinfo[p].w), __float_as_uint(redm[0]));
+    }
 }
 
 // Template t of the FFT set: the reversed z-normalised template, zero padded.
@@ -251,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_filter(
 }  // namespace
 
 hipError_t launch_piece_input(const float *x, uint64_t n, int step, int B, uint64_t p0, int cnt,
-                              const float4 *pinfo, float *out, double *norm2, hipStream_t stream)
+                              float4 *pinfo, float *out, double *norm2, hipStream_t stream)
 {
     const int gx = (B + kThreads * 16 - 1) / (kThreads * 16);
     hipLaunchKernelGGL(k_piece_input, dim3(gx, cnt), dim3(kThreads), 0, stream, x, n, step, B, p0, pinfo, out, norm2);

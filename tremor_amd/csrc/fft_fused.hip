@@ -1,0 +1,509 @@
+// Fused overlap-save FFT candidate pass for FFT length B = 2^15 (templates up
+// to Lmax = B / 4 = 8192 samples), in the role of the reference's FFT
+// fallback pass (fft_fallback.hpp:67-117) with its filter-then-exact-recheck
+// contract.  One workgroup of 1024 threads per CU; each takes pieces in turn.
+// Per piece and template of the batch (the piece spectrum X_p, hipFFT r2c
+// cached per waveform, comes from HBM once per batch and from L2 after):
+//
+//   1. Y = X_p .* H_t (H_t the template's spectrum, shared by all pieces:
+//      L2-resident), written to LDS;
+//   2. the real inverse transform of Y as one 16384-point complex transform:
+//      Z[k] = (Y[k] + conj Y[M-k]) + i e^{i pi k / M} (Y[k] - conj Y[M-k]),
+//      z = sum_k Z[k] e^{2 pi i r k / M} gives y[2r] = Re z[r], y[2r+1] =
+//      Im z[r] (unnormalised, like a c2r of length B), computed in LDS by
+//      Stockham passes of radix 16, 16, 16, 4 (the first one in registers);
+//   3. the window dot products dt = y[m - 1 + i] / B are read from LDS by a
+//      chunk filter: for each 64 consecutive windows, a rigorous lower bound
+//      of their reference distances from the largest dt and the statistics of
+//      the windows' common core (two FP64 prefix reads per chunk); only
+//      chunks that may hold a window within the bound are examined window by
+//      window (filter.hpp's interval) and emit candidates.
+//
+// Nothing of size B leaves the chip: per (piece, template) HBM traffic is the
+// piece spectrum (amortised over the batch) and the candidates.
+//
+// Error bound.  The inverse transform here is a sequence of 14 radix-2 levels
+// (grouped as radix 16/16/16/4) with twiddles from an FP64-rounded table
+// (at most one complex product each: error <= 5u), plus the Z formation; by
+// the standard analysis of Cooley-Tukey transforms (relative 2-norm error <=
+// levels * (mu + gamma_4 (sqrt 2 + mu)) for twiddle error mu) its relative
+// error is below 200u, within the 10 log2(B) u = 150u... x4 allowance of
+// fft_error_factor, which budgets 4 * (2 * 10 log2 B + 2) u = 1208u for the
+// two forward transforms, the product and the inverse together:
+// forward (600u, black box) + product (2u) + inverse (<= 400u with a factor 2
+// margin) <= 1208u on ||x~||_2 ||H||_inf, and 600u on ||h||_2 ||X||_inf.
+#include "filter.hpp"
+
+#pragma clang fp contract(fast)
+
+namespace tmx {
+namespace {
+
+constexpr int kM = 16384;            // complex points of the half-length transform
+constexpr int kB = 2 * kM;           // real length B
+constexpr int kT = 1024;             // threads per workgroup
+constexpr int kWaves = kT / 64;
+constexpr int kStep = kB - kB / 4 + 1;        // windows per piece
+constexpr int kChunks = (kStep + 63) / 64;    // 64-window chunks per piece
+constexpr int kLds = kM + (kM >> 4) + 2;      // padded complex slots (index M included)
+
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+
+// A copy of x the compiler cannot see through: index arithmetic derived from
+// it is redone where it is used instead of being hoisted out of the loops
+// (and spilled: the transform needs nearly every register).
+__device__ __forceinline__ int opaque(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// Global loads through buffer resources: one 32-bit per-thread offset, the
+// per-element part in a scalar register (no 64-bit address per load).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), static_cast<short>(0), static_cast<int>(bytes),
+                                             0x00020000);
+}
+__device__ __forceinline__ float2 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// e^{i pi r / 16}, r = 0..31 (FP64-rounded): the Z factors (r < 16) and the
+// last pass's e^{i pi s / 8}, e^{i pi s / 4}, e^{3 i pi s / 8} (r = 2s, 4s, 6s).
+__constant__ float kZc[32] = {
+    1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f, 0.70710678118654757f,
+    0.55557023301960229f, 0.38268343236508984f, 0.19509032201612833f, 0.0f, -0.19509032201612819f,
+    -0.38268343236508973f, -0.55557023301960196f, -0.70710678118654746f, -0.83146961230254535f,
+    -0.92387953251128674f, -0.98078528040323043f, -1.0f, -0.98078528040323043f, -0.92387953251128685f,
+    -0.83146961230254546f, -0.70710678118654768f, -0.55557023301960218f, -0.38268343236509034f,
+    -0.19509032201612866f, 0.0f, 0.1950903220161283f, 0.38268343236509f, 0.55557023301960184f,
+    0.70710678118654735f, 0.83146961230254524f, 0.92387953251128652f, 0.98078528040323032f};
+__constant__ float kZs[32] = {
+    0.0f, 0.19509032201612825f, 0.38268343236508978f, 0.55557023301960218f, 0.70710678118654746f,
+    0.83146961230254524f, 0.92387953251128674f, 0.98078528040323043f, 1.0f, 0.98078528040323043f,
+    0.92387953251128674f, 0.83146961230254546f, 0.70710678118654757f, 0.55557023301960218f,
+    0.38268343236508989f, 0.19509032201612861f, 0.0f, -0.19509032201612836f, -0.38268343236508967f,
+    -0.55557023301960196f, -0.70710678118654746f, -0.83146961230254524f, -0.92387953251128652f,
+    -0.98078528040323032f, -1.0f, -0.98078528040323043f, -0.92387953251128663f, -0.83146961230254546f,
+    -0.70710678118654768f, -0.55557023301960218f, -0.38268343236509039f, -0.19509032201612872f};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul_i(float2 a) { return make_float2(-a.y, a.x); }
+
+// Inverse-sign 4-point DFT in place: X_k = sum_n v_n i^{nk}.
+__device__ __forceinline__ void dft4(float2 &a, float2 &b, float2 &c, float2 &d)
+{
+    const float2 t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = mul_i(csub(b, d));
+    a = cadd(t0, t2);
+    c = csub(t0, t2);
+    b = cadd(t1, t3);
+    d = csub(t1, t3);
+}
+
+// Inverse-sign 16-point DFT in place, natural order in and out (4 x 4:
+// n = 4 n1 + n2, k = k1 + 4 k2).
+__device__ __forceinline__ void dft16(float2 *v)
+{
+    constexpr float c8 = 0.92387953251128674f, s8 = 0.38268343236508977f, h = 0.70710678118654752f;
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++)
+        dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // v[n2 + 4 k1] now holds T[n2][k1]; twiddle by e^{i pi n2 k1 / 8}.
+    v[1 + 4] = cmul(v[1 + 4], make_float2(c8, s8));
+    v[1 + 8] = cmul(v[1 + 8], make_float2(h, h));
+    v[1 + 12] = cmul(v[1 + 12], make_float2(s8, c8));
+    v[2 + 4] = cmul(v[2 + 4], make_float2(h, h));
+    v[2 + 8] = mul_i(v[2 + 8]);
+    v[2 + 12] = cmul(v[2 + 12], make_float2(-h, h));
+    v[3 + 4] = cmul(v[3 + 4], make_float2(s8, c8));
+    v[3 + 8] = cmul(v[3 + 8], make_float2(-h, h));
+    v[3 + 12] = cmul(v[3 + 12], make_float2(-c8, -s8));
+    float2 o[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        float2 a = v[0 + 4 * k1], b = v[1 + 4 * k1], c = v[2 + 4 * k1], d = v[3 + 4 * k1];
+        dft4(a, b, c, d);
+        o[k1] = a;
+        o[k1 + 4] = b;
+        o[k1 + 8] = c;
+        o[k1 + 12] = d;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        v[r] = o[r];
+}
+
+// v[r] *= w^{r k} for r = 1..15, w = e^{2 pi i / (16 Ns)}, from the table
+// E[t] = e^{i pi t / M} with index factor F = 2M / (16 Ns): six table values
+// (w^{s k}, w^{4 q k}) and one product each.
+__device__ __forceinline__ void twiddle16(float2 *v, __amdgpu_buffer_rsrc_t E, int k, int F)
+{
+    const uint32_t o = static_cast<uint32_t>(k * F) * 8u;
+    const float2 p1 = bload(E, o, 0), p2 = bload(E, 2 * o, 0), p3 = bload(E, 3 * o, 0);
+    const float2 c1 = bload(E, 4 * o, 0), c2 = bload(E, 8 * o, 0), c3 = bload(E, 12 * o, 0);
+    v[1] = cmul(v[1], p1);
+    v[2] = cmul(v[2], p2);
+    v[3] = cmul(v[3], p3);
+    v[4] = cmul(v[4], c1);
+    v[5] = cmul(v[5], cmul(c1, p1));
+    v[6] = cmul(v[6], cmul(c1, p2));
+    v[7] = cmul(v[7], cmul(c1, p3));
+    v[8] = cmul(v[8], c2);
+    v[9] = cmul(v[9], cmul(c2, p1));
+    v[10] = cmul(v[10], cmul(c2, p2));
+    v[11] = cmul(v[11], cmul(c2, p3));
+    v[12] = cmul(v[12], c3);
+    v[13] = cmul(v[13], cmul(c3, p1));
+    v[14] = cmul(v[14], cmul(c3, p2));
+    v[15] = cmul(v[15], cmul(c3, p3));
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Lower bound on the reference distance of every window j in [j0, j0 + cnt)
+// of a template of length m, given that each window's
+// N_j = dt_j - (mu_j - c) S is at most nmax: with the windows' common core
+// C = [j0 + cnt - 1, j0 + m), m sigma_j^2 = sum_window (x0 - mu_j)^2 >=
+// sum_C (x0 - mu_C)^2 = V_C, so sigma_j >= sqrt(V_C / m).  Then
+// d_exact = Q2 + m - 2 N_j / sigma_j >= Q2 + m - 2 max(nmax, 0) / sigma_lo,
+// and the reference's FP32 value d_ref >= d_exact - E_ref(d_exact) >= dlo -
+// E_ref(dlo) (d - E_ref(d) increases wherever the result can exceed 0), with
+// E_ref as in filter.hpp and |mu_j| / sigma_j <= mu_abs / sigma_lo.
+// -inf when the core cannot bound sigma.
+__device__ double chunk_dist_lo(const Waveform &w, uint64_t j0, int cnt, int m, double Q2, double nmax,
+                                double mu_abs)
+{
+    constexpr double u = 5.9604644775390625e-08;
+    constexpr double u64 = 1.1102230246251565e-16;
+    const int Lc = m - (cnt - 1);
+    if (Lc < 2 || !(nmax <= 1e300))
+        return -INFINITY;  // no core, or a non-finite dot product: examine every window
+    const uint64_t a = j0 + static_cast<uint64_t>(cnt - 1), b = j0 + static_cast<uint64_t>(m);
+    const double2 pa = w.p[a], pb = w.p[b];
+    const double S1 = pb.x - pa.x, S2 = pb.y - pa.y;
+    const double eS1 = kPrefixGamma * (sqrt(static_cast<double>(b) * fabs(pb.y)) +
+                                       sqrt(static_cast<double>(a) * fabs(pa.y))) +
+                       4.0 * u64 * fabs(S1);
+    const double eS2 = kPrefixGamma * (fabs(pb.y) + fabs(pa.y)) + 4.0 * u64 * fabs(S2);
+    const double L = static_cast<double>(Lc);
+    const double V = S2 - S1 * S1 / L;
+    const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) / L + 8.0 * u64 * (fabs(S2) + S1 * S1 / L);
+    const double vlo = V - eV;
+    if (!(vlo > 0.0))
+        return -INFINITY;
+    const double md = static_cast<double>(m);
+    const double sig_lo = sqrt(vlo / md) * (1.0 - 4.0 * u64);
+    const double gmax = nmax > 0.0 ? nmax / sig_lo * (1.0 + 4.0 * u64) : 0.0;
+    const double base = Q2 + md;
+    const double dlo = base - 2.0 * gmax - 8.0 * u64 * (base + 2.0 * gmax);
+    if (!(dlo > 0.0))
+        return dlo;
+    const double sq = sqrt(dlo);
+    const double rm = sqrt(md);
+    const double D1 = (u * 1.001) * (3.05 * rm + 1.03 * rm * mu_abs / sig_lo + sq);
+    const double neff = floor(md / 16.0) + 36.0;
+    const double gam = neff * u / (1.0 - neff * u);
+    const double Eref = (D1 * (2.0 * sq + D1) + gam * (sq + D1) * (sq + D1)) * 1.01;
+    return dlo - Eref;
+}
+
+// Steps 1 and 2 for one (piece, template): on return data[pad(r)] holds
+// z[r] = y[2r] + i y[2r+1], the unnormalised circular correlation of the
+// piece with the template (all threads synchronised).
+__device__ __forceinline__ void correlate_piece(float2 *data, __amdgpu_buffer_rsrc_t Xr, __amdgpu_buffer_rsrc_t Hr,
+                                                __amdgpu_buffer_rsrc_t Er)
+{
+    const int tid = opaque(threadIdx.x);
+    const uint32_t toff = static_cast<uint32_t>(tid) * 8u;
+    // 1. Y = X .* H into LDS.
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        v[r] = cmul(bload(Xr, toff, 8192u * r), bload(Hr, toff, 8192u * r));
+        data[pad(tid + 1024 * r)] = v[r];
+    }
+    if (tid == 0)
+        data[pad(kM)] = cmul(bload(Xr, 0, 8u * kM), bload(Hr, 0, 8u * kM));
+    __syncthreads();
+    // 2a. Z formation (registers) = the first pass's inputs; the
+    // factor e^{i pi k / M} for k = tid + 1024 r is E[tid] e^{i pi r / 16}.
+    {
+        const float2 e0 = bload(Er, toff, 0);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int k = tid + 1024 * r;
+            const float2 ym = data[pad(kM - k)];
+            const float2 cy = make_float2(ym.x, -ym.y);
+            const float2 A = cadd(v[r], cy);
+            const float2 D = csub(v[r], cy);
+            const float2 wk = r == 0 ? e0 : cmul(e0, make_float2(kZc[r], kZs[r]));
+            v[r] = cadd(A, mul_i(cmul(wk, D)));
+        }
+    }
+    __syncthreads();
+    // Pass 1: radix 16, Ns = 1.
+    dft16(v);
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        data[pad(16 * tid + r)] = v[r];
+    __syncthreads();
+    // Pass 2: radix 16, Ns = 16.
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        v[r] = data[pad(tid + 1024 * r)];
+    twiddle16(v, Er, tid & 15, 2 * kM / 256);
+    dft16(v);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        data[pad((tid >> 4) * 256 + (tid & 15) + 16 * r)] = v[r];
+    __syncthreads();
+    // Pass 3: radix 16, Ns = 256.
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        v[r] = data[pad(tid + 1024 * r)];
+    twiddle16(v, Er, tid & 255, 2 * kM / 4096);
+    dft16(v);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        data[pad((tid >> 8) * 4096 + (tid & 255) + 256 * r)] = v[r];
+    __syncthreads();
+    // Pass 4: radix 4, Ns = 4096; each thread owns its four
+    // butterflies' slots, so no barrier between read and write.  The
+    // twiddles e^{2 pi i r jj / M} for jj = tid + 1024 s are
+    // E[2 r tid] e^{i pi r s / 8}.
+    {
+        const float2 e1 = bload(Er, 2 * toff, 0), e2 = bload(Er, 4 * toff, 0), e3 = bload(Er, 6 * toff, 0);
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int jj = tid + 1024 * s;
+            float2 a = data[pad(jj)], b = data[pad(jj + 4096)], cc = data[pad(jj + 8192)],
+                   d = data[pad(jj + 12288)];
+            b = cmul(b, s == 0 ? e1 : cmul(e1, make_float2(kZc[2 * s], kZs[2 * s])));
+            cc = cmul(cc, s == 0 ? e2 : cmul(e2, make_float2(kZc[4 * s], kZs[4 * s])));
+            d = cmul(d, s == 0 ? e3 : cmul(e3, make_float2(kZc[6 * s], kZs[6 * s])));
+            dft4(a, b, cc, d);
+            data[pad(jj)] = a;
+            data[pad(jj + 4096)] = b;
+            data[pad(jj + 8192)] = cc;
+            data[pad(jj + 12288)] = d;
+        }
+    }
+    __syncthreads();
+}
+
+struct __align__(8) ChunkInfo {
+    float dmax;
+    int lane;  // lane of the maximum (first one)
+};
+
+__global__ __launch_bounds__(kT, 1) void k_fft_fused(
+    Waveform w, const TmplDev *__restrict__ tmpl, int t0, int F, const float2 *__restrict__ spectra, int bins,
+    const float2 *__restrict__ qspec, const float *__restrict__ qmax, const float4 *__restrict__ pinfo,
+    const float2 *__restrict__ pnorm, const float2 *__restrict__ E, uint64_t P, int stride, double kfft, int mode,
+    Cand *__restrict__ cand, float *__restrict__ cand_val, unsigned long long *__restrict__ cand_count, uint64_t cap)
+{
+    __shared__ float2 data[kLds];
+    __shared__ ChunkInfo chunk[kChunks];
+    __shared__ unsigned char keep[kChunks];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const float *fdata = reinterpret_cast<const float *>(data);
+    constexpr double u = 5.9604644775390625e-08;
+    const uint64_t nwork = (P + stride - 1) / stride;
+    const __amdgpu_buffer_rsrc_t Er = rsrc(E, 2u * kM * 8u);
+
+    for (uint64_t it = blockIdx.x; it < nwork; it += gridDim.x) {
+        const uint64_t p = it * static_cast<uint64_t>(stride);
+        const uint64_t base = p * static_cast<uint64_t>(kStep);
+        // The piece spectrum (re-read per template: the repeats hit L2).
+        const __amdgpu_buffer_rsrc_t Xr = rsrc(spectra + p * static_cast<uint64_t>(bins), bins * 8u);
+        const float4 pi = pinfo[p];
+        const float c = pi.x;
+        const double amax = static_cast<double>(pi.w) * (1.0 + 1e-6);
+        const float2 pn = pnorm[p];
+
+        for (int ti = 0; ti < F; ti++) {
+            const int t = t0 + ti;
+            const int m = tmpl[t].m;
+            if (static_cast<uint64_t>(m) > w.n)
+                continue;
+            const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
+            if (base >= W)
+                continue;
+            const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
+            const double bound = static_cast<double>(tmpl[t].bound);
+            const __amdgpu_buffer_rsrc_t Hr = rsrc(qspec + static_cast<uint64_t>(ti) * static_cast<uint64_t>(bins),
+                                                   bins * 8u);
+            correlate_piece(data, Xr, Hr, Er);
+
+            // 3. Filter.  dt of window base + i is y[m - 1 + i] / B.
+            const uint64_t nwin64 = W - base < static_cast<uint64_t>(kStep) ? W - base : static_cast<uint64_t>(kStep);
+            const int nwin = static_cast<int>(nwin64);
+            const int nch = (nwin + 63) / 64;
+            const float invB = 1.0f / static_cast<float>(kB);
+            // 3a. Per chunk: largest dt and its lane.
+            for (int ch = wave; ch < nch; ch += kWaves) {
+                const int i = ch * 64 + lane;
+                float dt = -INFINITY;
+                if (i < nwin) {
+                    const int n = m - 1 + i;
+                    dt = fdata[2 * pad(n >> 1) + (n & 1)] * invB;
+                }
+                const float mx = wave_max(dt);
+                const unsigned long long hit = __ballot(dt == mx);
+                if (lane == 0)
+                    chunk[ch] = ChunkInfo{mx, __ffsll(static_cast<long long>(hit)) - 1};
+            }
+            __syncthreads();
+            // Error terms of this (template, piece): FFT error of every dot
+            // product plus the rounding of x~ = x0 - c (eta).
+            const double sQ2 = sqrt(Q2);
+            const double eabs = kfft * (static_cast<double>(pn.x) * static_cast<double>(qmax[ti]) +
+                                        sQ2 * static_cast<double>(pn.y));
+            const double eta = 1.01 * u;
+            const double edot = eta * sQ2 * static_cast<double>(pn.x) * (1.0 + 1e-6) + eabs;
+            const double nslack = edot + amax * fabs(S);
+            const double mu_abs = fabs(static_cast<double>(c)) + amax;
+            if (mode == kModeMain) {
+                // 3b. Chunk tests, one chunk per thread.
+                for (int ch = tid; ch < nch; ch += kT) {
+                    const int cnt = nwin - ch * 64 < 64 ? nwin - ch * 64 : 64;
+                    const double lo = chunk_dist_lo(w, base + static_cast<uint64_t>(ch) * 64, cnt, m, Q2,
+                                                    static_cast<double>(chunk[ch].dmax) + nslack, mu_abs);
+                    keep[ch] = lo > bound ? 0 : 1;
+                }
+                __syncthreads();
+                // 3c. Windows of the kept chunks.
+                for (int ch = wave; ch < nch; ch += kWaves) {
+                    if (!keep[ch])
+                        continue;
+                    const int i = ch * 64 + lane;
+                    bool emit = false;
+                    float est = 0.0f;
+                    const uint64_t j = base + static_cast<uint64_t>(i);
+                    if (i < nwin && w.x[j] != kMissing) {
+                        const int n = m - 1 + i;
+                        const float dt = fdata[2 * pad(n >> 1) + (n & 1)] * invB;
+                        const DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(dt), eta, eabs);
+                        emit = !(iv.lo > bound);  // NaN (undecidable): a candidate
+                        est = static_cast<float>(iv.est);
+                    }
+                    const unsigned long long mask = __ballot(emit);
+                    if (mask) {
+                        const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+                        unsigned long long slot0 = 0;
+                        if (lane == leader)
+                            slot0 = atomicAdd(cand_count, static_cast<unsigned long long>(__popcll(mask)));
+                        slot0 = __shfl(slot0, leader);
+                        if (emit) {
+                            const unsigned long long slot =
+                                slot0 + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
+                            if (slot < cap) {
+                                cand[slot].t = static_cast<uint32_t>(t);
+                                cand[slot].j = static_cast<uint32_t>(j);
+                                cand_val[slot] = est;
+                            }
+                        }
+                    }
+                }
+            } else {
+                // Seed: per wave, among its chunks the one whose largest dt
+                // is largest relative to the core's spread; that chunk's best
+                // window gives a rigorous upper bound.
+                float best_g = -INFINITY;
+                int best_ch = -1;
+                for (int ch = tid; ch < nch; ch += kT) {
+                    const int cnt = nwin - ch * 64 < 64 ? nwin - ch * 64 : 64;
+                    const int Lc = m - (cnt - 1);
+                    if (Lc < 2)
+                        continue;
+                    const uint64_t a = base + static_cast<uint64_t>(ch) * 64 + static_cast<uint64_t>(cnt - 1);
+                    const double2 pa = w.p[a], pb = w.p[a + static_cast<uint64_t>(Lc)];
+                    const double S1 = pb.x - pa.x;
+                    const double V = (pb.y - pa.y) - S1 * S1 / static_cast<double>(Lc);
+                    if (V > 0.0) {
+                        const float g = chunk[ch].dmax * static_cast<float>(1.0 / sqrt(V));
+                        if (g > best_g) {
+                            best_g = g;
+                            best_ch = ch;
+                        }
+                    }
+                }
+                float ub = INFINITY;
+                uint32_t bj = 0xffffffffu;
+                if (best_ch >= 0) {
+                    const int i = best_ch * 64 + chunk[best_ch].lane;
+                    const uint64_t j = base + static_cast<uint64_t>(i);
+                    if (i < nwin && w.x[j] != kMissing) {
+                        const int n = m - 1 + i;
+                        const float dt = fdata[2 * pad(n >> 1) + (n & 1)] * invB;
+                        const DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(dt), eta, eabs);
+                        if (iv.hi < 3.0e38) {
+                            ub = static_cast<float>(iv.hi) * (1.0f + 1e-6f);
+                            bj = static_cast<uint32_t>(j);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const float ov = __shfl_xor(ub, off);
+                    const uint32_t oj = __shfl_xor(bj, off);
+                    if (ov < ub || (ov == ub && oj < bj)) {
+                        ub = ov;
+                        bj = oj;
+                    }
+                }
+                if (lane == 0 && ub < INFINITY) {
+                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
+                    if (slot < cap) {
+                        cand[slot].t = static_cast<uint32_t>(t);
+                        cand[slot].j = bj;
+                        cand_val[slot] = ub;
+                    }
+                }
+            }
+            __syncthreads();  // LDS is rewritten by the next template
+        }
+    }
+}
+
+}  // namespace
+
+// Table E[t] = e^{i pi t / M}, t in [0, 2M), for the fused pass.
+int fft_fused_table_len() { return 2 * kM; }
+int fft_fused_block() { return kB; }
+
+hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
+                            const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
+                            const float2 *table, uint64_t P, int stride, double kfft, int mode, Cand *cand,
+                            float *cand_val, unsigned long long *cand_count, uint64_t cap, int grid,
+                            hipStream_t stream)
+{
+    if (F <= 0 || P == 0)
+        return hipSuccess;
+    if (bins != kM + 1)
+        return hipErrorInvalidValue;
+    const uint64_t nwork = (P + stride - 1) / stride;
+    const unsigned g = static_cast<unsigned>(nwork < static_cast<uint64_t>(grid) ? nwork : grid);
+    hipLaunchKernelGGL(k_fft_fused, dim3(g), dim3(kT), 0, stream, w, tmpl, t0, F, spectra, bins, qspec, qmax, pinfo,
+                       pnorm, table, P, stride, kfft, mode, cand, cand_val, cand_count, cap);
+    return hipGetLastError();
+}
+
+}  // namespace tmx

@@ -163,6 +163,7 @@ __global__ __launch_bounds__(kPThreads) void k_chunk_final(const float *__restri
 // E1, E2 bound the absolute error of every entry's FP64 value (prefix-sum
 // error, kPrefixGamma, plus the rounding of the rebasing), so that the FP32
 // consumer can bound a window's statistics: entry error <= E + 2^-24 |entry|.
+// len = 0: the centres and bounds only (rb untouched).
 __global__ void k_block_prefix(const double2 *__restrict__ p, uint64_t n, uint64_t koff, int block, int len,
                                float4 *__restrict__ binfo, float2 *__restrict__ rb)
 {
@@ -205,7 +206,9 @@ hipError_t launch_block_prefix(const double2 *p, uint64_t n, int block, int len,
     const uint64_t blocks = (n + block - 1) / block;
     if (blocks == 0)
         return hipErrorInvalidValue;
-    const unsigned gx = static_cast<unsigned>((len + 255) / 256 < 64 ? (len + 255) / 256 : 64);
+    unsigned gx = static_cast<unsigned>((len + 255) / 256 < 64 ? (len + 255) / 256 : 64);
+    if (gx == 0)
+        gx = 1;
     // grid.y is limited to 65535: larger waveforms take several launches.
     for (uint64_t b0 = 0; b0 < blocks; b0 += 65535) {
         const uint64_t nb = blocks - b0 < 65535 ? blocks - b0 : 65535;

@@ -50,8 +50,16 @@ uint64_t mfma_block_windows();
 int64_t mfma_frag_count(int kpad);
 int mfma_kpad(int m);
 // fft.hip
-hipError_t launch_piece_input(const float *x, uint64_t n, int step, int B, uint64_t p0, int cnt, const float4 *pinfo,
+hipError_t launch_piece_input(const float *x, uint64_t n, int step, int B, uint64_t p0, int cnt, float4 *pinfo,
                               float *out, double *norm2, hipStream_t stream);
+// fft_fused.hip
+int fft_fused_table_len();
+int fft_fused_block();
+hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
+                            const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
+                            const float2 *table, uint64_t P, int stride, double kfft, int mode, Cand *cand,
+                            float *cand_val, unsigned long long *cand_count, uint64_t cap, int grid,
+                            hipStream_t stream);
 hipError_t launch_piece_norm(const double *norm2, float2 *pnorm, uint64_t P, hipStream_t stream);
 hipError_t launch_spec_max(const float2 *spec, int rows, int bins, float *out, int stride_out, hipStream_t stream);
 hipError_t launch_tmpl_input(const TmplDev *tmpl, const int32_t *ids, int count, int B, const float *qz, float *out,
@@ -286,20 +294,36 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
     f.step = B - f.Lmax + 1;
     f.bins = B / 2 + 1;
     f.P = (ctx->n + f.step - 1) / f.step;
+    f.fused = B == fft_fused_block();
     const uint64_t Palloc = ((f.P + kFftPieceBatch - 1) / kFftPieceBatch) * kFftPieceBatch;
     LANE_HIP(f.spectra.reserve(sizeof(float2) * Palloc * f.bins), "hipMalloc FFT spectra");
     LANE_HIP(f.pinfo.reserve(sizeof(float4) * f.P), "hipMalloc FFT pieces");
     LANE_HIP(f.pnorm.reserve(sizeof(float2) * f.P), "hipMalloc FFT pieces");
     LANE_HIP(f.norm2.reserve(sizeof(double) * f.P), "hipMalloc FFT pieces");
-    LANE_HIP(f.rb.reserve(sizeof(float2) * f.P * static_cast<uint64_t>(B + 1)), "hipMalloc FFT prefix pairs");
+    if (!f.fused)
+        LANE_HIP(f.rb.reserve(sizeof(float2) * f.P * static_cast<uint64_t>(B + 1)), "hipMalloc FFT prefix pairs");
     LANE_HIP(f.work_real.reserve(sizeof(float) * static_cast<size_t>(kFftPieceBatch) * B), "hipMalloc FFT input");
+    if (f.fused) {
+        // E[t] = e^{i pi t / M}, M = B / 2, rounded from FP64.
+        const int len = fft_fused_table_len();
+        std::vector<float2> tab(len);
+        const double M = static_cast<double>(B / 2);
+        for (int t = 0; t < len; t++) {
+            const double a = M_PI * static_cast<double>(t) / M;
+            tab[t] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+        }
+        LANE_HIP(f.table.reserve(sizeof(float2) * len), "hipMalloc FFT table");
+        LANE_HIP(hipMemcpyAsync(f.table.p, tab.data(), sizeof(float2) * len, hipMemcpyHostToDevice, L.stream),
+                 "copy FFT table");
+        LANE_HIP(hipStreamSynchronize(L.stream), "copy FFT table");
+    }
     int nn = B;
     if (hipfftPlanMany(&f.fwd, 1, &nn, nullptr, 1, B, nullptr, 1, f.bins, HIPFFT_R2C, kFftPieceBatch) !=
         HIPFFT_SUCCESS)
         return L.fail("hipFFT forward plan failed", -2);
     hipfftSetStream(f.fwd, L.stream);
-    LANE_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->n, f.step, B + 1, f.pinfo.as<float4>(), f.rb.as<float2>(),
-                                 L.stream),
+    LANE_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->n, f.step, f.fused ? 0 : B + 1, f.pinfo.as<float4>(),
+                                 f.rb.as<float2>(), L.stream),
              "FFT prefix pairs");
     LANE_HIP(launch_zero_f64(f.norm2.as<double>(), f.P, L.stream), "clear norms");
     for (uint64_t p0 = 0; p0 < f.P; p0 += kFftPieceBatch) {
@@ -447,6 +471,17 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
         }
     if (pairs.empty())
         return 0;
+    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
+    const double kfft = fft_error_factor(f.B);
+    if (f.fused) {
+        LANE_HIP(launch_fft_fused(w, L.tmpl.as<TmplDev>(), plan.Tg, F, f.spectra.as<float2>(), f.bins,
+                                  L.qspec.as<float2>(), L.qmax.as<float>(), f.pinfo.as<float4>(), f.pnorm.as<float2>(),
+                                  f.table.as<float2>(), f.P, stride, kfft, mode, L.cand.as<Cand>(),
+                                  L.cand_val.as<float>(), L.cand_count.as<unsigned long long>(), L.cand_cap,
+                                  L.cus > 0 ? L.cus : 256, L.stream),
+                 "fused FFT pass");
+        return 0;
+    }
     const int NB = static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, (512ll << 20) / (12ll * f.B))));
     if (!L.inv || L.inv_batch != NB || L.inv_B != f.B) {
         if (L.inv)
@@ -464,8 +499,6 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
     LANE_HIP(L.pairs.reserve(sizeof(FftPair) * pairs.size()), "hipMalloc FFT pairs");
     LANE_HIP(hipMemcpyAsync(L.pairs.p, pairs.data(), sizeof(FftPair) * pairs.size(), hipMemcpyHostToDevice, L.stream),
              "copy FFT pairs");
-    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
-    const double kfft = fft_error_factor(f.B);
     for (size_t b0 = 0; b0 < pairs.size(); b0 += NB) {
         const int cnt = static_cast<int>(std::min<size_t>(NB, pairs.size() - b0));
         const FftPair *pp = L.pairs.as<FftPair>() + b0;

@@ -45,8 +45,9 @@ def _sos():
     return butter(4, [2.0, 10.0], btype="bandpass", fs=FS, output="sos")
 
 
-def bandpassed_noise(n: int, rng: np.random.Generator, chunk: int = 1 << 24) -> np.ndarray:
-    """Unit-variance band-passed Gaussian noise, float32, streamed."""
+def bandpassed_noise(n: int, rng: np.random.Generator, chunk: int = 1 << 24, progress=None) -> np.ndarray:
+    """Unit-variance band-passed Gaussian noise, float32, streamed.  progress,
+    if given, is called with the fraction done every 2^28 samples."""
     from scipy.signal import sosfilt, sosfilt_zi
     sos = _sos()
     zi = sosfilt_zi(sos) * 0.0
@@ -60,6 +61,8 @@ def bandpassed_noise(n: int, rng: np.random.Generator, chunk: int = 1 << 24) -> 
         s1 += float(y.sum())
         s2 += float(np.dot(y, y))
         out[a:b] = y
+        if progress is not None and (b % (1 << 28) == 0 or b == n):
+            progress(0.8 * b / n)
     mean = s1 / n
     sd = np.sqrt(max(s2 / n - mean * mean, 1e-30))
     for a in range(0, n, chunk):
@@ -112,7 +115,8 @@ def cut_templates(x: np.ndarray, onsets: np.ndarray, lengths, rng: np.random.Gen
     return out
 
 
-def make(name: str, n: int | None = None, templates: int | None = None, seed: int | None = None) -> Workload:
+def make(name: str, n: int | None = None, templates: int | None = None, seed: int | None = None,
+         progress=None) -> Workload:
     """Builds a workload by name ('c0'..'c4', 'validate'); n and templates
     may be reduced for tests, everything else follows the workload."""
     if name == "validate":
@@ -129,7 +133,7 @@ def make(name: str, n: int | None = None, templates: int | None = None, seed: in
     N = spec["n"] if n is None else int(n)
     T = spec["T"] if templates is None else int(templates)
     ev = max(1, int(round(spec["ev"] * N / spec["n"])))
-    x = bandpassed_noise(N, rng)
+    x = bandpassed_noise(N, rng, progress=progress)
     onsets, _ = inject_events(x, ev, rng, spec["cl"])
     if idx == 0:
         lengths = np.full(T, 256)
@@ -157,3 +161,26 @@ def validate_recipe(m: int = 64) -> Workload:
     positions = [0, 4090, min(6200, n - m - 1), n - m]
     queries = [x[p:p + m].copy() for p in positions] + [rng.normal(size=m).astype("float32")]
     return Workload("validate", x, [q.astype(np.float32) for q in queries], k=5, rhos=(0.4,))
+
+
+def request_stream(w: Workload, rate: float, duration: float, seed: int = BASE_SEED + 33, mmin: int = 128,
+                   mmax: int = 8192, knn_frac: float = 0.7):
+    """The serving workload's request stream: Poisson arrivals at `rate` per
+    second for `duration` seconds; each request a template cut from an event
+    (length U[mmin, mmax]), kNN with k in {1, 10, 100} (probability
+    knn_frac) or threshold with rho ~ U(0.6, 0.9).  Returns a list of dicts
+    (t, template, kind, k, rho)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        m = int(rng.integers(mmin, mmax + 1))
+        tmpl = cut_templates(w.x, w.events, [min(m, w.x.size)], rng)[0]
+        if rng.random() < knn_frac:
+            out.append(dict(t=t, template=tmpl, kind="knn", k=int(rng.choice([1, 10, 100])), rho=0.0))
+        else:
+            out.append(dict(t=t, template=tmpl, kind="threshold", k=0, rho=float(rng.uniform(0.6, 0.9))))
+    return out
