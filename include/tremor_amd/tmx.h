@@ -73,13 +73,15 @@ typedef struct {
     double seconds_verify;   /* exact reference-order recomputation of the candidates */
     double seconds_select;   /* selection / merge */
     int64_t comparisons;     /* sum over templates of (n - m + 1) */
-    int64_t candidates;      /* windows recomputed exactly */
+    int64_t candidates;      /* windows the candidate pass emitted */
     int32_t retries;         /* candidate-buffer overflows resolved by a tighter bound or a bigger buffer */
     float max_abs_rho_err;   /* max |rho_fast - rho_exact| over the verified candidates */
     double kernel_ms_main;   /* device time of the main candidate pass (HIP events on the context stream) */
     double flops_main;       /* algorithmic flops of that pass: sum over templates of 2 m (n - m + 1) */
     int32_t fft_templates;   /* templates that took the FFT form */
     int32_t fft_block;       /* FFT length B of those templates (0: none) */
+    int64_t verified;        /* candidates left for the exact recomputation after the kNN bound was
+                                tightened from the candidates' own upper bounds */
 } tmx_stats;
 
 typedef struct {

@@ -43,7 +43,7 @@ class Stats(ctypes.Structure):
                 ("seconds_select", ctypes.c_double), ("comparisons", ctypes.c_int64),
                 ("candidates", ctypes.c_int64), ("retries", ctypes.c_int32), ("max_abs_rho_err", ctypes.c_float),
                 ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double),
-                ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32)]
+                ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -77,8 +77,9 @@ struct Lane {
     // templates of the batch
     DevMem raw, qz, order, tmpl, groups, afrag;
     // candidates, grouping, results
-    DevMem cand, cand_val, cand_d, cand_count;
-    DevMem counts, offsets, cursor, keys_tmp, keys, sort_tmp;
+    DevMem cand, cand_val, cand_lo, cand_hi, cand_d, cand_count;
+    DevMem cand2, val2, lo2, hi2, total;  // second candidate set (refinement)
+    DevMem counts, offsets, cursor, keys_tmp, keys, tv_a, tv_b, sort_tmp;
     DevMem out_pos, out_d, errbits;
     uint64_t cand_cap = 0;
     // FFT form: template spectra, products, inverse transforms

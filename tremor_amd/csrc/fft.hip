@@ -135,8 +135,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_filter(
     Waveform w, const TmplDev *__restrict__ tmpl, const FftPair *__restrict__ pairs, const float *__restrict__ y,
     int B, int step, const float2 *__restrict__ rb, int rb_len, const float4 *__restrict__ pinfo,
     const float2 *__restrict__ pnorm, const float *__restrict__ qmax, const int32_t *__restrict__ slot, double kfft,
-    int mode, Cand *__restrict__ cand, float *__restrict__ cand_val, unsigned long long *__restrict__ cand_count,
-    uint64_t cap)
+    int mode, CandOut co)
 {
     __shared__ float s_ub[kThreads / 64];
     __shared__ uint32_t s_j[kThreads / 64];
@@ -200,14 +199,9 @@ __global__ __launch_bounds__(kThreads) void k_fft_filter(
             }
             if (survive && w.x[j] != kMissing) {
                 const DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(dt), eta, eabs);
-                if (iv.lo <= bound) {
-                    const unsigned long long slot_c = atomicAdd(cand_count, 1ULL);
-                    if (slot_c < cap) {
-                        cand[slot_c].t = static_cast<uint32_t>(t);
-                        cand[slot_c].j = static_cast<uint32_t>(j);
-                        cand_val[slot_c] = static_cast<float>(iv.est);
-                    }
-                }
+                if (!(iv.lo > bound))  // NaN (undecidable): a candidate
+                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), static_cast<uint32_t>(j),
+                             static_cast<float>(iv.est), iv.lo, iv.hi);
             }
         } else if (judge && V > 0.0f) {
             const float g = dt * __frsqrt_rn(V);
@@ -245,14 +239,8 @@ __global__ __launch_bounds__(kThreads) void k_fft_filter(
                     ub = s_ub[q];
                     bj = s_j[q];
                 }
-            if (ub < INFINITY) {
-                const unsigned long long slot_c = atomicAdd(cand_count, 1ULL);
-                if (slot_c < cap) {
-                    cand[slot_c].t = static_cast<uint32_t>(t);
-                    cand[slot_c].j = bj;
-                    cand_val[slot_c] = ub;
-                }
-            }
+            if (ub < INFINITY)
+                cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), bj, ub, ub, ub);
         }
     }
 }
@@ -313,14 +301,14 @@ hipError_t launch_product(const FftPair *pairs, int npairs, const float2 *spectr
 
 hipError_t launch_fft_filter(const Waveform &w, const TmplDev *tmpl, const FftPair *pairs, int npairs, const float *y,
                              int B, int step, const float2 *rb, int rb_len, const float4 *pinfo, const float2 *pnorm,
-                             const float *qmax, const int32_t *slot, double kfft, int mode, Cand *cand,
-                             float *cand_val, unsigned long long *cand_count, uint64_t cap, hipStream_t stream)
+                             const float *qmax, const int32_t *slot, double kfft, int mode, const CandOut &co,
+                             hipStream_t stream)
 {
     if (npairs <= 0)
         return hipSuccess;
     const int gx = (step + kThreads * 16 - 1) / (kThreads * 16);
     hipLaunchKernelGGL(k_fft_filter, dim3(gx, npairs), dim3(kThreads), 0, stream, w, tmpl, pairs, y, B, step, rb,
-                       rb_len, pinfo, pnorm, qmax, slot, kfft, mode, cand, cand_val, cand_count, cap);
+                       rb_len, pinfo, pnorm, qmax, slot, kfft, mode, co);
     return hipGetLastError();
 }
 

@@ -165,14 +165,6 @@ __device__ __forceinline__ void twiddle16(float2 *v, __amdgpu_buffer_rsrc_t E, i
     v[15] = cmul(v[15], cmul(c3, p3));
 }
 
-__device__ __forceinline__ float wave_max(float v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v = fmaxf(v, __shfl_xor(v, off));
-    return v;
-}
-
 // Lower bound on the reference distance of every window j in [j0, j0 + cnt)
 // of a template of length m, given that each window's
 // N_j = dt_j - (mu_j - c) S is at most nmax: with the windows' common core
@@ -182,17 +174,15 @@ __device__ __forceinline__ float wave_max(float v)
 // and the reference's FP32 value d_ref >= d_exact - E_ref(d_exact) >= dlo -
 // E_ref(dlo) (d - E_ref(d) increases wherever the result can exceed 0), with
 // E_ref as in filter.hpp and |mu_j| / sigma_j <= mu_abs / sigma_lo.
-// -inf when the core cannot bound sigma.
-__device__ double chunk_dist_lo(const Waveform &w, uint64_t j0, int cnt, int m, double Q2, double nmax,
-                                double mu_abs)
+// pa = P[a], pb = P[b] are the prefix pairs at the core's ends a = j0 + cnt - 1
+// and b = j0 + m (Lc = b - a).  -inf when the core cannot bound sigma.
+__device__ double chunk_dist_lo(double2 pa, double2 pb, uint64_t a, uint64_t b, int Lc, int m, double Q2,
+                                double nmax, double mu_abs)
 {
     constexpr double u = 5.9604644775390625e-08;
     constexpr double u64 = 1.1102230246251565e-16;
-    const int Lc = m - (cnt - 1);
     if (Lc < 2 || !(nmax <= 1e300))
         return -INFINITY;  // no core, or a non-finite dot product: examine every window
-    const uint64_t a = j0 + static_cast<uint64_t>(cnt - 1), b = j0 + static_cast<uint64_t>(m);
-    const double2 pa = w.p[a], pb = w.p[b];
     const double S1 = pb.x - pa.x, S2 = pb.y - pa.y;
     const double eS1 = kPrefixGamma * (sqrt(static_cast<double>(b) * fabs(pb.y)) +
                                        sqrt(static_cast<double>(a) * fabs(pa.y))) +
@@ -306,19 +296,14 @@ __device__ __forceinline__ void correlate_piece(float2 *data, __amdgpu_buffer_rs
     __syncthreads();
 }
 
-struct __align__(8) ChunkInfo {
-    float dmax;
-    int lane;  // lane of the maximum (first one)
-};
 
 __global__ __launch_bounds__(kT, 1) void k_fft_fused(
     Waveform w, const TmplDev *__restrict__ tmpl, int t0, int F, const float2 *__restrict__ spectra, int bins,
     const float2 *__restrict__ qspec, const float *__restrict__ qmax, const float4 *__restrict__ pinfo,
     const float2 *__restrict__ pnorm, const float2 *__restrict__ E, uint64_t P, int stride, double kfft, int mode,
-    Cand *__restrict__ cand, float *__restrict__ cand_val, unsigned long long *__restrict__ cand_count, uint64_t cap)
+    CandOut co)
 {
     __shared__ float2 data[kLds];
-    __shared__ ChunkInfo chunk[kChunks];
     __shared__ unsigned char keep[kChunks];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -348,29 +333,45 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
                 continue;
             const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
             const double bound = static_cast<double>(tmpl[t].bound);
+            const uint64_t nwin64 = W - base < static_cast<uint64_t>(kStep) ? W - base : static_cast<uint64_t>(kStep);
+            const int nwin = static_cast<int>(nwin64);
+            const int nch = (nwin + 63) / 64;
+            // Thread ch owns the 64-window chunk ch; its core's prefix pairs
+            // are loaded ahead of the transform.
+            const bool has_ch = tid < nch;
+            const int cnt = has_ch ? (nwin - 64 * tid < 64 ? nwin - 64 * tid : 64) : 0;
+            const int Lc = m - (cnt - 1);
+            const uint64_t ca = base + static_cast<uint64_t>(64 * tid + cnt - 1);
+            const uint64_t cb = base + static_cast<uint64_t>(64 * tid + m);
+            double2 pa = make_double2(0.0, 0.0), pb = make_double2(0.0, 0.0);
+            if (has_ch && mode == kModeMain && Lc >= 2) {
+                pa = w.p[ca];
+                pb = w.p[cb];
+            }
             const __amdgpu_buffer_rsrc_t Hr = rsrc(qspec + static_cast<uint64_t>(ti) * static_cast<uint64_t>(bins),
                                                    bins * 8u);
             correlate_piece(data, Xr, Hr, Er);
 
-            // 3. Filter.  dt of window base + i is y[m - 1 + i] / B.
-            const uint64_t nwin64 = W - base < static_cast<uint64_t>(kStep) ? W - base : static_cast<uint64_t>(kStep);
-            const int nwin = static_cast<int>(nwin64);
-            const int nch = (nwin + 63) / 64;
+            // 3. Filter.  dt of window base + i is y[n] / B, n = m - 1 + i, at
+            // float n + 2 (n >> 5) of the padded array.
             const float invB = 1.0f / static_cast<float>(kB);
-            // 3a. Per chunk: largest dt and its lane.
-            for (int ch = wave; ch < nch; ch += kWaves) {
-                const int i = ch * 64 + lane;
-                float dt = -INFINITY;
-                if (i < nwin) {
-                    const int n = m - 1 + i;
-                    dt = fdata[2 * pad(n >> 1) + (n & 1)] * invB;
+            float dmax = -INFINITY;
+            int arg = 0;
+            bool nonfinite = false;
+            if (has_ch) {
+                const int n0 = m - 1 + 64 * tid;
+#pragma unroll 8
+                for (int l = 0; l < cnt; l++) {
+                    const int n = n0 + l;
+                    const float v = fdata[n + 2 * (n >> 5)];
+                    nonfinite |= !(v == v);
+                    if (v > dmax) {
+                        dmax = v;
+                        arg = l;
+                    }
                 }
-                const float mx = wave_max(dt);
-                const unsigned long long hit = __ballot(dt == mx);
-                if (lane == 0)
-                    chunk[ch] = ChunkInfo{mx, __ffsll(static_cast<long long>(hit)) - 1};
+                dmax *= invB;
             }
-            __syncthreads();
             // Error terms of this (template, piece): FFT error of every dot
             // product plus the rounding of x~ = x0 - c (eta).
             const double sQ2 = sqrt(Q2);
@@ -381,79 +382,56 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
             const double nslack = edot + amax * fabs(S);
             const double mu_abs = fabs(static_cast<double>(c)) + amax;
             if (mode == kModeMain) {
-                // 3b. Chunk tests, one chunk per thread.
-                for (int ch = tid; ch < nch; ch += kT) {
-                    const int cnt = nwin - ch * 64 < 64 ? nwin - ch * 64 : 64;
-                    const double lo = chunk_dist_lo(w, base + static_cast<uint64_t>(ch) * 64, cnt, m, Q2,
-                                                    static_cast<double>(chunk[ch].dmax) + nslack, mu_abs);
-                    keep[ch] = lo > bound ? 0 : 1;
+                if (has_ch) {
+                    const double lo = nonfinite ? -INFINITY
+                                                : chunk_dist_lo(pa, pb, ca, cb, Lc, m, Q2,
+                                                                static_cast<double>(dmax) + nslack, mu_abs);
+                    keep[tid] = lo > bound ? 0 : 1;
                 }
                 __syncthreads();
-                // 3c. Windows of the kept chunks.
+                // Windows of the kept chunks, one wave per chunk.
                 for (int ch = wave; ch < nch; ch += kWaves) {
                     if (!keep[ch])
                         continue;
                     const int i = ch * 64 + lane;
                     bool emit = false;
                     float est = 0.0f;
+                    double ilo = 0.0, ihi = 0.0;
                     const uint64_t j = base + static_cast<uint64_t>(i);
                     if (i < nwin && w.x[j] != kMissing) {
                         const int n = m - 1 + i;
-                        const float dt = fdata[2 * pad(n >> 1) + (n & 1)] * invB;
+                        const float dt = fdata[n + 2 * (n >> 5)] * invB;
                         const DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(dt), eta, eabs);
                         emit = !(iv.lo > bound);  // NaN (undecidable): a candidate
                         est = static_cast<float>(iv.est);
+                        ilo = iv.lo;
+                        ihi = iv.hi;
                     }
                     const unsigned long long mask = __ballot(emit);
                     if (mask) {
                         const int leader = __ffsll(static_cast<long long>(mask)) - 1;
                         unsigned long long slot0 = 0;
                         if (lane == leader)
-                            slot0 = atomicAdd(cand_count, static_cast<unsigned long long>(__popcll(mask)));
+                            slot0 = atomicAdd(co.count, static_cast<unsigned long long>(__popcll(mask)));
                         slot0 = __shfl(slot0, leader);
                         if (emit) {
                             const unsigned long long slot =
                                 slot0 + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
-                            if (slot < cap) {
-                                cand[slot].t = static_cast<uint32_t>(t);
-                                cand[slot].j = static_cast<uint32_t>(j);
-                                cand_val[slot] = est;
-                            }
+                            cand_put(co, slot, static_cast<uint32_t>(t), static_cast<uint32_t>(j), est, ilo, ihi);
                         }
                     }
                 }
             } else {
-                // Seed: per wave, among its chunks the one whose largest dt
-                // is largest relative to the core's spread; that chunk's best
-                // window gives a rigorous upper bound.
-                float best_g = -INFINITY;
-                int best_ch = -1;
-                for (int ch = tid; ch < nch; ch += kT) {
-                    const int cnt = nwin - ch * 64 < 64 ? nwin - ch * 64 : 64;
-                    const int Lc = m - (cnt - 1);
-                    if (Lc < 2)
-                        continue;
-                    const uint64_t a = base + static_cast<uint64_t>(ch) * 64 + static_cast<uint64_t>(cnt - 1);
-                    const double2 pa = w.p[a], pb = w.p[a + static_cast<uint64_t>(Lc)];
-                    const double S1 = pb.x - pa.x;
-                    const double V = (pb.y - pa.y) - S1 * S1 / static_cast<double>(Lc);
-                    if (V > 0.0) {
-                        const float g = chunk[ch].dmax * static_cast<float>(1.0 / sqrt(V));
-                        if (g > best_g) {
-                            best_g = g;
-                            best_ch = ch;
-                        }
-                    }
-                }
+                // Seed: the window of largest dt of each chunk gives a
+                // rigorous upper bound; one entry (the smallest) per wave.
                 float ub = INFINITY;
                 uint32_t bj = 0xffffffffu;
-                if (best_ch >= 0) {
-                    const int i = best_ch * 64 + chunk[best_ch].lane;
+                if (has_ch) {
+                    const int i = 64 * tid + arg;
                     const uint64_t j = base + static_cast<uint64_t>(i);
-                    if (i < nwin && w.x[j] != kMissing) {
-                        const int n = m - 1 + i;
-                        const float dt = fdata[2 * pad(n >> 1) + (n & 1)] * invB;
-                        const DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(dt), eta, eabs);
+                    if (w.x[j] != kMissing) {
+                        const DistInterval iv =
+                            dist_interval(w, j, m, Q2, S, c, static_cast<double>(dmax), eta, eabs);
                         if (iv.hi < 3.0e38) {
                             ub = static_cast<float>(iv.hi) * (1.0f + 1e-6f);
                             bj = static_cast<uint32_t>(j);
@@ -469,14 +447,8 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
                         bj = oj;
                     }
                 }
-                if (lane == 0 && ub < INFINITY) {
-                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
-                    if (slot < cap) {
-                        cand[slot].t = static_cast<uint32_t>(t);
-                        cand[slot].j = bj;
-                        cand_val[slot] = ub;
-                    }
-                }
+                if (lane == 0 && ub < INFINITY)
+                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), bj, ub, ub, ub);
             }
             __syncthreads();  // LDS is rewritten by the next template
         }
@@ -491,9 +463,8 @@ int fft_fused_block() { return kB; }
 
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
-                            const float2 *table, uint64_t P, int stride, double kfft, int mode, Cand *cand,
-                            float *cand_val, unsigned long long *cand_count, uint64_t cap, int grid,
-                            hipStream_t stream)
+                            const float2 *table, uint64_t P, int stride, double kfft, int mode, const CandOut &co,
+                            int grid, hipStream_t stream)
 {
     if (F <= 0 || P == 0)
         return hipSuccess;
@@ -502,7 +473,7 @@ hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int 
     const uint64_t nwork = (P + stride - 1) / stride;
     const unsigned g = static_cast<unsigned>(nwork < static_cast<uint64_t>(grid) ? nwork : grid);
     hipLaunchKernelGGL(k_fft_fused, dim3(g), dim3(kT), 0, stream, w, tmpl, t0, F, spectra, bins, qspec, qmax, pinfo,
-                       pnorm, table, P, stride, kfft, mode, cand, cand_val, cand_count, cap);
+                       pnorm, table, P, stride, kfft, mode, co);
     return hipGetLastError();
 }
 

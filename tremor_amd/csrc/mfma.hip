@@ -171,9 +171,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                                                            int block_stride, int groups, uint32_t nblocks,
                                                            const float2 *__restrict__ rb,
                                                            const float4 *__restrict__ binfo, int rb_len,
-                                                           Cand *__restrict__ cand,
-                                                           float *__restrict__ cand_val,
-                                                           unsigned long long *__restrict__ cand_count, uint64_t cap)
+                                                           CandOut co)
 {
     constexpr int kLds = cmax(PREC == TMX_PREC_FP16 ? kTileBytesF16 : kTileBytesBf16, kEpiBytes);
     __shared__ __attribute__((aligned(16))) unsigned char lds[kLds];
@@ -502,12 +500,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                     const uint64_t j = tile0 + static_cast<uint64_t>(wl);
                     if (j >= W || w.x[j] == kMissing)
                         continue;
-                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
-                    if (slot < cap) {
-                        cand[slot].t = static_cast<uint32_t>(t);
-                        cand[slot].j = static_cast<uint32_t>(j);
-                        cand_val[slot] = NAN;
-                    }
+                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN,
+                             -INFINITY, INFINITY);
                 }
             }
             const int qe = qn < kQ ? qn : kQ;
@@ -531,14 +525,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                     const double ex = 6.103515625e-05 * static_cast<double>(scale) * 128.0 * sqrt(md * Q2) * 1.01;
                     iv.lo -= 2.0 * ex / (sig * (1.0 - 1e-3)) * 1.01;
                 }
-                if (iv.lo <= static_cast<double>(tmpl[t].bound)) {
-                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
-                    if (slot < cap) {
-                        cand[slot].t = static_cast<uint32_t>(t);
-                        cand[slot].j = static_cast<uint32_t>(j);
-                        cand_val[slot] = static_cast<float>(iv.est);
-                    }
-                }
+                if (iv.lo <= static_cast<double>(tmpl[t].bound))
+                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), static_cast<uint32_t>(j),
+                             static_cast<float>(iv.est), iv.lo, iv.hi);
             }
         } else {
             for (int i = 0; i < nt; i++) {
@@ -573,14 +562,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                         bj = oj;
                     }
                 }
-                if (lane == 0 && ub < INFINITY) {
-                    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
-                    if (slot < cap) {
-                        cand[slot].t = static_cast<uint32_t>(t);
-                        cand[slot].j = bj;
-                        cand_val[slot] = ub;
-                    }
-                }
+                if (lane == 0 && ub < INFINITY)
+                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), bj, ub, ub, ub);
             }
         }
     }
@@ -608,8 +591,7 @@ hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, const float *qz, int
 
 hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, const int32_t *group_begin, int groups,
                             int precision, const void *a_frag, const float2 *rb, const float4 *binfo, int rb_len,
-                            int mode, int block_stride, Cand *cand, float *cand_val, unsigned long long *cand_count,
-                            uint64_t cand_cap, hipStream_t stream)
+                            int mode, int block_stride, const CandOut &co, hipStream_t stream)
 {
     (void)count;
     if (groups <= 0 || w.n == 0)
@@ -623,12 +605,10 @@ hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, c
     const Frag *a = static_cast<const Frag *>(a_frag);
     if (precision == TMX_PREC_FP16)
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_FP16>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a, mode,
-                           block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, cand, cand_val,
-                           cand_count, cand_cap);
+                           block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, co);
     else
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_BF16X3>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a,
-                           mode, block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, cand,
-                           cand_val, cand_count, cand_cap);
+                           mode, block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, co);
     return hipGetLastError();
 }
 

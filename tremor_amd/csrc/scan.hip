@@ -26,23 +26,15 @@ static_assert(kWB == kScanWindows, "window block size");
 __device__ inline int xs_at(int e) { return e + ((e >> 4) << 2); }
 constexpr int kXsLen = (kWB + kKC) + ((kWB + kKC) >> 2);
 
-__device__ inline void emit(Cand *cand, float *cand_val, unsigned long long *cand_count, uint64_t cap, uint32_t t,
-                            uint32_t j, float v)
+__device__ inline void emit(const CandOut &co, uint32_t t, uint32_t j, float est, double lo, double hi)
 {
-    const unsigned long long slot = atomicAdd(cand_count, 1ULL);
-    if (slot < cap) {
-        cand[slot].t = t;
-        cand[slot].j = j;
-        cand_val[slot] = v;
-    }
+    cand_put(co, atomicAdd(co.count, 1ULL), t, j, est, lo, hi);
 }
 
 __global__ __launch_bounds__(kThreads) void k_scan_fp32(Waveform w, const TmplDev *__restrict__ tmpl,
                                                         const float *__restrict__ qz,
                                                         const int32_t *__restrict__ group_begin, int mode,
-                                                        int block_stride, Cand *__restrict__ cand,
-                                                        float *__restrict__ cand_val,
-                                                        unsigned long long *__restrict__ cand_count, uint64_t cap)
+                                                        int block_stride, CandOut co)
 {
     __shared__ float xs[kXsLen];
     __shared__ float qs[kKC];
@@ -115,8 +107,8 @@ __global__ __launch_bounds__(kThreads) void k_scan_fp32(Waveform w, const TmplDe
             const DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(acc[r]), eta);
             if (mode == kModeMain) {
                 if (iv.lo <= bound)
-                    emit(cand, cand_val, cand_count, cap, static_cast<uint32_t>(t), static_cast<uint32_t>(j),
-                         static_cast<float>(iv.est));
+                    emit(co, static_cast<uint32_t>(t), static_cast<uint32_t>(j), static_cast<float>(iv.est), iv.lo,
+                         iv.hi);
             } else {
                 const float hi = iv.hi < 3.0e38 ? static_cast<float>(iv.hi) : INFINITY;
                 if (hi < best) {
@@ -136,7 +128,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_fp32(Waveform w, const TmplDe
                 }
             }
             if ((tid & 63) == 0 && best < INFINITY)
-                emit(cand, cand_val, cand_count, cap, static_cast<uint32_t>(t), best_j, best);
+                emit(co, static_cast<uint32_t>(t), best_j, best, best, best);
         }
     }
 }
@@ -144,8 +136,8 @@ __global__ __launch_bounds__(kThreads) void k_scan_fp32(Waveform w, const TmplDe
 }  // namespace
 
 hipError_t launch_scan_fp32(const Waveform &w, const TmplDev *tmpl, const float *qz, int count,
-                            const int32_t *group_begin, int groups, int mode, int block_stride, Cand *cand,
-                            float *cand_val, unsigned long long *cand_count, uint64_t cand_cap, hipStream_t stream)
+                            const int32_t *group_begin, int groups, int mode, int block_stride, const CandOut &co,
+                            hipStream_t stream)
 {
     (void)count;
     if (groups <= 0 || w.n == 0)
@@ -153,8 +145,7 @@ hipError_t launch_scan_fp32(const Waveform &w, const TmplDev *tmpl, const float 
     const uint64_t blocks = (w.n + kWB - 1) / kWB;
     const uint64_t launched = (blocks + block_stride - 1) / block_stride;
     hipLaunchKernelGGL(k_scan_fp32, dim3(static_cast<unsigned>(launched), static_cast<unsigned>(groups)),
-                       dim3(kThreads), 0, stream, w, tmpl, qz, group_begin, mode, block_stride, cand, cand_val,
-                       cand_count, cand_cap);
+                       dim3(kThreads), 0, stream, w, tmpl, qz, group_begin, mode, block_stride, co);
     return hipGetLastError();
 }
 

@@ -85,6 +85,13 @@ __global__ __launch_bounds__(1024) void k_group_offsets(const uint32_t *__restri
         offsets[T] = part[1023];
 }
 
+__device__ inline uint64_t make_key(int keykind, const TmplDev *tmpl, const Cand &c, float v)
+{
+    const uint64_t vb = static_cast<uint64_t>(__float_as_uint(v));
+    const bool value_first = keykind == kKeyValuePos || (keykind == kKeyQuery && tmpl[c.t].k > 0);
+    return value_first ? ((vb << 32) | c.j) : ((static_cast<uint64_t>(c.j) << 32) | vb);
+}
+
 __global__ void k_group_scatter(const Cand *__restrict__ cand, const float *__restrict__ val, uint64_t count,
                                 const TmplDev *__restrict__ tmpl, int pred, int keykind,
                                 const uint32_t *__restrict__ offsets, uint32_t *__restrict__ cursor,
@@ -98,9 +105,7 @@ __global__ void k_group_scatter(const Cand *__restrict__ cand, const float *__re
     if (!pred_ok(pred, tmpl, c.t, v))
         return;
     const uint32_t slot = offsets[c.t] + atomicAdd(&cursor[c.t], 1u);
-    const uint64_t vb = static_cast<uint64_t>(__float_as_uint(v));
-    const bool value_first = keykind == kKeyValuePos || (keykind == kKeyQuery && tmpl[c.t].k > 0);
-    keys[slot] = value_first ? ((vb << 32) | c.j) : ((static_cast<uint64_t>(c.j) << 32) | vb);
+    keys[slot] = make_key(keykind, tmpl, c, v);
 }
 
 __global__ void k_zero_counts(uint32_t *__restrict__ counts, int T)
@@ -109,6 +114,81 @@ __global__ void k_zero_counts(uint32_t *__restrict__ counts, int T)
     if (i < T)
         counts[i] = 0u;
 }
+
+
+// Large inputs: the entries satisfying pred as (key, template) pairs in any
+// order, behind a prefill of sentinels (key ~0, template T) up to count.
+__global__ void k_fill_sentinel(uint64_t *__restrict__ keys, uint32_t *__restrict__ tv, uint64_t count, uint32_t T)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < count) {
+        keys[i] = ~0ull;
+        tv[i] = T;
+    }
+}
+
+__global__ void k_group_compact(const Cand *__restrict__ cand, const float *__restrict__ val, uint64_t count,
+                                const TmplDev *__restrict__ tmpl, int pred, int keykind,
+                                unsigned long long *__restrict__ total, uint64_t *__restrict__ keys,
+                                uint32_t *__restrict__ tv)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    bool ok = false;
+    Cand c{0u, 0u};
+    float v = 0.0f;
+    if (i < count) {
+        c = cand[i];
+        v = val[i];
+        ok = pred_ok(pred, tmpl, c.t, v);
+    }
+    const unsigned long long mask = __ballot(ok);
+    if (!mask)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader)
+        base = atomicAdd(total, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (ok) {
+        const unsigned long long slot = base + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
+        keys[slot] = make_key(keykind, tmpl, c, v);
+        tv[slot] = c.t;
+    }
+}
+
+// Candidates of kNN templates whose lower bound exceeds the (tightened)
+// bound cannot be answers: the rest are compacted into the second set.
+__global__ void k_refine(const Cand *__restrict__ cand, const float *__restrict__ est, const float *__restrict__ lo,
+                         const float *__restrict__ hi, uint64_t count, const TmplDev *__restrict__ tmpl,
+                         Cand *__restrict__ cand2, float *__restrict__ est2, float *__restrict__ lo2,
+                         float *__restrict__ hi2, unsigned long long *__restrict__ total)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    bool ok = false;
+    if (i < count) {
+        const uint32_t t = cand[i].t;
+        ok = !(tmpl[t].k > 0 && lo[i] > tmpl[t].bound);
+    }
+    const unsigned long long mask = __ballot(ok);
+    if (!mask)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader)
+        base = atomicAdd(total, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (ok) {
+        const unsigned long long slot = base + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
+        cand2[slot] = cand[i];
+        est2[slot] = est[i];
+        lo2[slot] = lo[i];
+        hi2[slot] = hi[i];
+    }
+}
+
+__global__ void k_zero_total(unsigned long long *p) { *p = 0ull; }
 
 __device__ inline float key_val(uint64_t k) { return __uint_as_float(static_cast<uint32_t>(k >> 32)); }
 __device__ inline uint32_t key_pos(uint64_t k) { return static_cast<uint32_t>(k); }
@@ -176,8 +256,9 @@ __global__ __launch_bounds__(64) void k_seed_bound(const uint64_t *__restrict__ 
         if (nk >= want)
             U = key_val(kept[want - 1]);
     }
+    // Only ever tightens: a bound from any sample of upper bounds is valid.
     if (threadIdx.x == 0)
-        tmpl[t].bound = U < FLT_MAX ? U : FLT_MAX;
+        tmpl[t].bound = fminf(tmpl[t].bound, U < FLT_MAX ? U : FLT_MAX);
 }
 
 // Canonical kNN answer per kNN template from (d, pos)-sorted exact
@@ -217,12 +298,25 @@ __global__ __launch_bounds__(64) void k_knn_select(const uint64_t *__restrict__ 
 }  // namespace
 
 // Groups the entries satisfying `pred` by template into `keys` (sorted within
-// each template).  offsets has T + 1 slots.  Returns the number grouped via
-// offsets[T] on the device.
+// each template); offsets has T + 1 slots.  Small inputs: a scatter into the
+// template segments and a segmented sort.  Large inputs: the pairs (key,
+// template) compacted behind sentinels and sorted by two stable device-wide
+// radix sorts, by key and then by template (a segmented sort gives one
+// workgroup to each segment, which a single large segment would serialise).
+constexpr uint64_t kSegmentedMax = 1u << 16;
+
+static int template_bits(int T)
+{
+    int b = 1;
+    while ((1u << b) <= static_cast<uint32_t>(T))
+        b++;
+    return b;
+}
+
 hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count, const TmplDev *tmpl, int T,
                              int pred, int keykind, uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
-                             uint64_t *keys_tmp, uint64_t *keys, void *sort_tmp, size_t sort_tmp_bytes,
-                             uint64_t key_capacity, hipStream_t stream)
+                             uint64_t *keys_tmp, uint64_t *keys, uint32_t *tv_a, uint32_t *tv_b,
+                             unsigned long long *total, void *sort_tmp, size_t sort_tmp_bytes, hipStream_t stream)
 {
     // Counters cleared by a kernel, not a memset node: the sequence stays
     // capturable in a HIP graph.
@@ -231,31 +325,75 @@ hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count,
     if (count > 0)
         hipLaunchKernelGGL(k_group_count, dim3(grid), dim3(kThreads), 0, stream, cand, val, count, tmpl, pred, counts);
     hipLaunchKernelGGL(k_group_offsets, dim3(1), dim3(1024), 0, stream, counts, T, offsets, cursor);
-    if (count > 0)
+    if (count == 0)
+        return hipGetLastError();
+    size_t bytes = sort_tmp_bytes;
+    if (count <= kSegmentedMax) {
         hipLaunchKernelGGL(k_group_scatter, dim3(grid), dim3(kThreads), 0, stream, cand, val, count, tmpl, pred,
                            keykind, offsets, cursor, keys_tmp);
-    const hipError_t err = hipGetLastError();
+        const hipError_t err = hipGetLastError();
+        if (err != hipSuccess)
+            return err;
+        return hipcub::DeviceSegmentedRadixSort::SortKeys(sort_tmp, bytes, keys_tmp, keys, static_cast<int>(count),
+                                                         T, offsets, offsets + 1, 0, 64, stream);
+    }
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(grid), dim3(kThreads), 0, stream, keys_tmp, tv_a, count,
+                       static_cast<uint32_t>(T));
+    hipLaunchKernelGGL(k_zero_total, dim3(1), dim3(1), 0, stream, total);
+    hipLaunchKernelGGL(k_group_compact, dim3(grid), dim3(kThreads), 0, stream, cand, val, count, tmpl, pred, keykind,
+                       total, keys_tmp, tv_a);
+    hipError_t err = hipGetLastError();
     if (err != hipSuccess)
         return err;
-    if (count == 0)
-        return hipSuccess;
-    size_t bytes = sort_tmp_bytes;
-    return hipcub::DeviceSegmentedRadixSort::SortKeys(sort_tmp, bytes, keys_tmp, keys, static_cast<int>(count),
-                                                     T, offsets, offsets + 1, 0, 64, stream);
-    (void)key_capacity;
+    // By key (keys_tmp -> keys, templates along), then stably by template
+    // (keys -> keys_tmp), then back into keys.
+    err = hipcub::DeviceRadixSort::SortPairs(sort_tmp, bytes, keys_tmp, keys, tv_a, tv_b, static_cast<int>(count), 0,
+                                             64, stream);
+    if (err != hipSuccess)
+        return err;
+    bytes = sort_tmp_bytes;
+    err = hipcub::DeviceRadixSort::SortPairs(sort_tmp, bytes, tv_b, tv_a, keys, keys_tmp, static_cast<int>(count), 0,
+                                             template_bits(T), stream);
+    if (err != hipSuccess)
+        return err;
+    return hipMemcpyAsync(keys, keys_tmp, sizeof(uint64_t) * count, hipMemcpyDeviceToDevice, stream);
 }
 
 size_t group_sort_tmp_bytes(uint64_t count, int T)
 {
-    size_t bytes = 0;
-    (void)hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, bytes, static_cast<const uint64_t *>(nullptr),
-                                               static_cast<uint64_t *>(nullptr), static_cast<int>(count), T,
-                                               static_cast<const uint32_t *>(nullptr),
-                                               static_cast<const uint32_t *>(nullptr), 0, 64);
-    return bytes;
+    size_t a = 0, b = 0, c = 0;
+    (void)hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, a, static_cast<const uint64_t *>(nullptr),
+                                                     static_cast<uint64_t *>(nullptr),
+                                                     static_cast<int>(count < kSegmentedMax ? count : kSegmentedMax),
+                                                     T, static_cast<const uint32_t *>(nullptr),
+                                                     static_cast<const uint32_t *>(nullptr), 0, 64);
+    if (count > kSegmentedMax) {
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<const uint64_t *>(nullptr),
+                                                 static_cast<uint64_t *>(nullptr),
+                                                 static_cast<const uint32_t *>(nullptr),
+                                                 static_cast<uint32_t *>(nullptr), static_cast<int>(count), 0, 64);
+        (void)hipcub::DeviceRadixSort::SortPairs(nullptr, c, static_cast<const uint32_t *>(nullptr),
+                                                 static_cast<uint32_t *>(nullptr),
+                                                 static_cast<const uint64_t *>(nullptr),
+                                                 static_cast<uint64_t *>(nullptr), static_cast<int>(count), 0,
+                                                 template_bits(T));
+    }
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
 }
 
-// max_want: the largest bound-set size (2k - 1 or k) over the templates.
+// Drops the kNN candidates whose lower bound exceeds their template's bound;
+// the kept ones go to the second set, their number to *total.
+hipError_t launch_refine(const Cand *cand, const float *est, const float *lo, const float *hi, uint64_t count,
+                         const TmplDev *tmpl, Cand *cand2, float *est2, float *lo2, float *hi2,
+                         unsigned long long *total, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_total, dim3(1), dim3(1), 0, stream, total);
+    if (count > 0)
+        hipLaunchKernelGGL(k_refine, dim3(static_cast<unsigned>((count + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           stream, cand, est, lo, hi, count, tmpl, cand2, est2, lo2, hi2, total);
+    return hipGetLastError();
+}
+
 hipError_t launch_seed_bound(const uint64_t *keys, const uint32_t *offsets, TmplDev *tmpl, int T, int max_want,
                              hipStream_t stream)
 {

@@ -29,10 +29,13 @@
 
 namespace tmx {
 // select.hip
-hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count, const TmplDev *tmpl, int T, int pred,
-                             int keykind, uint32_t *counts, uint32_t *offsets, uint32_t *cursor, uint64_t *keys_tmp,
-                             uint64_t *keys, void *sort_tmp, size_t sort_tmp_bytes, uint64_t key_capacity,
-                             hipStream_t stream);
+hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count, const TmplDev *tmpl, int T,
+                             int pred, int keykind, uint32_t *counts, uint32_t *offsets, uint32_t *cursor,
+                             uint64_t *keys_tmp, uint64_t *keys, uint32_t *tv_a, uint32_t *tv_b,
+                             unsigned long long *total, void *sort_tmp, size_t sort_tmp_bytes, hipStream_t stream);
+hipError_t launch_refine(const Cand *cand, const float *est, const float *lo, const float *hi, uint64_t count,
+                         const TmplDev *tmpl, Cand *cand2, float *est2, float *lo2, float *hi2,
+                         unsigned long long *total, hipStream_t stream);
 size_t group_sort_tmp_bytes(uint64_t count, int T);
 hipError_t launch_seed_bound(const uint64_t *keys, const uint32_t *offsets, TmplDev *tmpl, int T, int max_want,
                              hipStream_t stream);
@@ -43,8 +46,7 @@ hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, const float *qz, int
                             hipStream_t stream);
 hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, const int32_t *group_begin, int groups,
                             int precision, const void *a_frag, const float2 *rb, const float4 *binfo, int rb_len,
-                            int mode, int block_stride, Cand *cand, float *cand_val, unsigned long long *cand_count,
-                            uint64_t cand_cap, hipStream_t stream);
+                            int mode, int block_stride, const CandOut &co, hipStream_t stream);
 int mfma_group_size();
 uint64_t mfma_block_windows();
 int64_t mfma_frag_count(int kpad);
@@ -57,9 +59,8 @@ int fft_fused_table_len();
 int fft_fused_block();
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
-                            const float2 *table, uint64_t P, int stride, double kfft, int mode, Cand *cand,
-                            float *cand_val, unsigned long long *cand_count, uint64_t cap, int grid,
-                            hipStream_t stream);
+                            const float2 *table, uint64_t P, int stride, double kfft, int mode, const CandOut &co,
+                            int grid, hipStream_t stream);
 hipError_t launch_piece_norm(const double *norm2, float2 *pnorm, uint64_t P, hipStream_t stream);
 hipError_t launch_spec_max(const float2 *spec, int rows, int bins, float *out, int stride_out, hipStream_t stream);
 hipError_t launch_tmpl_input(const TmplDev *tmpl, const int32_t *ids, int count, int B, const float *qz, float *out,
@@ -68,8 +69,8 @@ hipError_t launch_product(const FftPair *pairs, int npairs, const float2 *spectr
                           const int32_t *slot, int bins, float2 *Y, hipStream_t stream);
 hipError_t launch_fft_filter(const Waveform &w, const TmplDev *tmpl, const FftPair *pairs, int npairs, const float *y,
                              int B, int step, const float2 *rb, int rb_len, const float4 *pinfo, const float2 *pnorm,
-                             const float *qmax, const int32_t *slot, double kfft, int mode, Cand *cand,
-                             float *cand_val, unsigned long long *cand_count, uint64_t cap, hipStream_t stream);
+                             const float *qmax, const int32_t *slot, double kfft, int mode, const CandOut &co,
+                             hipStream_t stream);
 hipError_t launch_zero_f64(double *p, uint64_t count, hipStream_t stream);
 double fft_error_factor(int B);
 // misc.hip
@@ -440,14 +441,19 @@ int upload_and_prepare(Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const 
 int ensure_cand(Lane &L, uint64_t cap, int T)
 {
     if (cap > L.cand_cap) {
-        LANE_HIP(L.cand.reserve(sizeof(Cand) * cap), "hipMalloc candidates");
-        LANE_HIP(L.cand_val.reserve(sizeof(float) * cap), "hipMalloc candidates");
-        LANE_HIP(L.cand_d.reserve(sizeof(float) * cap), "hipMalloc candidates");
-        LANE_HIP(L.keys_tmp.reserve(sizeof(uint64_t) * cap), "hipMalloc candidate keys");
-        LANE_HIP(L.keys.reserve(sizeof(uint64_t) * cap), "hipMalloc candidate keys");
+        L.cand_cap = 0;
+        for (DevMem *d : {&L.cand, &L.cand2})
+            LANE_HIP(d->reserve(sizeof(Cand) * cap), "hipMalloc candidates");
+        for (DevMem *d : {&L.cand_val, &L.cand_lo, &L.cand_hi, &L.cand_d, &L.val2, &L.lo2, &L.hi2})
+            LANE_HIP(d->reserve(sizeof(float) * cap), "hipMalloc candidates");
+        for (DevMem *d : {&L.keys_tmp, &L.keys})
+            LANE_HIP(d->reserve(sizeof(uint64_t) * cap), "hipMalloc candidate keys");
+        for (DevMem *d : {&L.tv_a, &L.tv_b})
+            LANE_HIP(d->reserve(sizeof(uint32_t) * cap), "hipMalloc candidate keys");
         L.cand_cap = cap;
     }
     LANE_HIP(L.cand_count.reserve(sizeof(unsigned long long)), "hipMalloc counter");
+    LANE_HIP(L.total.reserve(sizeof(unsigned long long)), "hipMalloc counter");
     LANE_HIP(L.counts.reserve(sizeof(uint32_t) * (T + 1)), "hipMalloc counts");
     LANE_HIP(L.offsets.reserve(sizeof(uint32_t) * (T + 1)), "hipMalloc offsets");
     LANE_HIP(L.cursor.reserve(sizeof(uint32_t) * (T + 1)), "hipMalloc cursors");
@@ -456,6 +462,12 @@ int ensure_cand(Lane &L, uint64_t cap, int T)
 }
 
 // The FFT form of a pass over every `stride`-th piece.
+CandOut cand_out(Lane &L)
+{
+    return CandOut{L.cand.as<Cand>(), L.cand_val.as<float>(), L.cand_lo.as<float>(), L.cand_hi.as<float>(),
+                   L.cand_count.as<unsigned long long>(), L.cand_cap};
+}
+
 int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f, int mode, int stride)
 {
     const int F = plan.T - plan.Tg;
@@ -476,9 +488,8 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
     if (f.fused) {
         LANE_HIP(launch_fft_fused(w, L.tmpl.as<TmplDev>(), plan.Tg, F, f.spectra.as<float2>(), f.bins,
                                   L.qspec.as<float2>(), L.qmax.as<float>(), f.pinfo.as<float4>(), f.pnorm.as<float2>(),
-                                  f.table.as<float2>(), f.P, stride, kfft, mode, L.cand.as<Cand>(),
-                                  L.cand_val.as<float>(), L.cand_count.as<unsigned long long>(), L.cand_cap,
-                                  L.cus > 0 ? L.cus : 256, L.stream),
+                                  f.table.as<float2>(), f.P, stride, kfft, mode, cand_out(L), L.cus > 0 ? L.cus : 256,
+                                  L.stream),
                  "fused FFT pass");
         return 0;
     }
@@ -510,8 +521,7 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
             return L.fail("hipFFT c2r failed", -2);
         LANE_HIP(launch_fft_filter(w, L.tmpl.as<TmplDev>(), pp, cnt, L.y.as<float>(), f.B, f.step, f.rb.as<float2>(),
                                    f.B + 1, f.pinfo.as<float4>(), f.pnorm.as<float2>(), L.qmax.as<float>(),
-                                   L.slot.as<int32_t>(), kfft, mode, L.cand.as<Cand>(), L.cand_val.as<float>(),
-                                   L.cand_count.as<unsigned long long>(), L.cand_cap, L.stream),
+                                   L.slot.as<int32_t>(), kfft, mode, cand_out(L), L.stream),
                  "FFT filter");
     }
     return 0;
@@ -538,14 +548,12 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     if (groups > 0) {
         if (cfg.precision == TMX_PREC_FP32) {
             LANE_HIP(launch_scan_fp32(w, L.tmpl.as<TmplDev>(), L.qz.as<float>(), plan.T, L.groups.as<int32_t>(),
-                                      groups, mode, block_stride, L.cand.as<Cand>(), L.cand_val.as<float>(),
-                                      L.cand_count.as<unsigned long long>(), L.cand_cap, L.stream),
+                                      groups, mode, block_stride, cand_out(L), L.stream),
                      "FP32 scan");
         } else {
             LANE_HIP(launch_scan_mfma(w, L.tmpl.as<TmplDev>(), plan.T, L.groups.as<int32_t>(), groups, cfg.precision,
                                       L.afrag.p, ctx->rb.as<float2>(), ctx->binfo.as<float4>(), ctx->rb_len, mode,
-                                      block_stride, L.cand.as<Cand>(), L.cand_val.as<float>(),
-                                      L.cand_count.as<unsigned long long>(), L.cand_cap, L.stream),
+                                      block_stride, cand_out(L), L.stream),
                      "MFMA scan");
         }
     }
@@ -582,14 +590,21 @@ int run_main(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     }
 }
 
+void swap_mem(DevMem &a, DevMem &b)
+{
+    std::swap(a.p, b.p);
+    std::swap(a.bytes, b.bytes);
+}
+
 int group_sort(Lane &L, const float *val, uint64_t count, int T, int pred, int keykind)
 {
     const size_t tmp = group_sort_tmp_bytes(count, T);
     LANE_HIP(L.sort_tmp.reserve(tmp + 256), "hipMalloc sort scratch");
     LANE_HIP(launch_group_sort(L.cand.as<Cand>(), val, count, L.tmpl.as<TmplDev>(), T, pred, keykind,
                                L.counts.as<uint32_t>(), L.offsets.as<uint32_t>(), L.cursor.as<uint32_t>(),
-                               L.keys_tmp.as<uint64_t>(), L.keys.as<uint64_t>(), L.sort_tmp.p, L.sort_tmp.bytes,
-                               L.cand_cap, L.stream),
+                               L.keys_tmp.as<uint64_t>(), L.keys.as<uint64_t>(), L.tv_a.as<uint32_t>(),
+                               L.tv_b.as<uint32_t>(), L.total.as<unsigned long long>(), L.sort_tmp.p,
+                               L.sort_tmp.bytes, L.stream),
              "group/sort");
     return 0;
 }
@@ -666,6 +681,31 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan
     const double t3 = now_s();
     st.seconds_main = t3 - t2;
     st.candidates = static_cast<int64_t>(ncand);
+    // Refinement (kNN templates): the candidates' own upper bounds give a
+    // bound at least as tight as the seed's (the k-th, or 2k-1-th separated,
+    // smallest); candidates whose lower bound exceeds it cannot be answers
+    // and skip the exact recomputation.
+    if (plan.max_want > 0 && ncand > 2 * static_cast<uint64_t>(plan.out_slots) + 1024) {
+        if ((rc = group_sort(L, L.cand_hi.as<float>(), ncand, T, kPredFiniteH, kKeyValuePosH)))
+            return rc;
+        LANE_HIP(launch_seed_bound(L.keys.as<uint64_t>(), L.offsets.as<uint32_t>(), L.tmpl.as<TmplDev>(), T,
+                                   plan.max_want, L.stream),
+                 "refined bound");
+        LANE_HIP(launch_refine(L.cand.as<Cand>(), L.cand_val.as<float>(), L.cand_lo.as<float>(),
+                               L.cand_hi.as<float>(), ncand, L.tmpl.as<TmplDev>(), L.cand2.as<Cand>(),
+                               L.val2.as<float>(), L.lo2.as<float>(), L.hi2.as<float>(),
+                               L.total.as<unsigned long long>(), L.stream),
+                 "refinement");
+        unsigned long long kept = 0;
+        LANE_HIP(hipMemcpyAsync(&kept, L.total.p, sizeof(kept), hipMemcpyDeviceToHost, L.stream), "read count");
+        LANE_HIP(hipStreamSynchronize(L.stream), "refinement");
+        swap_mem(L.cand, L.cand2);
+        swap_mem(L.cand_val, L.val2);
+        swap_mem(L.cand_lo, L.lo2);
+        swap_mem(L.cand_hi, L.hi2);
+        ncand = kept;
+    }
+    st.verified = static_cast<int64_t>(ncand);
     if ((rc = verify_and_err(ctx, L, ncand, &st)))
         return rc;
     const double t4 = now_s();

@@ -47,6 +47,47 @@ struct Cand {
     uint32_t j;  // window start
 };
 
+// Output of a candidate pass.  Main mode: the window, the fast estimate of
+// its reference distance and bounds [lo, hi] on it (floats rounded outward;
+// -inf / +inf when unknown).  Seed mode: est holds the entry's upper bound.
+struct CandOut {
+    Cand *cand;
+    float *est;
+    float *lo;
+    float *hi;
+    unsigned long long *count;
+    uint64_t cap;
+};
+
+__device__ inline float round_down_f(double v)
+{
+    float f = static_cast<float>(v);
+    if (static_cast<double>(f) > v)
+        f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+__device__ inline float round_up_f(double v)
+{
+    float f = static_cast<float>(v);
+    if (static_cast<double>(f) < v)
+        f = nextafterf(f, INFINITY);
+    return f;
+}
+
+// Stores one entry in a slot already claimed from o.count.
+__device__ inline void cand_put(const CandOut &o, unsigned long long slot, uint32_t t, uint32_t j, float est,
+                                double lo, double hi)
+{
+    if (slot < o.cap) {
+        o.cand[slot].t = t;
+        o.cand[slot].j = j;
+        o.est[slot] = est;
+        o.lo[slot] = round_down_f(lo);
+        o.hi[slot] = round_up_f(hi);
+    }
+}
+
 // Filter mode of the inner-product passes.
 enum FilterMode : int {
     kModeMain = 0,  // emit every window whose distance lower bound <= bound
@@ -83,8 +124,7 @@ hipError_t launch_prepare(const float *raw, TmplDev *tmpl, int count, float *qz,
 
 // Plain FP32 sliding-dot pass with the fused candidate filter.
 hipError_t launch_scan_fp32(const Waveform &w, const TmplDev *tmpl, const float *qz, int count,
-                            const int32_t *group_begin, int groups, int mode, int block_stride, Cand *cand,
-                            float *cand_val, unsigned long long *cand_count, uint64_t cand_cap,
+                            const int32_t *group_begin, int groups, int mode, int block_stride, const CandOut &co,
                             hipStream_t stream);
 constexpr int kScanWindows = 4096;  // windows per workgroup of the FP32 scan
 
