@@ -276,3 +276,27 @@ def test_cost_model_routes_long_templates(oracle):
         po, do, _ = oracle.scan_knn(x, tm, 5, 0)
         assert np.array_equal(pos, po)
         assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+
+
+def test_fused_fft_many_templates_matches_mfma(oracle):
+    """A batch large enough that the FFT seed pass queues more chunk records
+    than the queue's default size: every template must still get its seed,
+    and the FFT and matrix-core forms must return the same exact answers."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(4_000_000, 31, events=200)
+    lengths = rng.integers(500, 8001, size=64)
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    with Context(precision="bf16x3", fft="always") as ctx:
+        ctx.load_waveform(x)
+        _log("fused-many", templates=len(tm))
+        pos, d, st = ctx.knn(tm, 100, 0)
+        print("  stats", st, flush=True)
+        assert st["fft_templates"] == len(tm)
+        ctx.set_fft_mode("off")
+        pos2, d2, _ = ctx.knn(tm, 100, 0)
+        assert np.array_equal(pos, pos2)
+        assert np.array_equal(d.view(np.uint32), d2.view(np.uint32))
+    po, do, _ = oracle.scan_knn(x, tm[:2], 100, 0)
+    assert np.array_equal(pos[:2], po)
+    assert np.array_equal(d[:2].view(np.uint32), do.view(np.uint32))

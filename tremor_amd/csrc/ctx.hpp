@@ -84,6 +84,10 @@ struct Lane {
     uint64_t cand_cap = 0;
     // FFT form: template spectra, products, inverse transforms
     DevMem Y, y, pairs, qspec, qmax, qin, ids, slot;
+    // fused FFT form: queue of chunks for the per-window pass
+    DevMem chunk_rec, chunk_dt, chunk_n;
+    uint64_t chunk_cap = 0;
+    uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
     hipfftHandle tfwd = 0, inv = 0;
     int tfwd_B = 0, inv_B = 0, inv_batch = 0;
 

@@ -42,9 +42,9 @@ namespace {
 constexpr int kM = 16384;            // complex points of the half-length transform
 constexpr int kB = 2 * kM;           // real length B
 constexpr int kT = 1024;             // threads per workgroup
-constexpr int kWaves = kT / 64;
 constexpr int kStep = kB - kB / 4 + 1;        // windows per piece
 constexpr int kChunks = (kStep + 63) / 64;    // 64-window chunks per piece
+static_assert(kChunks <= kT, "one thread per chunk");
 constexpr int kLds = kM + (kM >> 4) + 2;      // padded complex slots (index M included)
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
@@ -147,22 +147,17 @@ __device__ __forceinline__ void twiddle16(float2 *v, __amdgpu_buffer_rsrc_t E, i
 {
     const uint32_t o = static_cast<uint32_t>(k * F) * 8u;
     const float2 p1 = bload(E, o, 0), p2 = bload(E, 2 * o, 0), p3 = bload(E, 3 * o, 0);
-    const float2 c1 = bload(E, 4 * o, 0), c2 = bload(E, 8 * o, 0), c3 = bload(E, 12 * o, 0);
     v[1] = cmul(v[1], p1);
     v[2] = cmul(v[2], p2);
     v[3] = cmul(v[3], p3);
-    v[4] = cmul(v[4], c1);
-    v[5] = cmul(v[5], cmul(c1, p1));
-    v[6] = cmul(v[6], cmul(c1, p2));
-    v[7] = cmul(v[7], cmul(c1, p3));
-    v[8] = cmul(v[8], c2);
-    v[9] = cmul(v[9], cmul(c2, p1));
-    v[10] = cmul(v[10], cmul(c2, p2));
-    v[11] = cmul(v[11], cmul(c2, p3));
-    v[12] = cmul(v[12], c3);
-    v[13] = cmul(v[13], cmul(c3, p1));
-    v[14] = cmul(v[14], cmul(c3, p2));
-    v[15] = cmul(v[15], cmul(c3, p3));
+#pragma unroll
+    for (int q = 1; q < 4; q++) {
+        const float2 cq = bload(E, 4 * q * o, 0);
+        v[4 * q] = cmul(v[4 * q], cq);
+        v[4 * q + 1] = cmul(v[4 * q + 1], cmul(cq, p1));
+        v[4 * q + 2] = cmul(v[4 * q + 2], cmul(cq, p2));
+        v[4 * q + 3] = cmul(v[4 * q + 3], cmul(cq, p3));
+    }
 }
 
 // Lower bound on the reference distance of every window j in [j0, j0 + cnt)
@@ -213,20 +208,20 @@ __device__ double chunk_dist_lo(double2 pa, double2 pb, uint64_t a, uint64_t b, 
 // Steps 1 and 2 for one (piece, template): on return data[pad(r)] holds
 // z[r] = y[2r] + i y[2r+1], the unnormalised circular correlation of the
 // piece with the template (all threads synchronised).
-__device__ __forceinline__ void correlate_piece(float2 *data, __amdgpu_buffer_rsrc_t Xr, __amdgpu_buffer_rsrc_t Hr,
+__device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, float2 Xn, __amdgpu_buffer_rsrc_t Hr,
                                                 __amdgpu_buffer_rsrc_t Er)
 {
     const int tid = opaque(threadIdx.x);
     const uint32_t toff = static_cast<uint32_t>(tid) * 8u;
-    // 1. Y = X .* H into LDS.
+    // 1. Y = X .* H into LDS (X: bins tid + 1024 r, and M for thread 0).
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        v[r] = cmul(bload(Xr, toff, 8192u * r), bload(Hr, toff, 8192u * r));
+        v[r] = cmul(X[r], bload(Hr, toff, 8192u * r));
         data[pad(tid + 1024 * r)] = v[r];
     }
     if (tid == 0)
-        data[pad(kM)] = cmul(bload(Xr, 0, 8u * kM), bload(Hr, 0, 8u * kM));
+        data[pad(kM)] = cmul(Xn, bload(Hr, 0, 8u * kM));
     __syncthreads();
     // 2a. Z formation (registers) = the first pass's inputs; the
     // factor e^{i pi k / M} for k = tid + 1024 r is E[tid] e^{i pi r / 16}.
@@ -297,17 +292,27 @@ __device__ __forceinline__ void correlate_piece(float2 *data, __amdgpu_buffer_rs
 }
 
 
+// A chunk that may hold a window within the bound (main mode: its windows'
+// dot products follow in the dt queue, 64 slots per record), or a seed
+// window (cnt = 1: the chunk's most promising window).
+struct ChunkRec {
+    uint32_t t;   // template
+    uint32_t j0;  // first window
+    int32_t cnt;  // windows
+    float c;      // the piece's centre
+    float eabs;   // FFT error bound of the dot products (rounded up)
+    uint32_t pad0;
+};
+
 __global__ __launch_bounds__(kT, 1) void k_fft_fused(
     Waveform w, const TmplDev *__restrict__ tmpl, int t0, int F, const float2 *__restrict__ spectra, int bins,
     const float2 *__restrict__ qspec, const float *__restrict__ qmax, const float4 *__restrict__ pinfo,
     const float2 *__restrict__ pnorm, const float2 *__restrict__ E, uint64_t P, int stride, double kfft, int mode,
-    CandOut co)
+    ChunkRec *__restrict__ rec, float *__restrict__ rec_dt, unsigned long long *__restrict__ nrec, uint64_t rec_cap)
 {
     __shared__ float2 data[kLds];
-    __shared__ unsigned char keep[kChunks];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
     const float *fdata = reinterpret_cast<const float *>(data);
     constexpr double u = 5.9604644775390625e-08;
     const uint64_t nwork = (P + stride - 1) / stride;
@@ -316,8 +321,18 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
     for (uint64_t it = blockIdx.x; it < nwork; it += gridDim.x) {
         const uint64_t p = it * static_cast<uint64_t>(stride);
         const uint64_t base = p * static_cast<uint64_t>(kStep);
-        // The piece spectrum (re-read per template: the repeats hit L2).
-        const __amdgpu_buffer_rsrc_t Xr = rsrc(spectra + p * static_cast<uint64_t>(bins), bins * 8u);
+        // The piece spectrum, once per piece, in registers for every template.
+        float2 X[16];
+        float2 Xn = make_float2(0.0f, 0.0f);
+        {
+            const __amdgpu_buffer_rsrc_t Xr = rsrc(spectra + p * static_cast<uint64_t>(bins), bins * 8u);
+            const uint32_t toff = static_cast<uint32_t>(opaque(tid)) * 8u;
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                X[r] = bload(Xr, toff, 8192u * r);
+            if (tid == 0)
+                Xn = bload(Xr, 0, 8u * kM);
+        }
         const float4 pi = pinfo[p];
         const float c = pi.x;
         const double amax = static_cast<double>(pi.w) * (1.0 + 1e-6);
@@ -331,8 +346,6 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
             const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
             if (base >= W)
                 continue;
-            const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
-            const double bound = static_cast<double>(tmpl[t].bound);
             const uint64_t nwin64 = W - base < static_cast<uint64_t>(kStep) ? W - base : static_cast<uint64_t>(kStep);
             const int nwin = static_cast<int>(nwin64);
             const int nch = (nwin + 63) / 64;
@@ -344,22 +357,22 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
             const uint64_t ca = base + static_cast<uint64_t>(64 * tid + cnt - 1);
             const uint64_t cb = base + static_cast<uint64_t>(64 * tid + m);
             double2 pa = make_double2(0.0, 0.0), pb = make_double2(0.0, 0.0);
-            if (has_ch && mode == kModeMain && Lc >= 2) {
+            if (has_ch && Lc >= 2) {
                 pa = w.p[ca];
                 pb = w.p[cb];
             }
             const __amdgpu_buffer_rsrc_t Hr = rsrc(qspec + static_cast<uint64_t>(ti) * static_cast<uint64_t>(bins),
                                                    bins * 8u);
-            correlate_piece(data, Xr, Hr, Er);
+            correlate_piece(data, X, Xn, Hr, Er);
 
             // 3. Filter.  dt of window base + i is y[n] / B, n = m - 1 + i, at
             // float n + 2 (n >> 5) of the padded array.
             const float invB = 1.0f / static_cast<float>(kB);
+            const int n0 = m - 1 + 64 * tid;
             float dmax = -INFINITY;
             int arg = 0;
             bool nonfinite = false;
             if (has_ch) {
-                const int n0 = m - 1 + 64 * tid;
 #pragma unroll 8
                 for (int l = 0; l < cnt; l++) {
                     const int n = n0 + l;
@@ -374,86 +387,120 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
             }
             // Error terms of this (template, piece): FFT error of every dot
             // product plus the rounding of x~ = x0 - c (eta).
+            const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
             const double sQ2 = sqrt(Q2);
             const double eabs = kfft * (static_cast<double>(pn.x) * static_cast<double>(qmax[ti]) +
                                         sQ2 * static_cast<double>(pn.y));
             const double eta = 1.01 * u;
-            const double edot = eta * sQ2 * static_cast<double>(pn.x) * (1.0 + 1e-6) + eabs;
-            const double nslack = edot + amax * fabs(S);
-            const double mu_abs = fabs(static_cast<double>(c)) + amax;
             if (mode == kModeMain) {
                 if (has_ch) {
+                    const double edot = eta * sQ2 * static_cast<double>(pn.x) * (1.0 + 1e-6) + eabs;
+                    const double nslack = edot + amax * fabs(S);
+                    const double mu_abs = fabs(static_cast<double>(c)) + amax;
+                    const double bound = static_cast<double>(tmpl[t].bound);
                     const double lo = nonfinite ? -INFINITY
                                                 : chunk_dist_lo(pa, pb, ca, cb, Lc, m, Q2,
                                                                 static_cast<double>(dmax) + nslack, mu_abs);
-                    keep[tid] = lo > bound ? 0 : 1;
-                }
-                __syncthreads();
-                // Windows of the kept chunks, one wave per chunk.
-                for (int ch = wave; ch < nch; ch += kWaves) {
-                    if (!keep[ch])
-                        continue;
-                    const int i = ch * 64 + lane;
-                    bool emit = false;
-                    float est = 0.0f;
-                    double ilo = 0.0, ihi = 0.0;
-                    const uint64_t j = base + static_cast<uint64_t>(i);
-                    if (i < nwin && w.x[j] != kMissing) {
-                        const int n = m - 1 + i;
-                        const float dt = fdata[n + 2 * (n >> 5)] * invB;
-                        const DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(dt), eta, eabs);
-                        emit = !(iv.lo > bound);  // NaN (undecidable): a candidate
-                        est = static_cast<float>(iv.est);
-                        ilo = iv.lo;
-                        ihi = iv.hi;
-                    }
-                    const unsigned long long mask = __ballot(emit);
-                    if (mask) {
-                        const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-                        unsigned long long slot0 = 0;
-                        if (lane == leader)
-                            slot0 = atomicAdd(co.count, static_cast<unsigned long long>(__popcll(mask)));
-                        slot0 = __shfl(slot0, leader);
-                        if (emit) {
-                            const unsigned long long slot =
-                                slot0 + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
-                            cand_put(co, slot, static_cast<uint32_t>(t), static_cast<uint32_t>(j), est, ilo, ihi);
+                    if (!(lo > bound)) {
+                        const unsigned long long slot = atomicAdd(nrec, 1ULL);
+                        if (slot < rec_cap) {
+                            rec[slot] = ChunkRec{static_cast<uint32_t>(t), static_cast<uint32_t>(base + 64 * tid), cnt,
+                                                 c, round_up_f(eabs), 0u};
+                            float *dst = rec_dt + slot * 64;
+                            for (int l = 0; l < cnt; l++) {
+                                const int n = n0 + l;
+                                dst[l] = fdata[n + 2 * (n >> 5)] * invB;
+                            }
                         }
                     }
                 }
             } else {
-                // Seed: the window of largest dt of each chunk gives a
-                // rigorous upper bound; one entry (the smallest) per wave.
-                float ub = INFINITY;
-                uint32_t bj = 0xffffffffu;
-                if (has_ch) {
-                    const int i = 64 * tid + arg;
-                    const uint64_t j = base + static_cast<uint64_t>(i);
-                    if (w.x[j] != kMissing) {
-                        const DistInterval iv =
-                            dist_interval(w, j, m, Q2, S, c, static_cast<double>(dmax), eta, eabs);
-                        if (iv.hi < 3.0e38) {
-                            ub = static_cast<float>(iv.hi) * (1.0f + 1e-6f);
-                            bj = static_cast<uint32_t>(j);
-                        }
-                    }
+                // Seed: per wave, the chunk whose largest dt is largest
+                // relative to its core's spread; its best window is queued
+                // for a rigorous upper bound.
+                float g = -INFINITY;
+                if (has_ch && !nonfinite && Lc >= 2) {
+                    const double S1 = pb.x - pa.x;
+                    const double V = (pb.y - pa.y) - S1 * S1 / static_cast<double>(Lc);
+                    if (V > 0.0)
+                        g = dmax * static_cast<float>(1.0 / sqrt(V));
                 }
+                int who = lane;
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) {
-                    const float ov = __shfl_xor(ub, off);
-                    const uint32_t oj = __shfl_xor(bj, off);
-                    if (ov < ub || (ov == ub && oj < bj)) {
-                        ub = ov;
-                        bj = oj;
+                    const float og = __shfl_xor(g, off);
+                    const int ow = __shfl_xor(who, off);
+                    if (og > g || (og == g && ow < who)) {
+                        g = og;
+                        who = ow;
                     }
                 }
-                if (lane == 0 && ub < INFINITY)
-                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), bj, ub, ub, ub);
+                if (lane == who && g > -INFINITY) {
+                    const unsigned long long slot = atomicAdd(nrec, 1ULL);
+                    if (slot < rec_cap) {
+                        rec[slot] = ChunkRec{static_cast<uint32_t>(t),
+                                             static_cast<uint32_t>(base + 64 * tid + arg), 1, c, round_up_f(eabs), 0u};
+                        rec_dt[slot * 64] = dmax;
+                    }
+                }
             }
             __syncthreads();  // LDS is rewritten by the next template
         }
     }
 }
+
+// The queued chunks, one wavefront each: the rigorous interval of every
+// window (filter.hpp) decides the candidates (main), or gives the seed
+// window's upper bound (seed).
+__global__ __launch_bounds__(256) void k_chunk_windows(Waveform w, const TmplDev *__restrict__ tmpl,
+                                                       const ChunkRec *__restrict__ rec,
+                                                       const float *__restrict__ rec_dt,
+                                                       const unsigned long long *__restrict__ nrec, uint64_t rec_cap,
+                                                       int mode, CandOut co)
+{
+    constexpr double eta = 1.01 * 5.9604644775390625e-08;
+    const uint64_t n = *nrec < rec_cap ? *nrec : rec_cap;
+    const int lane = threadIdx.x & 63;
+    const uint64_t wv = (static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
+    const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * 256) >> 6;
+    for (uint64_t r = wv; r < n; r += nw) {
+        const ChunkRec R = rec[r];
+        const int t = static_cast<int>(R.t);
+        const int m = tmpl[t].m;
+        const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
+        const uint64_t j = static_cast<uint64_t>(R.j0) + static_cast<uint64_t>(lane);
+        bool emit = false;
+        float est = 0.0f;
+        double ilo = 0.0, ihi = 0.0;
+        if (lane < R.cnt && w.x[j] != kMissing) {
+            const double dt = static_cast<double>(rec_dt[r * 64 + static_cast<uint64_t>(lane)]);
+            const DistInterval iv = dist_interval(w, j, m, Q2, S, R.c, dt, eta, static_cast<double>(R.eabs));
+            if (mode == kModeMain) {
+                emit = !(iv.lo > static_cast<double>(tmpl[t].bound));  // NaN (undecidable): a candidate
+                est = static_cast<float>(iv.est);
+                ilo = iv.lo;
+                ihi = iv.hi;
+            } else if (iv.hi < 3.0e38) {
+                emit = true;
+                est = static_cast<float>(iv.hi) * (1.0f + 1e-6f);
+                ilo = ihi = est;
+            }
+        }
+        const unsigned long long mask = __ballot(emit);
+        if (mask) {
+            const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+            unsigned long long slot0 = 0;
+            if (lane == leader)
+                slot0 = atomicAdd(co.count, static_cast<unsigned long long>(__popcll(mask)));
+            slot0 = __shfl(slot0, leader);
+            if (emit)
+                cand_put(co, slot0 + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull))), R.t,
+                         static_cast<uint32_t>(j), est, ilo, ihi);
+        }
+    }
+}
+
+__global__ void k_zero_nrec(unsigned long long *p) { *p = 0ull; }
 
 }  // namespace
 
@@ -461,10 +508,16 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
 int fft_fused_table_len() { return 2 * kM; }
 int fft_fused_block() { return kB; }
 
+size_t fft_chunk_rec_bytes() { return sizeof(ChunkRec); }
+
+// rec / rec_dt: queue of rec_cap records (rec_dt: 64 floats each); *nrec gets
+// the number the pass produced (more than rec_cap: the pass must be rerun
+// with a larger queue in main mode; in seed mode the entries kept suffice).
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
-                            const float2 *table, uint64_t P, int stride, double kfft, int mode, const CandOut &co,
-                            int grid, hipStream_t stream)
+                            const float2 *table, uint64_t P, int stride, double kfft, int mode, void *rec,
+                            float *rec_dt, unsigned long long *nrec, uint64_t rec_cap, const CandOut &co, int grid,
+                            hipStream_t stream)
 {
     if (F <= 0 || P == 0)
         return hipSuccess;
@@ -472,8 +525,11 @@ hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int 
         return hipErrorInvalidValue;
     const uint64_t nwork = (P + stride - 1) / stride;
     const unsigned g = static_cast<unsigned>(nwork < static_cast<uint64_t>(grid) ? nwork : grid);
+    hipLaunchKernelGGL(k_zero_nrec, dim3(1), dim3(1), 0, stream, nrec);
     hipLaunchKernelGGL(k_fft_fused, dim3(g), dim3(kT), 0, stream, w, tmpl, t0, F, spectra, bins, qspec, qmax, pinfo,
-                       pnorm, table, P, stride, kfft, mode, co);
+                       pnorm, table, P, stride, kfft, mode, static_cast<ChunkRec *>(rec), rec_dt, nrec, rec_cap);
+    hipLaunchKernelGGL(k_chunk_windows, dim3(static_cast<unsigned>(4 * grid)), dim3(256), 0, stream, w, tmpl,
+                       static_cast<const ChunkRec *>(rec), rec_dt, nrec, rec_cap, mode, co);
     return hipGetLastError();
 }
 

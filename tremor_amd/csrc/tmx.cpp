@@ -57,10 +57,12 @@ hipError_t launch_piece_input(const float *x, uint64_t n, int step, int B, uint6
 // fft_fused.hip
 int fft_fused_table_len();
 int fft_fused_block();
+size_t fft_chunk_rec_bytes();
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
-                            const float2 *table, uint64_t P, int stride, double kfft, int mode, const CandOut &co,
-                            int grid, hipStream_t stream);
+                            const float2 *table, uint64_t P, int stride, double kfft, int mode, void *rec,
+                            float *rec_dt, unsigned long long *nrec, uint64_t rec_cap, const CandOut &co, int grid,
+                            hipStream_t stream);
 hipError_t launch_piece_norm(const double *norm2, float2 *pnorm, uint64_t P, hipStream_t stream);
 hipError_t launch_spec_max(const float2 *spec, int rows, int bins, float *out, int stride_out, hipStream_t stream);
 hipError_t launch_tmpl_input(const TmplDev *tmpl, const int32_t *ids, int count, int B, const float *qz, float *out,
@@ -117,10 +119,10 @@ struct CallPlan {
 // Cost model (per template, seconds per window), in the role of the
 // reference's adaptive choice between scan and FFT (Tremor.cpp:2265-2286,
 // fft_fallback.hpp:55-60): the matrix-core form costs in proportion to the
-// padded template length plus a fixed epilogue, the FFT form in proportion
-// to B log2 B per piece of (B - Lmax + 1) windows.  Constants measured on
-// MI355X (matrix-core loop and epilogue of the split-BF16 / FP16 passes,
-// batched hipFFT c2r with the product and filter kernels); FFT iff it is
+// padded template length plus a fixed epilogue; the fused FFT form (B = 2^15,
+// templates up to 8192 samples) a constant per window; the FFT form with
+// hipFFT transforms (longer templates) in proportion to B log2 B per piece of
+// (B - Lmax + 1) windows.  Constants measured on MI355X; FFT iff it is
 // estimated below 0.9 x the inner-product form, as in the reference.
 double gemm_seconds_per_window(int precision, int kpad)
 {
@@ -139,6 +141,8 @@ int fft_block_for(int L)
 double fft_seconds_per_window(int L)
 {
     const int B = fft_block_for(L);
+    if (B == (1 << 15))
+        return 4.5e-12;
     double lg = 0.0;
     while ((1 << static_cast<int>(lg)) < B)
         lg += 1.0;
@@ -486,10 +490,24 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
     Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
     const double kfft = fft_error_factor(f.B);
     if (f.fused) {
+        // Seed passes queue at most one record per wave (16 per piece and
+        // template) and must keep them all: a template without seed entries
+        // would keep an infinite bound.
+        const uint64_t nwork = (f.P + static_cast<uint64_t>(stride) - 1) / static_cast<uint64_t>(stride);
+        const uint64_t seed_need = mode == kModeSeed ? nwork * static_cast<uint64_t>(F) * 16 : 0;
+        const uint64_t want = std::max<uint64_t>(std::max<uint64_t>(L.chunk_need, seed_need), 1u << 16);
+        if (want > L.chunk_cap) {
+            L.chunk_cap = 0;
+            LANE_HIP(L.chunk_rec.reserve(fft_chunk_rec_bytes() * want), "hipMalloc chunk queue");
+            LANE_HIP(L.chunk_dt.reserve(sizeof(float) * 64 * want), "hipMalloc chunk queue");
+            L.chunk_cap = want;
+        }
+        LANE_HIP(L.chunk_n.reserve(sizeof(unsigned long long)), "hipMalloc chunk queue");
         LANE_HIP(launch_fft_fused(w, L.tmpl.as<TmplDev>(), plan.Tg, F, f.spectra.as<float2>(), f.bins,
                                   L.qspec.as<float2>(), L.qmax.as<float>(), f.pinfo.as<float4>(), f.pnorm.as<float2>(),
-                                  f.table.as<float2>(), f.P, stride, kfft, mode, cand_out(L), L.cus > 0 ? L.cus : 256,
-                                  L.stream),
+                                  f.table.as<float2>(), f.P, stride, kfft, mode, L.chunk_rec.p,
+                                  L.chunk_dt.as<float>(), L.chunk_n.as<unsigned long long>(), L.chunk_cap,
+                                  cand_out(L), L.cus > 0 ? L.cus : 256, L.stream),
                  "fused FFT pass");
         return 0;
     }
@@ -563,9 +581,15 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
             return rc;
     }
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
-    unsigned long long c = 0;
+    unsigned long long c = 0, nchunk = 0;
     LANE_HIP(hipMemcpyAsync(&c, L.cand_count.p, sizeof(c), hipMemcpyDeviceToHost, L.stream), "read counter");
+    const bool fused = f && f->fused && plan.T > plan.Tg;
+    if (fused)
+        LANE_HIP(hipMemcpyAsync(&nchunk, L.chunk_n.p, sizeof(nchunk), hipMemcpyDeviceToHost, L.stream),
+                 "read chunk counter");
     LANE_HIP(hipStreamSynchronize(L.stream), "candidate pass");
+    // A main pass whose chunk queue overflowed lost candidates: rerun.
+    L.chunk_need = (fused && mode == kModeMain && nchunk > L.chunk_cap) ? nchunk + (nchunk >> 3) + 1024 : 0;
     LANE_HIP(hipEventElapsedTime(&L.last_pass_ms, L.ev0, L.ev1), "event time");
     *count = c;
     return 0;
@@ -579,6 +603,10 @@ int run_main(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
         int rc = run_pass(ctx, L, cfg, plan, f, kModeMain, 1, count);
         if (rc)
             return rc;
+        if (L.chunk_need > 0) {
+            (*retries)++;
+            continue;  // the next pass reserves the larger chunk queue
+        }
         if (*count <= L.cand_cap)
             return 0;
         if (*count > (1ull << 31) - 1)
