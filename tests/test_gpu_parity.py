@@ -300,3 +300,21 @@ def test_fused_fft_many_templates_matches_mfma(oracle):
     po, do, _ = oracle.scan_knn(x, tm[:2], 100, 0)
     assert np.array_equal(pos[:2], po)
     assert np.array_equal(d[:2].view(np.uint32), do.view(np.uint32))
+
+
+@pytest.mark.parametrize("rho", [0.9, 0.7, 0.5])
+def test_threshold_at_scale_shape(oracle, rho):
+    """The threshold workload's shape (repeating-source clusters, lengths
+    log-uniform 128..4096, FP16 candidate forms with the cost model's FFT
+    choice) at a size the oracle scans quickly: rows identical."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    w = synth.make("c2", n=1 << 21, templates=12)
+    with Context(precision="fp16", fft="auto") as ctx:
+        ctx.load_waveform(w.x)
+        _log("c2-shape", rho=rho)
+        r = ctx.threshold(w.templates, rho, 0)
+        print("  stats", r.stats, flush=True)
+    tt, tp, td = oracle.scan_threshold(w.x, w.templates, rho, 0)
+    assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+    assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
