@@ -10,7 +10,14 @@
 // sum ((v0+v4)+(v2+v6))+((v1+v5)+(v3+v7)) of acc0+acc1, then the tail in
 // index order, fused.  The reference's GCC build contracts exactly these
 // products (pinned by the oracle tests against the reference header).
-// One thread per candidate: the FP64 sums are sequential by definition.
+// Short templates: one thread per candidate.  Long templates (kWaveMin
+// samples and more): one wavefront per candidate.  The 16 FMA chains of the
+// distance then run on 16 lanes in the reference's order (bit-identical), and
+// the statistics come from a 64-lane FP64 sum with a rigorous bound on its
+// distance from the reference's sequential sum: when the whole interval of
+// possible reference values rounds to one float, that float is the
+// reference's; otherwise (values within ~1e-12 relative of a float rounding
+// boundary, about 1 window in 10^5) one lane runs the sequential code.
 #include "tmx_internal.hpp"
 
 namespace tmx {
@@ -87,6 +94,157 @@ __device__ inline float exact_dist(const float *q, const float *r, int m, float 
     return dist;
 }
 
+constexpr int kWaveMin = 512;
+
+__device__ inline double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off);
+    return v;
+}
+
+// gamma_k of the FP64 error analysis (k u / (1 - k u), u = 2^-53), rounded up.
+__device__ inline double gamma64(double k)
+{
+    constexpr double u = 1.1102230246251565e-16;
+    return k * u / (1.0 - k * u) * (1.0 + 1e-10);
+}
+
+// The reference statistics (exact_stats) of one window by a wavefront; w[0]
+// is not missing.  Returns false when the fast path cannot decide a value.
+__device__ bool wave_stats(const float *w, int m, float &mean_f, float &sd_f)
+{
+    constexpr double u = 1.1102230246251565e-16;
+    const int lane = threadIdx.x & 63;
+    const double md = static_cast<double>(m);
+    double s = 0.0, a = 0.0;
+    for (int d = lane; d < m; d += 64) {
+        const float v = w[d];
+        const double dv = v == kMissing ? 0.0 : static_cast<double>(v);
+        s += dv;
+        a += fabs(dv);
+    }
+    // Every lane takes lane 0's sums, so the decisions below are uniform.
+    s = __shfl(wave_sum_f64(s), 0);
+    a = __shfl(wave_sum_f64(a), 0);
+    // Parallel sum: ceil(m / 64) sequential terms then 6 tree levels.
+    const double gp = gamma64(static_cast<double>((m + 63) / 64 + 6));
+    const double A = a * (1.0 + 2.0 * gp);                      // >= sum |v|
+    const double Es = (gamma64(static_cast<double>(m - 1)) + gp) * A;  // |sum_seq - s|
+    // mean_ref = fl(sum_seq / m) lies in [ml, mh].
+    double ml = (s - Es) / md, mh = (s + Es) / md;
+    ml -= fabs(ml) * 4.0 * u;
+    mh += fabs(mh) * 4.0 * u;
+    mean_f = static_cast<float>(ml);
+    if (static_cast<float>(mh) != mean_f)
+        return false;
+    // Squares about the midpoint, then the reference's sum about mean_ref.
+    const double mu = 0.5 * (ml + mh);
+    const double W = 0.5 * (mh - ml) * (1.0 + 4.0 * u);
+    double q = 0.0;
+    for (int d = lane; d < m; d += 64) {
+        const float v = w[d];
+        const double dv = (v == kMissing ? 0.0 : static_cast<double>(v)) - mu;
+        q = __builtin_fma(dv, dv, q);
+    }
+    q = __shfl(wave_sum_f64(q), 0);
+    const double e2 = 2.01 * u + gp;
+    const double Qlo0 = q / (1.0 + e2), Qhi0 = q / (1.0 - e2);       // Q(mu)
+    const double T = (fabs(s - md * mu) + Es) * (1.0 + 8.0 * u) + 4.0 * u * fabs(md * mu);  // >= |S - m mu|
+    const double Qlo = Qlo0 - 2.0 * W * T, Qhi = Qhi0 + md * W * W + 2.0 * W * T;  // Q(mean_ref)
+    const double e1 = 2.01 * u + gamma64(md);
+    const double sqlo = Qlo * (1.0 - e1), sqhi = Qhi * (1.0 + e1);  // the reference's squares
+    if (!(sqlo >= 0.0))
+        return false;
+    double sdl = sqrt(sqlo / md), sdh = sqrt(sqhi / md);
+    sdl *= 1.0 - 4.0 * u;
+    sdh *= 1.0 + 4.0 * u;
+    if (sdh <= 1e-12) {
+        sd_f = 0.0f;
+        return true;
+    }
+    if (!(sdl > 1e-12))
+        return false;
+    sd_f = static_cast<float>(sdl);
+    return static_cast<float>(sdh) == sd_f;
+}
+
+// The reference distance of one window by lanes 0..15 of a wavefront (the
+// 16 accumulators), combined and finished with the tail on lane 0; the
+// result is valid on lane 0.
+__device__ float wave_dist(const float *q, const float *r, int m, float mean, float sd, const int32_t *order)
+{
+    const int lane = threadIdx.x & 63;
+    if (sd <= 1e-12f)
+        return FLT_MAX;
+    const float inv = 1.0f / sd;
+    float acc = 0.0f;
+    const int blocks = m / kBlock;
+    if (lane < 16) {
+        for (int b = 0; b < blocks; b++) {
+            const int off = order[b];
+#pragma unroll
+            for (int h = 0; h < kBlock; h += 16) {
+                float rv = r[off + h + lane];
+                rv = rv == kMissing ? 0.0f : rv;
+                const float dd = __builtin_fmaf(-(rv - mean), inv, q[off + h + lane]);
+                acc = __builtin_fmaf(dd, dd, acc);
+            }
+        }
+    }
+    // v[l] = a0[l] + a1[l]: accumulators l and l + 8.
+    float v = acc + __shfl(acc, (lane + 8) & 63);
+    float vl[8];
+#pragma unroll
+    for (int l = 0; l < 8; l++)
+        vl[l] = __shfl(v, l);
+    float dist = 0.0f;
+    if (lane == 0) {
+        const float s0 = vl[0] + vl[4], s1 = vl[1] + vl[5], s2 = vl[2] + vl[6], s3 = vl[3] + vl[7];
+        dist = (s0 + s2) + (s1 + s3);
+        for (int i = blocks * kBlock; i < m; i++) {
+            float rv = r[i];
+            rv = rv == kMissing ? 0.0f : rv;
+            const float diff = __builtin_fmaf(-(rv - mean), inv, q[i]);
+            dist = __builtin_fmaf(diff, diff, dist);
+        }
+    }
+    return dist;
+}
+
+// Long-template candidates, one wavefront each (others skip).
+__global__ __launch_bounds__(kThreads) void k_verify_wave(const float *__restrict__ x, uint64_t n,
+                                                          const TmplDev *__restrict__ tmpl,
+                                                          const float *__restrict__ qz,
+                                                          const int32_t *__restrict__ order,
+                                                          const Cand *__restrict__ cand, uint64_t count,
+                                                          float *__restrict__ d_out)
+{
+    const uint64_t i = (static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (i >= count)
+        return;
+    const Cand c = cand[i];
+    const int m = tmpl[c.t].m;
+    if (m < kWaveMin)
+        return;
+    float out = FLT_MAX;
+    if (static_cast<uint64_t>(c.j) + static_cast<uint64_t>(m) <= n && x[c.j] != kMissing) {
+        const float *w = x + c.j;
+        float mean, sd;
+        if (!wave_stats(w, m, mean, sd)) {
+            if (lane == 0)
+                exact_stats(w, m, mean, sd);
+            mean = __shfl(mean, 0);
+            sd = __shfl(sd, 0);
+        }
+        out = wave_dist(qz + tmpl[c.t].q_off, w, m, mean, sd, order + tmpl[c.t].order_off);
+    }
+    if (lane == 0)
+        d_out[i] = out;
+}
+
 __global__ __launch_bounds__(kThreads) void k_verify(const float *__restrict__ x, uint64_t n,
                                                      const TmplDev *__restrict__ tmpl, const float *__restrict__ qz,
                                                      const int32_t *__restrict__ order, const Cand *__restrict__ cand,
@@ -97,6 +255,8 @@ __global__ __launch_bounds__(kThreads) void k_verify(const float *__restrict__ x
         return;
     const Cand c = cand[i];
     const int m = tmpl[c.t].m;
+    if (m >= kWaveMin)
+        return;  // k_verify_wave
     if (static_cast<uint64_t>(c.j) + static_cast<uint64_t>(m) > n) {
         d_out[i] = FLT_MAX;
         return;
@@ -155,6 +315,13 @@ hipError_t launch_verify(const float *x, uint64_t n, const TmplDev *tmpl, const 
     const uint64_t grid = (count + kThreads - 1) / kThreads;
     hipLaunchKernelGGL(k_verify, dim3(static_cast<unsigned>(grid)), dim3(kThreads), 0, stream, x, n, tmpl, qz, order,
                        cand, count, d_out);
+    const uint64_t wgrid = (count + kThreads / 64 - 1) / (kThreads / 64);
+    for (uint64_t g0 = 0; g0 < wgrid; g0 += (1u << 31) - 1) {
+        const uint64_t g = wgrid - g0 < (1u << 31) - 1 ? wgrid - g0 : (1u << 31) - 1;
+        const uint64_t first = g0 * (kThreads / 64);
+        hipLaunchKernelGGL(k_verify_wave, dim3(static_cast<unsigned>(g)), dim3(kThreads), 0, stream, x, n, tmpl, qz,
+                           order, cand + first, count - first, d_out + first);
+    }
     return hipGetLastError();
 }
 
