@@ -49,6 +49,10 @@ def _args():
     ap.add_argument("--single-duration", type=float, default=3.0,
                     help="c3: seconds of the same stream replayed one request per launch")
     ap.add_argument("--lanes", type=int, default=2)
+    ap.add_argument("--sweep", type=float, nargs="*", default=[80.0, 60.0, 40.0, 25.0],
+                    help="c3: lower arrival rates to replay for the p99 <= 50 ms search")
+    ap.add_argument("--sweep-duration", type=float, default=4.0)
+    ap.add_argument("--sweep-batch", type=int, default=16)
     ap.add_argument("--max-batch", type=int, default=64)
     return ap.parse_args()
 
@@ -268,6 +272,17 @@ def bench_serving(a):
           f"{single['requests_per_s']:.1f} req/s", file=sys.stderr, flush=True)
     same = all(np.array_equal(x["pos"], y["pos"]) and np.array_equal(x["d"].view(np.uint32), y["d"].view(np.uint32))
                for x, y in zip(res_b, res_s))
+    # Lower arrival rates (same generator, shorter streams): the highest rate
+    # whose p99 stays within 50 ms.
+    sweep = []
+    for rate in a.sweep:
+        st = synth.request_stream(w, rate, a.sweep_duration, seed=synth.BASE_SEED + 34 + int(rate))
+        r, _ = _serve(ctx, st, a.lanes, a.sweep_batch, f"rate {rate:g}")
+        r["rate"] = rate
+        sweep.append(r)
+        print(f"[bench] rate {rate:g}/s: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms, "
+              f"{r['requests_per_s']:.1f} req/s, mean batch {r['mean_batch']:.1f}", file=sys.stderr, flush=True)
+    ok = [r["rate"] for r in sweep if r["p99_ms"] <= 50.0]
     out = {"metric": METRIC, "value": comps / batched["span_s"], "unit": "comparisons/s", "n_gpus": 1,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": a.prec + " MFMA / fp32 FFT + exact verify", "data": "synthetic",
@@ -276,6 +291,7 @@ def bench_serving(a):
                       "n": int(w.x.size), "rate": a.rate, "duration_s": a.duration},
            "p50_ms": batched["p50_ms"], "p99_ms": batched["p99_ms"], "batched": batched,
            "one_per_launch": single, "same_answers_batched_vs_single": bool(same),
+           "rate_sweep": sweep, "max_rate_p99_within_50ms": max(ok) if ok else None,
            "stats_build_s": load_s, "derived_build_s": warm_s}
     print(json.dumps(out), flush=True)
 
