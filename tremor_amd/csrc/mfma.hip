@@ -209,20 +209,23 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
 
     // FP16: power-of-two scale so the staged block fits FP16.  The waveform
     // buffer carries zero padding past n, so tile reads need no guard.
+    // Largest |x~| over the block's samples: bounds every window's |mu - c|
+    // for the tile test (and sets the FP16 scale).
+    float mx = 0.0f;
+    for (int e = tid; e < kNW + kmax; e += kThreads) {
+        float v = w.x[J0 + e];
+        v = v == kMissing ? 0.0f : v;
+        mx = fmaxf(mx, fabsf(v - c));
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+    if (lane == 0)
+        s_red[wave] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+    const float aD_max = mx * 1.0001f;  // >= |x0 - c| over the block, hence >= |mu_j - c|
     float scale = 1.0f, inv_scale = 1.0f;
     if (PREC == TMX_PREC_FP16) {
-        float mx = 0.0f;
-        for (int e = tid; e < kNW + kmax; e += kThreads) {
-            float v = w.x[J0 + e];
-            v = v == kMissing ? 0.0f : v;
-            mx = fmaxf(mx, fabsf(v - c));
-        }
-        for (int off = 32; off > 0; off >>= 1)
-            mx = fmaxf(mx, __shfl_xor(mx, off));
-        if (lane == 0)
-            s_red[wave] = mx;
-        __syncthreads();
-        mx = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
         // max |x~| * 2^-e <= 2^14
         int ex = 0;
         if (mx > 0.0f) {
@@ -389,9 +392,57 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
             float best_g = -INFINITY;
             uint32_t best_j = 0xffffffffu;
             float best_dt = 0.0f;
+            // Tile test constants: survive iff dt >= K sqrt(V) - rest, V the
+            // window's m sigma^2 (the per-window threshold below, rewritten).
+            const float Kc = gmin * rsqrt_m * 0.999f - (eta_sq + eabs_q) * 1.001f;
+            const float rest = ((eta_sq + eabs_q) * sqrt_m * aD_max + aD_max * absS + eabs_x) * 1.001f + 1e-30f;
+            const float amu_max = fabsf(c) + aD_max;
 #pragma unroll
             for (int jn = 0; jn < kNN; jn++) {
                 const uint64_t tile0 = J0 + 32ull * static_cast<uint64_t>((wave * kNN + jn) * 32);
+                if (mode == kModeMain) {
+                    // Tile test.  Column col holds windows j0 + s, s < 32 (this
+                    // lane and lane ^ 32 hold 16 each); their common core
+                    // [j0 + cnt - 1, j0 + m) bounds every window's m sigma^2
+                    // from below, |x~| bounds |mu - c|, and the column's
+                    // largest dot product bounds every dt.  A tile none of
+                    // whose columns can reach the threshold is skipped.
+                    float cm = -INFINITY;
+                    bool nonfinite = false;
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const float v = acc[i][jn][r];
+                        nonfinite |= !(v - v == 0.0f);
+                        cm = fmaxf(cm, v);
+                    }
+                    cm = fmaxf(cm, __shfl_xor(cm, 32)) * scale;
+                    nonfinite = __shfl_xor(static_cast<int>(nonfinite), 32) != 0 || nonfinite;
+                    const uint64_t j0 = tile0 + 32ull * static_cast<uint64_t>(col);
+                    bool surv = false;
+                    if (j0 < W) {
+                        surv = true;
+                        const int cnt = W - j0 < 32 ? static_cast<int>(W - j0) : 32;
+                        const int Lc = m - (cnt - 1);
+                        if (!nonfinite && Lc >= 2 && gmin > 0.0f && Kc > 0.0f) {
+                            const float2 ra = rbb[static_cast<int>(j0 - J0) + cnt - 1];
+                            const float2 rz = rbb[static_cast<int>(j0 - J0) + m];
+                            const float S1 = rz.x - ra.x, S2 = rz.y - ra.y;
+                            const float eS1 = 2.0f * bi.y + 1.2e-7f * (fabsf(ra.x) + fabsf(rz.x) + fabsf(S1));
+                            const float eS2 = 2.0f * bi.z + 1.2e-7f * (fabsf(ra.y) + fabsf(rz.y) + fabsf(S2));
+                            const float invL = 1.0f / static_cast<float>(Lc);
+                            const float V = S2 - S1 * S1 * invL;
+                            const float eV = eS2 + (2.0f * fabsf(S1) * eS1 + eS1 * eS1) * invL +
+                                             2.4e-7f * (fabsf(S2) + S1 * S1 * invL);
+                            const float Vlo = V - eV;
+                            if (Vlo > 1e3f * eV && Vlo > mf * 1e-20f && !(amu_max * amu_max * mf > 1e6f * Vlo)) {
+                                const float thr = Kc * (Vlo * __frsqrt_rn(Vlo)) * 0.999f - rest;
+                                surv = !(cm < thr);
+                            }
+                        }
+                    }
+                    if (__ballot(surv) == 0ull)
+                        continue;
+                }
                 // Per window of the tile, coalesced, from the block-rebased
                 // prefix pairs: main mode, the threshold the dot product must
                 // reach to survive level 1 (-inf: cannot be judged - stats
