@@ -90,6 +90,9 @@ struct Lane {
     uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
     hipfftHandle tfwd = 0, inv = 0;
     int tfwd_B = 0, inv_B = 0, inv_batch = 0;
+    // pinned host staging of the threshold hits (grown on demand)
+    void *host_keys = nullptr;
+    size_t host_keys_bytes = 0;
 
     Lane() = default;
     Lane(const Lane &) = delete;
@@ -115,6 +118,10 @@ struct Lane {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
         }
+        if (host_keys)
+            (void)hipHostFree(host_keys);
+        host_keys = nullptr;
+        host_keys_bytes = 0;
         if (tfwd)
             (void)hipfftDestroy(tfwd);
         if (inv)
@@ -133,6 +140,22 @@ struct Lane {
     {
         err = msg;
         return code;
+    }
+    hipError_t reserve_host_keys(size_t want)
+    {
+        if (want <= host_keys_bytes)
+            return hipSuccess;
+        if (host_keys)
+            (void)hipHostFree(host_keys);
+        host_keys = nullptr;
+        host_keys_bytes = 0;
+        const size_t b = want < (1u << 20) ? (1u << 20) : want + want / 4;
+        const hipError_t e = hipHostMalloc(&host_keys, b, hipHostMallocDefault);
+        if (e == hipSuccess)
+            host_keys_bytes = b;
+        else
+            host_keys = nullptr;
+        return e;
     }
     int hip_fail(hipError_t e, const char *what)
     {

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_filter(
             if (survive && w.x[j] != kMissing) {
                 const DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(dt), eta, eabs);
                 if (!(iv.lo > bound))  // NaN (undecidable): a candidate
-                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), static_cast<uint32_t>(j),
+                    cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j),
                              static_cast<float>(iv.est), iv.lo, iv.hi);
             }
         } else if (judge && V > 0.0f) {

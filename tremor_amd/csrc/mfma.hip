@@ -551,7 +551,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                     const uint64_t j = tile0 + static_cast<uint64_t>(wl);
                     if (j >= W || w.x[j] == kMissing)
                         continue;
-                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN,
+                    cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN,
                              -INFINITY, INFINITY);
                 }
             }
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                     iv.lo -= 2.0 * ex / (sig * (1.0 - 1e-3)) * 1.01;
                 }
                 if (iv.lo <= static_cast<double>(tmpl[t].bound))
-                    cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(t), static_cast<uint32_t>(j),
+                    cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j),
                              static_cast<float>(iv.est), iv.lo, iv.hi);
             }
         } else {

@@ -28,7 +28,7 @@ constexpr int kXsLen = (kWB + kKC) + ((kWB + kKC) >> 2);
 
 __device__ inline void emit(const CandOut &co, uint32_t t, uint32_t j, float est, double lo, double hi)
 {
-    cand_put(co, atomicAdd(co.count, 1ULL), t, j, est, lo, hi);
+    cand_put(co, wave_claim(co.count), t, j, est, lo, hi);
 }
 
 __global__ __launch_bounds__(kThreads) void k_scan_fp32(Waveform w, const TmplDev *__restrict__ tmpl,

@@ -52,7 +52,7 @@ __global__ void k_group_count(const Cand *__restrict__ cand, const float *__rest
         return;
     const Cand c = cand[i];
     if (pred_ok(pred, tmpl, c.t, val[i]))
-        atomicAdd(&counts[c.t], 1u);
+        (void)wave_add_by_key(counts, c.t);
 }
 
 // Exclusive scan of T counts into offsets[0..T], one workgroup; also resets
@@ -104,7 +104,7 @@ __global__ void k_group_scatter(const Cand *__restrict__ cand, const float *__re
     const float v = val[i];
     if (!pred_ok(pred, tmpl, c.t, v))
         return;
-    const uint32_t slot = offsets[c.t] + atomicAdd(&cursor[c.t], 1u);
+    const uint32_t slot = offsets[c.t] + wave_add_by_key(cursor, c.t);
     keys[slot] = make_key(keykind, tmpl, c, v);
 }
 

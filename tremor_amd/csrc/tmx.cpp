@@ -477,15 +477,10 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
     const int F = plan.T - plan.Tg;
     if (F <= 0)
         return 0;
-    std::vector<FftPair> pairs;
-    for (uint64_t p = 0; p < f.P; p += static_cast<uint64_t>(stride))
-        for (int i = 0; i < F; i++) {
-            const int t = plan.Tg + i;
-            const uint64_t m = static_cast<uint64_t>(plan.tmpl[t].m);
-            if (m <= ctx->n && p * f.step < ctx->n - m + 1)
-                pairs.push_back(FftPair{static_cast<uint32_t>(t), static_cast<uint32_t>(p)});
-        }
-    if (pairs.empty())
+    bool any = false;
+    for (int i = 0; i < F && !any; i++)
+        any = static_cast<uint64_t>(plan.tmpl[plan.Tg + i].m) <= ctx->n;
+    if (!any)
         return 0;
     Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
     const double kfft = fft_error_factor(f.B);
@@ -511,6 +506,16 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
                  "fused FFT pass");
         return 0;
     }
+    std::vector<FftPair> pairs;
+    for (uint64_t p = 0; p < f.P; p += static_cast<uint64_t>(stride))
+        for (int i = 0; i < F; i++) {
+            const int t = plan.Tg + i;
+            const uint64_t m = static_cast<uint64_t>(plan.tmpl[t].m);
+            if (m <= ctx->n && p * f.step < ctx->n - m + 1)
+                pairs.push_back(FftPair{static_cast<uint32_t>(t), static_cast<uint32_t>(p)});
+        }
+    if (pairs.empty())
+        return 0;
     const int NB = static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, (512ll << 20) / (12ll * f.B))));
     if (!L.inv || L.inv_batch != NB || L.inv_B != f.B) {
         if (L.inv)
@@ -758,17 +763,19 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan
                  "read results");
     }
     std::vector<uint32_t> offs;
-    std::vector<uint64_t> keys;
+    const uint64_t *keys = nullptr;
     if (plan.has_threshold) {
         offs.resize(T + 1);
         LANE_HIP(hipMemcpyAsync(offs.data(), L.offsets.p, sizeof(uint32_t) * (T + 1), hipMemcpyDeviceToHost, L.stream),
                  "read hit offsets");
         LANE_HIP(hipStreamSynchronize(L.stream), "hit offsets");
-        keys.resize(offs[T]);
-        if (offs[T])
-            LANE_HIP(hipMemcpyAsync(keys.data(), L.keys.p, sizeof(uint64_t) * offs[T], hipMemcpyDeviceToHost,
+        if (offs[T]) {
+            LANE_HIP(L.reserve_host_keys(sizeof(uint64_t) * offs[T]), "hipHostMalloc hit staging");
+            LANE_HIP(hipMemcpyAsync(L.host_keys, L.keys.p, sizeof(uint64_t) * offs[T], hipMemcpyDeviceToHost,
                                     L.stream),
                      "read hits");
+            keys = static_cast<const uint64_t *>(L.host_keys);
+        }
     }
     LANE_HIP(hipStreamSynchronize(L.stream), "results");
     out.assign(T, QueryOut{});
@@ -785,6 +792,8 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan
         // the last kept one.
         bool has_last = false;
         uint64_t last = 0;
+        o.pos.reserve(offs[d + 1] - offs[d]);
+        o.d.reserve(offs[d + 1] - offs[d]);
         for (uint32_t e = offs[d]; e < offs[d + 1]; e++) {
             const uint64_t pos = keys[e] >> 32;
             const uint32_t bits = static_cast<uint32_t>(keys[e]);

@@ -75,6 +75,43 @@ __device__ inline float round_up_f(double v)
     return f;
 }
 
+// One slot of *counter for each calling lane, with one atomic per wavefront
+// (the lanes active at the call are the claimants).
+__device__ inline unsigned long long wave_claim(unsigned long long *counter)
+{
+    const unsigned long long mask = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned long long base = 0ull;
+    if (lane == leader)
+        base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, leader);
+    return base + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+// Adds 1 to counts[key] for each calling lane, with one atomic per distinct
+// key in the wavefront; returns the value counts[key] had for this lane (the
+// lanes of one key get consecutive values, in lane order).
+__device__ inline uint32_t wave_add_by_key(uint32_t *counts, uint32_t key)
+{
+    unsigned long long pending = __ballot(1);
+    const int lane = __lane_id();
+    uint32_t mine = 0u;
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const uint32_t kl = __shfl(key, leader);
+        const unsigned long long same = __ballot(key == kl) & pending;
+        uint32_t base = 0u;
+        if (lane == leader)
+            base = atomicAdd(&counts[kl], static_cast<uint32_t>(__popcll(same)));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull)
+            mine = base + static_cast<uint32_t>(__popcll(same & ((1ull << lane) - 1ull)));
+        pending &= ~same;
+    }
+    return mine;
+}
+
 // Stores one entry in a slot already claimed from o.count.
 __device__ inline void cand_put(const CandOut &o, unsigned long long slot, uint32_t t, uint32_t j, float est,
                                 double lo, double hi)

@@ -182,6 +182,8 @@ __device__ float wave_dist(const float *q, const float *r, int m, float mean, fl
     float acc = 0.0f;
     const int blocks = m / kBlock;
     if (lane < 16) {
+        // Unrolled so that several blocks' loads are in flight at once.
+#pragma unroll 8
         for (int b = 0; b < blocks; b++) {
             const int off = order[b];
 #pragma unroll
