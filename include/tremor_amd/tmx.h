@@ -30,6 +30,8 @@
  *   tmx_distance_batch   l2_dist_znorm_subsequence           distance_computers/ZnormSearch.hpp:103-165
  *   tmx_window_stats     two-pass window statistics          algos/sss/SkipSequential.cpp:157-179
  *   tmx_prepare_template znormalize_inplace + query_block_order ZnormSearch.hpp:33-90
+ *   tmx_set_fft_mode,    Tremor::setRefinementMode          algos/htremor/Tremor.hpp:37-44,161
+ *   tmx_set_scan_mode    (scan / FFT / adaptive refinement)
  */
 #ifndef TREMOR_AMD_TMX_H
 #define TREMOR_AMD_TMX_H
@@ -57,6 +59,15 @@ enum tmx_fft_mode {
     TMX_FFT_ALWAYS = 2   /* every template on the FFT form */
 };
 
+/* Which templates take the early-abandoning scan (the reference's sequential
+ * scan with its per-block abandon test), decided once the kNN bounds are
+ * known. */
+enum tmx_scan_mode {
+    TMX_SCAN_AUTO = 0,   /* cost model on a sampled abandon test, as the reference's adaptive refinement */
+    TMX_SCAN_OFF = 1,    /* never */
+    TMX_SCAN_ALWAYS = 2  /* every template */
+};
+
 typedef struct {
     int32_t device;          /* HIP device ordinal */
     int32_t precision;       /* enum tmx_precision */
@@ -64,6 +75,7 @@ typedef struct {
     int32_t seed_target;     /* kNN bound seeding: target seed entries per template; 0 = default */
     int32_t verbose;         /* 1 = print per-phase timings to stderr */
     int32_t fft_mode;        /* enum tmx_fft_mode; ignored for TMX_PREC_FP32 (plain scan) */
+    int32_t scan_mode;       /* enum tmx_scan_mode */
 } tmx_cfg;
 
 typedef struct {
@@ -82,6 +94,7 @@ typedef struct {
     int32_t fft_block;       /* FFT length B of those templates (0: none) */
     int64_t verified;        /* candidates left for the exact recomputation after the kNN bound was
                                 tightened from the candidates' own upper bounds */
+    int32_t scan_templates;  /* templates that took the early-abandoning scan */
 } tmx_stats;
 
 typedef struct {
@@ -98,6 +111,7 @@ void tmx_destroy(tmx_ctx *ctx);
 const char *tmx_last_error(const tmx_ctx *ctx);
 int tmx_set_precision(tmx_ctx *ctx, int32_t precision);
 int tmx_set_fft_mode(tmx_ctx *ctx, int32_t mode);
+int tmx_set_scan_mode(tmx_ctx *ctx, int32_t mode);
 
 /* Copies the waveform to the device and builds its FP64 prefix sums of x and
  * x^2 (missing samples as 0).  Replaces any waveform loaded before. */

@@ -18,9 +18,11 @@ PREC_BF16X3 = 1
 PREC_FP16 = 2
 PRECISIONS = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp16": PREC_FP16}
 FFT_MODES = {"auto": 0, "off": 1, "always": 2}
+SCAN_MODES = {"auto": 0, "off": 1, "always": 2}
 
 EXPORTS = [
     "tmx_cfg_default", "tmx_create", "tmx_destroy", "tmx_last_error", "tmx_set_precision", "tmx_set_fft_mode",
+    "tmx_set_scan_mode",
     "tmx_load_waveform",
     "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch", "tmx_window_stats",
     "tmx_prepare_template", "tmx_sync",
@@ -34,7 +36,8 @@ REQ_THRESHOLD = 1
 
 class Cfg(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32), ("cand_capacity", ctypes.c_int64),
-                ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32), ("fft_mode", ctypes.c_int32)]
+                ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32), ("fft_mode", ctypes.c_int32),
+                ("scan_mode", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -43,7 +46,8 @@ class Stats(ctypes.Structure):
                 ("seconds_select", ctypes.c_double), ("comparisons", ctypes.c_int64),
                 ("candidates", ctypes.c_int64), ("retries", ctypes.c_int32), ("max_abs_rho_err", ctypes.c_float),
                 ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double),
-                ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64)]
+                ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64),
+                ("scan_templates", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -91,6 +95,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.tmx_last_error.restype = ctypes.c_char_p
     L.tmx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.tmx_set_fft_mode.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.tmx_set_scan_mode.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.tmx_load_waveform.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint64]
     L.tmx_waveform_length.argtypes = [ctypes.c_void_p]
     L.tmx_waveform_length.restype = ctypes.c_uint64
@@ -156,7 +161,7 @@ class Context:
     """One device context: a resident waveform and search calls on it."""
 
     def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
-                 seed_target: int = 0, verbose: bool = False, fft: str = "auto"):
+                 seed_target: int = 0, verbose: bool = False, fft: str = "auto", scan: str = "auto"):
         L = lib()
         cfg = Cfg()
         L.tmx_cfg_default(ctypes.byref(cfg))
@@ -166,6 +171,7 @@ class Context:
         cfg.seed_target = seed_target
         cfg.verbose = 1 if verbose else 0
         cfg.fft_mode = FFT_MODES[fft]
+        cfg.scan_mode = SCAN_MODES[scan]
         h = ctypes.c_void_p()
         rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:
@@ -200,6 +206,10 @@ class Context:
 
     def set_fft_mode(self, mode: str):
         self._check(self._L.tmx_set_fft_mode(self._h, FFT_MODES[mode]))
+
+    def set_scan_mode(self, mode: str):
+        """Early-abandoning scan: "auto" (cost model), "off" or "always"."""
+        self._check(self._L.tmx_set_scan_mode(self._h, SCAN_MODES[mode]))
 
     def load_waveform(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -86,6 +86,8 @@ struct Lane {
     DevMem Y, y, pairs, qspec, qmax, qin, ids, slot;
     // fused FFT form: queue of chunks for the per-window pass
     DevMem chunk_rec, chunk_dt, chunk_n;
+    // early-abandoning scan: its templates and the sample counters
+    DevMem scan_ids, scan_visit;
     uint64_t chunk_cap = 0;
     uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
     hipfftHandle tfwd = 0, inv = 0;

@@ -1,0 +1,212 @@
+// Early-abandoning scan as a candidate pass: the reference's skip-sequential
+// scan without lower bounds (SkipSequential.cpp:223-294 threshold, :296-393
+// kNN, per window the kernel of ZnormSearch.hpp:103-165 with its abandon
+// test after every 32-sample block, blocks in the template's energy order of
+// ZnormSearch.hpp:72-90).  One thread per window, so consecutive lanes read
+// consecutive samples and the template's values are wavefront-uniform
+// (scalar loads).
+//
+// The window statistics come from the FP64 prefix sums (as in filter.hpp)
+// instead of the reference's two-pass sums, so the running sum differs from
+// the reference's by a bounded amount.  A window is abandoned only once its
+// running sum proves that the reference's distance exceeds the bound; a
+// window that runs to the end becomes a candidate with an interval [lo, hi]
+// of its reference distance when lo <= bound, and the exact recomputation
+// decides it.  The candidate set therefore contains every answer, and the
+// answers are those of the reference.
+//
+// Error bound (u = 2^-24).  With the exact window statistics mu, sigma and
+// w_i = (x_i - mu) / sigma (so sum over the window of w_i^2 = m):
+//   this pass:  dz_i = fma(-(x_i - mean_f), inv_f, q_i) with mean_f, inv_f
+//               from the prefix sums; |dz_i - (q_i - w_i)| <= |w_i| (e_sig + u)
+//               (1 + u) + |mean_f - mu| inv_f (1 + u) + u |dz_i|, e_sig the
+//               relative error of inv_f; block sums of 32 fused products
+//               added in sequence: relative error gamma_{nb + 34};
+//   reference:  the same terms with its float statistics (relative error of
+//               1 / std_f <= 2u + 1e-10, |mean_f - mu| <= u |mu| + 1e-11
+//               (|mu| + sigma)); 16 FMA chains, lane sum and tail: relative
+//               error gamma_{m/16 + 40}.
+// Over the blocks done, the 2-norm of the difference of the two term vectors
+// is at most T = (e_sig + 5u + 1e-9) sqrt(m) + sqrt(m) (dmu + dmu_ref) /
+// sigma_lo (1.001), hence with s this pass's running sum
+//   d_ref >= (1 - gamma_ref) ((sqrt(s) (1 - gamma_acc - 1.01u) - T) / (1 + u))^2,
+// and the window is abandoned once s exceeds
+//   thr = ((sqrt(bound / (1 - gamma_ref)) (1 + u) + T) / (1 - gamma_acc - 1.01u))^2.
+// The reference's partial sums only grow (fused products of squares onto
+// non-negative accumulators), so its final distance is at least this bound
+// whichever block the test fires at.
+#include "tmx_internal.hpp"
+
+namespace tmx {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kModeSample = 2;  // count the blocks each sampled window visits
+
+__device__ inline double gamma32(double k)
+{
+    constexpr double u = 5.9604644775390625e-08;
+    return k * u / (1.0 - k * u);
+}
+
+__device__ inline double sq(double v) { return v * v; }
+
+// ids[0..count): the templates of this pass (blockIdx.y).  Main mode: the
+// windows [256 blockIdx.x, + 256) of each; sample mode: sample_n windows
+// spread evenly over the template's windows, visited[t] += blocks visited,
+// visited[T + t] += windows sampled.
+__global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev *__restrict__ tmpl, int T,
+                                                      const float *__restrict__ qz, const int32_t *__restrict__ order,
+                                                      const int32_t *__restrict__ ids, int mode, uint64_t sample_n,
+                                                      CandOut co, unsigned long long *__restrict__ visited)
+{
+    constexpr double u = 5.9604644775390625e-08;
+    constexpr double u64 = 1.1102230246251565e-16;
+    const int t = ids[blockIdx.y];
+    const int m = tmpl[t].m;
+    if (static_cast<uint64_t>(m) > w.n)
+        return;
+    const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    uint64_t j;
+    if (mode == kModeSample) {
+        if (i >= sample_n)
+            return;
+        j = (i * W) / sample_n;
+    } else {
+        if (i >= W)
+            return;
+        j = i;
+    }
+    if (w.x[j] == kMissing)
+        return;  // never searched (SkipSequential.cpp:256)
+    const int nb = m / kBlock;
+    const double md = static_cast<double>(m);
+    // Statistics from the prefix sums with their error bounds (filter.hpp).
+    const double2 pa = w.p[j], pb = w.p[j + m];
+    const double S1 = pb.x - pa.x, S2 = pb.y - pa.y;
+    const double eS1 = kPrefixGamma * (sqrt(static_cast<double>(j + m) * fabs(pb.y)) +
+                                       sqrt(static_cast<double>(j) * fabs(pa.y))) +
+                       4.0 * u64 * fabs(S1);
+    const double eS2 = kPrefixGamma * (fabs(pb.y) + fabs(pa.y)) + 4.0 * u64 * fabs(S2);
+    const double V = S2 - S1 * S1 / md;
+    const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) / md + 8.0 * u64 * (fabs(S2) + S1 * S1 / md);
+    const double vlo = V - eV;
+    if (!(vlo > md * 4e-24) || !(eV < 0.25 * V)) {
+        // Undecidable here (near-constant window or swamped prefix
+        // differences): the exact recomputation decides.
+        if (mode == kModeSample)
+            atomicAdd(&visited[t], static_cast<unsigned long long>(nb + 1));
+        else
+            cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN, -INFINITY,
+                     INFINITY);
+        if (mode == kModeSample)
+            atomicAdd(&visited[T + t], 1ull);
+        return;
+    }
+    const double sig = sqrt(V / md);
+    const double sig_lo = sqrt(vlo / md) * (1.0 - 4.0 * u64);
+    const double sig_hi = sqrt((V + eV) / md) * (1.0 + 4.0 * u64);
+    const double mu = S1 / md;
+    const double emu = eS1 / md + 2.0 * u64 * fabs(mu);
+    const float mean_f = static_cast<float>(mu);
+    const float inv_f = static_cast<float>(1.0 / sig);
+    const double e_sig = fmax(sig_hi / sig - 1.0, 1.0 - sig_lo / sig) + 2.0 * u;
+    const double dmu = u * fabs(mu) + emu;
+    const double dmu_ref = u * (fabs(mu) + emu) + 1e-11 * (fabs(mu) + emu + sig_hi);
+    const double rm = sqrt(md);
+    const double Tm = (e_sig + 5.0 * u + 1e-9) * rm + rm * (dmu + dmu_ref) * 1.001 / sig_lo;
+    const double g_acc = gamma32(static_cast<double>(nb + m % kBlock + 34));
+    const double g_ref = gamma32(floor(md / 16.0) + 40.0);
+    const double bound = static_cast<double>(tmpl[t].bound);
+    const double shrink = 1.0 - g_acc - 1.01 * u;
+    float thr = INFINITY;
+    if (bound < 3.0e38) {
+        const double th = sq((sqrt(fmax(bound, 0.0) / (1.0 - g_ref)) * (1.0 + u) + Tm) / shrink) * (1.0 + 1e-5);
+        thr = th < 3.0e38 ? static_cast<float>(th) * (1.0f + 1e-6f) : INFINITY;
+    }
+
+    const float *q = qz + tmpl[t].q_off;
+    const int32_t *ord = order + tmpl[t].order_off;
+    const float *r = w.x + j;
+    float s = 0.0f;
+    int b = 0;
+    bool abandoned = false;
+    for (; b < nb; b++) {
+        const int off = ord[b];
+        float bs = 0.0f;
+#pragma unroll
+        for (int h = 0; h < kBlock; h++) {
+            float rv = r[off + h];
+            rv = rv == kMissing ? 0.0f : rv;
+            const float dz = __builtin_fmaf(-(rv - mean_f), inv_f, q[off + h]);
+            bs = __builtin_fmaf(dz, dz, bs);
+        }
+        s += bs;
+        if (s > thr) {
+            abandoned = true;
+            b++;
+            break;
+        }
+    }
+    if (mode == kModeSample) {
+        atomicAdd(&visited[t], static_cast<unsigned long long>(abandoned ? b : nb + 1));
+        atomicAdd(&visited[T + t], 1ull);
+        return;
+    }
+    if (abandoned)
+        return;
+    float ts = 0.0f;
+    for (int k = nb * kBlock; k < m; k++) {
+        float rv = r[k];
+        rv = rv == kMissing ? 0.0f : rv;
+        const float dz = __builtin_fmaf(-(rv - mean_f), inv_f, q[k]);
+        ts = __builtin_fmaf(dz, dz, ts);
+    }
+    s += ts;
+    const double rs = sqrt(static_cast<double>(s));
+    const double lo = (1.0 - g_ref) * sq(fmax(0.0, rs * shrink - Tm) / (1.0 + u));
+    const double hi = (1.0 + g_ref) * sq((rs * (1.0 + g_acc + 1.01 * u) + Tm) / (1.0 - u));
+    if (!(lo > bound))
+        cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), s, lo, hi);
+}
+
+__global__ void k_mark_ea(TmplDev *__restrict__ tmpl, const int32_t *__restrict__ ids, int count)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < count)
+        tmpl[ids[i]].ea = 1;
+}
+
+}  // namespace
+
+// Main pass (mode kModeMain) over the templates ids[0..count) (all windows),
+// or a sample pass (sample_n windows each) accumulating the blocks visited.
+hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const float *qz, const int32_t *order,
+                          const int32_t *ids, int count, uint64_t max_windows, int sample, uint64_t sample_n,
+                          const CandOut &co, unsigned long long *visited, hipStream_t stream)
+{
+    if (count <= 0)
+        return hipSuccess;
+    const uint64_t win = sample ? sample_n : max_windows;
+    const uint64_t gx = (win + kThreads - 1) / kThreads;
+    if (gx == 0)
+        return hipSuccess;
+    if (gx > 0x7fffffffull || count > 65535)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_ea, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(count)), dim3(kThreads), 0,
+                       stream, w, tmpl, T, qz, order, ids, sample ? kModeSample : static_cast<int>(kModeMain),
+                       sample_n, co, visited);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_ea(TmplDev *tmpl, const int32_t *ids, int count, hipStream_t stream)
+{
+    if (count <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_mark_ea, dim3((count + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, tmpl, ids,
+                       count);
+    return hipGetLastError();
+}
+
+}  // namespace tmx

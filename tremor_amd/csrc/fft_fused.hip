@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
         for (int ti = 0; ti < F; ti++) {
             const int t = t0 + ti;
             const int m = tmpl[t].m;
-            if (static_cast<uint64_t>(m) > w.n)
+            if (static_cast<uint64_t>(m) > w.n || tmpl[t].ea)
                 continue;
             const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
             if (base >= W)

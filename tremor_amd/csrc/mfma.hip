@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
 #pragma unroll
         for (int i = 0; i < kNM; i++) {
             const int ti = tp + (i < nt ? i : 0);
-            ks[i] = i < nt ? tmpl[ti].kpad / 16 : 0;
+            ks[i] = i < nt && !tmpl[ti].ea ? tmpl[ti].kpad / 16 : 0;
             abase[i] = afrag + tmpl[ti].a_off + lane;
         }
         const int kt = (ks[0] > ks[kNM - 1] ? ks[0] : ks[kNM - 1]) * 16;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                 continue;
             const int t = tp + i;
             const int m = tmpl[t].m;
-            if (static_cast<uint64_t>(m) > w.n)
+            if (static_cast<uint64_t>(m) > w.n || tmpl[t].ea)
                 continue;
             const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
             const double Q2 = tmpl[t].Q2;
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                 const int m = tmpl[t].m;
                 float ub = INFINITY;
                 const uint32_t bj0 = qj[i * 64 + lane];
-                if (static_cast<uint64_t>(m) <= w.n && bj0 != 0xffffffffu && w.x[bj0] != kMissing) {
+                if (static_cast<uint64_t>(m) <= w.n && !tmpl[t].ea && bj0 != 0xffffffffu && w.x[bj0] != kMissing) {
                     const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
                     const double md = static_cast<double>(m);
                     const double abs_unit = PREC == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_fp32(Waveform w, const TmplDe
     const int tb = group_begin[g], te = group_begin[g + 1];
     for (int t = tb; t < te; t++) {
         const int m = tmpl[t].m;
-        if (static_cast<uint64_t>(m) > w.n)
+        if (static_cast<uint64_t>(m) > w.n || tmpl[t].ea)
             continue;
         const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
         if (j0 >= W)

@@ -77,6 +77,10 @@ hipError_t launch_zero_f64(double *p, uint64_t count, hipStream_t stream);
 double fft_error_factor(int B);
 // misc.hip
 hipError_t launch_zero_u64(unsigned long long *p, hipStream_t stream);
+hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const float *qz, const int32_t *order,
+                          const int32_t *ids, int count, uint64_t max_windows, int sample, uint64_t sample_n,
+                          const CandOut &co, unsigned long long *visited, hipStream_t stream);
+hipError_t launch_mark_ea(TmplDev *tmpl, const int32_t *ids, int count, hipStream_t stream);
 hipError_t launch_rho_err(const Cand *cand, const float *est, const float *exact, uint64_t count,
                           const TmplDev *tmpl, unsigned int *out_bits, hipStream_t stream);
 }  // namespace tmx
@@ -114,6 +118,8 @@ struct CallPlan {
     int max_k = 0;                  // largest k
     int64_t out_slots = 0;          // sum of k over the kNN templates
     bool has_threshold = false;
+    std::vector<int32_t> scan_ids;  // templates on the early-abandoning scan (set after seeding)
+    uint64_t scan_windows = 0;      // largest window count among them
 };
 
 // Cost model (per template, seconds per window), in the role of the
@@ -148,6 +154,27 @@ double fft_seconds_per_window(int L)
         lg += 1.0;
     const double per_piece = 0.45e-12 * B * lg + 1.6e-12 * B;
     return per_piece / static_cast<double>(B - L + 1);
+}
+
+// The early-abandoning scan (a third form, chosen once the bounds are known,
+// as the reference decides after its sampled early-abandon test,
+// Tremor.cpp:2265-2286): a fixed cost per window (statistics) plus one per
+// 32-sample block visited, the blocks measured on a sample of windows.
+constexpr double kScanWindow = 3.0e-12;
+constexpr double kScanBlock = 5.6e-12;
+constexpr uint64_t kScanSample = 4096;
+
+double scan_seconds_per_window(double blocks) { return kScanWindow + kScanBlock * blocks; }
+
+// The planned form's estimate for template d of the plan.
+double planned_seconds_per_window(const tmx_cfg &cfg, const CallPlan &plan, int d)
+{
+    const TmplDev &t = plan.tmpl[d];
+    if (d >= plan.Tg)
+        return fft_seconds_per_window(plan.fft_L);
+    if (cfg.precision == TMX_PREC_FP32)
+        return 8.0e-15 * t.m + 2.0e-12;
+    return gemm_seconds_per_window(cfg.precision, t.kpad);
 }
 
 int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count, CallPlan &plan,
@@ -511,7 +538,7 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
         for (int i = 0; i < F; i++) {
             const int t = plan.Tg + i;
             const uint64_t m = static_cast<uint64_t>(plan.tmpl[t].m);
-            if (m <= ctx->n && p * f.step < ctx->n - m + 1)
+            if (m <= ctx->n && !plan.tmpl[t].ea && p * f.step < ctx->n - m + 1)
                 pairs.push_back(FftPair{static_cast<uint32_t>(t), static_cast<uint32_t>(p)});
         }
     if (pairs.empty())
@@ -567,8 +594,11 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
     LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
     const int groups = static_cast<int>(plan.group_begin.size()) - 1;
+    bool inner = false, fft_any = false;
+    for (int d = 0; d < plan.T; d++)
+        (d < plan.Tg ? inner : fft_any) |= !plan.tmpl[d].ea;
     LANE_HIP(hipEventRecord(L.ev0, L.stream), "event");
-    if (groups > 0) {
+    if (groups > 0 && inner) {
         if (cfg.precision == TMX_PREC_FP32) {
             LANE_HIP(launch_scan_fp32(w, L.tmpl.as<TmplDev>(), L.qz.as<float>(), plan.T, L.groups.as<int32_t>(),
                                       groups, mode, block_stride, cand_out(L), L.stream),
@@ -580,15 +610,20 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
                      "MFMA scan");
         }
     }
-    if (f) {
+    if (f && fft_any) {
         const int rc = run_fft_pass(ctx, L, plan, *f, mode, fft_stride);
         if (rc)
             return rc;
     }
+    if (mode == kModeMain && !plan.scan_ids.empty())
+        LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
+                                L.scan_ids.as<int32_t>(), static_cast<int>(plan.scan_ids.size()), plan.scan_windows,
+                                0, 0, cand_out(L), nullptr, L.stream),
+                 "early-abandoning scan");
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
     unsigned long long c = 0, nchunk = 0;
     LANE_HIP(hipMemcpyAsync(&c, L.cand_count.p, sizeof(c), hipMemcpyDeviceToHost, L.stream), "read counter");
-    const bool fused = f && f->fused && plan.T > plan.Tg;
+    const bool fused = f && f->fused && fft_any;
     if (fused)
         LANE_HIP(hipMemcpyAsync(&nchunk, L.chunk_n.p, sizeof(nchunk), hipMemcpyDeviceToHost, L.stream),
                  "read chunk counter");
@@ -665,8 +700,71 @@ uint64_t waveform_blocks(tmx_ctx *ctx, const tmx_cfg &cfg)
     return (ctx->n + wb - 1) / wb;
 }
 
+// Which templates take the early-abandoning scan: every one (TMX_SCAN_ALWAYS)
+// or, with the bounds known, those whose sampled scan estimate is not beaten
+// by 10% by the planned form (the reference's rule, Tremor.cpp:2286: the FFT
+// only when fft < 0.9 scan).  Marks them on the device and in the plan.
+int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
+{
+    plan.scan_ids.clear();
+    plan.scan_windows = 0;
+    if (cfg.scan_mode == TMX_SCAN_OFF)
+        return 0;
+    std::vector<int32_t> ids;
+    for (int d = 0; d < plan.T; d++) {
+        if (static_cast<uint64_t>(plan.tmpl[d].m) > ctx->n)
+            continue;
+        // The scan visits at least one block per window.
+        if (cfg.scan_mode == TMX_SCAN_ALWAYS ||
+            planned_seconds_per_window(cfg, plan, d) >= 0.9 * scan_seconds_per_window(1.0))
+            ids.push_back(d);
+    }
+    if (ids.empty())
+        return 0;
+    LANE_HIP(L.scan_ids.reserve(sizeof(int32_t) * ids.size()), "hipMalloc scan ids");
+    LANE_HIP(hipMemcpyAsync(L.scan_ids.p, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice, L.stream),
+             "copy scan ids");
+    std::vector<int32_t> chosen;
+    if (cfg.scan_mode == TMX_SCAN_ALWAYS) {
+        chosen = ids;
+    } else {
+        const size_t vb = sizeof(unsigned long long) * 2 * static_cast<size_t>(plan.T);
+        LANE_HIP(L.scan_visit.reserve(vb), "hipMalloc scan sample");
+        LANE_HIP(launch_zero_f64(L.scan_visit.as<double>(), 2 * static_cast<uint64_t>(plan.T), L.stream),
+                 "clear scan sample");
+        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
+        LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
+                                L.scan_ids.as<int32_t>(), static_cast<int>(ids.size()), 0, 1, kScanSample,
+                                cand_out(L), L.scan_visit.as<unsigned long long>(), L.stream),
+                 "scan sample");
+        std::vector<unsigned long long> visit(2 * static_cast<size_t>(plan.T));
+        LANE_HIP(hipMemcpyAsync(visit.data(), L.scan_visit.p, vb, hipMemcpyDeviceToHost, L.stream), "read scan sample");
+        LANE_HIP(hipStreamSynchronize(L.stream), "scan sample");
+        for (int d : ids) {
+            const double nw = static_cast<double>(std::max<unsigned long long>(1, visit[plan.T + d]));
+            const double blocks = static_cast<double>(visit[d]) / nw;
+            if (!(planned_seconds_per_window(cfg, plan, d) < 0.9 * scan_seconds_per_window(blocks)))
+                chosen.push_back(d);
+        }
+        if (chosen.empty())
+            return 0;
+        LANE_HIP(hipMemcpyAsync(L.scan_ids.p, chosen.data(), sizeof(int32_t) * chosen.size(), hipMemcpyHostToDevice,
+                                L.stream),
+                 "copy scan ids");
+    }
+    LANE_HIP(launch_mark_ea(L.tmpl.as<TmplDev>(), L.scan_ids.as<int32_t>(), static_cast<int>(chosen.size()),
+                            L.stream),
+             "mark scan templates");
+    for (int d : chosen) {
+        plan.tmpl[d].ea = 1;
+        plan.scan_windows = std::max<uint64_t>(plan.scan_windows, ctx->n - static_cast<uint64_t>(plan.tmpl[d].m) + 1);
+    }
+    plan.scan_ids = chosen;
+    return 0;
+}
+
 // The batch under the shared lock (resident structures ready).
-int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan,
+int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
                  const std::vector<float> &raw_packed, std::vector<QueryOut> &out, tmx_stats &st)
 {
     const int T = plan.T;
@@ -701,6 +799,9 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan
                  "seed bound");
         LANE_HIP(hipStreamSynchronize(L.stream), "seed");
     }
+    if ((rc = choose_scan(ctx, L, cfg, plan)))
+        return rc;
+    st.scan_templates = static_cast<int32_t>(plan.scan_ids.size());
     const double t2 = now_s();
     st.seconds_seed = t2 - t1;
 
@@ -709,7 +810,9 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan
         return rc;
     st.kernel_ms_main = L.last_pass_ms;
     st.flops_main = plan.flops;
-    st.fft_templates = plan.T - plan.Tg;
+    st.fft_templates = 0;
+    for (int d = plan.Tg; d < plan.T; d++)
+        st.fft_templates += plan.tmpl[d].ea ? 0 : 1;
     st.fft_block = f ? f->B : 0;
     const double t3 = now_s();
     st.seconds_main = t3 - t2;
@@ -901,6 +1004,7 @@ int tmx_cfg_default(tmx_cfg *cfg)
     cfg->seed_target = 0;
     cfg->verbose = 0;
     cfg->fft_mode = TMX_FFT_AUTO;
+    cfg->scan_mode = TMX_SCAN_AUTO;
     return 0;
 }
 
@@ -953,6 +1057,17 @@ int tmx_set_fft_mode(tmx_ctx *ctx, int32_t mode)
         return ctx->fail("unknown FFT mode");
     std::lock_guard<std::mutex> lk(ctx->call_mu);
     ctx->cfg.fft_mode = mode;
+    return 0;
+}
+
+int tmx_set_scan_mode(tmx_ctx *ctx, int32_t mode)
+{
+    if (!ctx)
+        return -1;
+    if (mode < TMX_SCAN_AUTO || mode > TMX_SCAN_ALWAYS)
+        return ctx->fail("unknown scan mode");
+    std::lock_guard<std::mutex> lk(ctx->call_mu);
+    ctx->cfg.scan_mode = mode;
     return 0;
 }
 

@@ -31,7 +31,7 @@ struct TmplDev {
     float bound;        // candidate cut: threshold distance, or the kNN seed bound
     int32_t k;          // kNN answers wanted (0: threshold template)
     int32_t merge;      // exclusion zone of the answers (merge offset)
-    int32_t pad0;
+    int32_t ea;         // 1: the early-abandoning scan takes this template (the other forms skip it)
     int64_t out_off;    // first kNN output slot of this template
 };
 
