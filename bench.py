@@ -54,6 +54,7 @@ def _args():
     ap.add_argument("--sweep-duration", type=float, default=4.0)
     ap.add_argument("--sweep-batch", type=int, default=16)
     ap.add_argument("--max-batch", type=int, default=64)
+    ap.add_argument("--long-subset", type=int, default=8, help="c4: templates of the GEMM and scan legs")
     return ap.parse_args()
 
 
@@ -156,39 +157,56 @@ def bench_threshold(a):
 
 def bench_long(a):
     """Long-template regime (c4 shape): n = 2^28, 64 templates m ~ logU[8192,
-    65536], kNN k = 10; the FFT form, the split-BF16 GEMM form and the cost
-    model's choice, all with exact verification."""
+    65536], kNN k = 10; the three forms of the candidate pass - FFT, split-BF16
+    GEMM, early-abandoning scan - each alone, then the cost model's choice,
+    all with exact verification and the same answers.  The GEMM and scan
+    forms, an order of magnitude slower here, run one step on the first
+    --long-subset templates."""
     from tremor_amd import synth
     from tremor_amd.capi import Context
     n = a.n if a.n != 10**8 else 2**28
     w = synth.make("c4", n=n, templates=a.templates if a.templates != 256 else 64)
     k = 10
-    ctx = Context(precision=a.prec, fft="always")
+    ctx = Context(precision=a.prec, fft="always", scan="off")
     ctx.load_waveform(w.x)
+    sub = w.templates[:a.long_subset]
     out = {"config": {"workload": f"long templates: n={w.x.size}, {len(w.templates)} templates "
                                   f"m~logU[8192,65536], kNN k={k}"}}
     res = {}
-    for mode, steps in (("always", a.steps), ("off", 1), ("auto", a.steps)):
-        ctx.set_fft_mode(mode)
-        pos, d, st = ctx.knn(w.templates, k, 0)  # warmup (builds spectra / operands)
+    legs = (("fft", "always", "off", w.templates, a.steps), ("fft_subset", "always", "off", sub, 1),
+            ("gemm_subset", "off", "off", sub, 1), ("scan_subset", "always", "always", sub, 1),
+            ("auto", "auto", "auto", w.templates, a.steps))
+    for name, fft, scan, tms, steps in legs:
+        ctx.set_fft_mode(fft)
+        ctx.set_scan_mode(scan)
+        pos, d, st = ctx.knn(tms, k, 0)  # warmup (builds spectra / operands)
         t0 = time.perf_counter()
         for _ in range(steps):
-            pos, d, st = ctx.knn(w.templates, k, 0)
+            pos, d, st = ctx.knn(tms, k, 0)
         el = (time.perf_counter() - t0) / steps
-        res[mode] = {"ms_per_step": 1e3 * el, "comparisons_per_s": st["comparisons"] / el,
+        res[name] = {"templates": len(tms), "ms_per_step": 1e3 * el, "comparisons_per_s": st["comparisons"] / el,
                      "kernel_ms": st["kernel_ms_main"], "fft_templates": st["fft_templates"],
+                     "scan_templates": st["scan_templates"], "scan_blocks_per_window": st["scan_blocks"],
                      "fft_block": st["fft_block"], "candidates": st["candidates"],
                      "max_abs_rho_err": st["max_abs_rho_err"], "answers": (pos.copy(), d.copy())}
-        print(f"[bench] c4 {mode}: {1e3 * el:.1f} ms/step, main {st['kernel_ms_main']:.1f} ms, "
-              f"fft templates {st['fft_templates']}", file=sys.stderr, flush=True)
-    same = all(np.array_equal(res[m]["answers"][0], res["off"]["answers"][0]) and
-               np.array_equal(res[m]["answers"][1].view(np.uint32), res["off"]["answers"][1].view(np.uint32))
-               for m in res)
+        print(f"[bench] c4 {name}: {len(tms)} templates, {1e3 * el:.1f} ms/step, main {st['kernel_ms_main']:.1f} ms, "
+              f"fft templates {st['fft_templates']}, scan templates {st['scan_templates']} "
+              f"({st['scan_blocks']:.1f} blocks/window)", file=sys.stderr, flush=True)
+
+    def same(x, y, rows=None):
+        px, dx = x["answers"]
+        py, dy = y["answers"]
+        if rows is not None:
+            px, dx = px[:rows], dx[:rows]
+        return np.array_equal(px, py[:len(px)]) and np.array_equal(dx.view(np.uint32), dy[:len(dx)].view(np.uint32))
+
+    agree = (same(res["fft"], res["auto"]) and same(res["fft_subset"], res["gemm_subset"]) and
+             same(res["fft_subset"], res["scan_subset"]) and same(res["fft_subset"], res["fft"], len(sub)))
     for m in res:
         del res[m]["answers"]
     out.update({"metric": METRIC, "unit": "comparisons/s", "n_gpus": 1, "forms": res,
-                "same_answers_all_forms": bool(same), "steps": a.steps, "higher_is_better": True,
-                "value": res["auto"]["comparisons_per_s"], "dtype": a.prec + " / fp32 FFT + exact verify",
+                "same_answers_all_forms": bool(agree), "steps": a.steps, "higher_is_better": True,
+                "value": res["auto"]["comparisons_per_s"], "dtype": a.prec + " / fp32 FFT / fp32 scan + exact verify",
                 "data": "synthetic"})
     print(json.dumps(out), flush=True)
 
@@ -319,7 +337,7 @@ def main():
 
     # The headline measures the matrix-core form (every template on MFMA);
     # the cost model's own choice is measured after it.
-    ctx = Context(precision=a.prec, fft="off")
+    ctx = Context(precision=a.prec, fft="off", scan="off")
     t0 = time.time()
     ctx.load_waveform(w.x)
     load_s = time.time() - t0
@@ -347,6 +365,7 @@ def main():
     auto = None
     if a.prec != "fp32":
         ctx.set_fft_mode("auto")
+        ctx.set_scan_mode("auto")
         ctx.knn(w.templates, a.k, 0)
         t1 = time.perf_counter()
         ast = []
@@ -355,11 +374,13 @@ def main():
             ast.append(s)
         el = time.perf_counter() - t1
         auto = {"value": comps * a.steps / el, "ms_per_step": 1e3 * el / a.steps,
-                "fft_templates": ast[-1]["fft_templates"], "kernel_ms": float(np.mean([s["kernel_ms_main"] for s in ast])),
+                "fft_templates": ast[-1]["fft_templates"], "scan_templates": ast[-1]["scan_templates"],
+                "kernel_ms": float(np.mean([s["kernel_ms_main"] for s in ast])),
                 "same_answers": bool(np.array_equal(apos, pos) and np.array_equal(ad.view(np.uint32), d.view(np.uint32)))}
         ctx.set_fft_mode("off")
+        ctx.set_scan_mode("off")
         print(f"[bench] cost-model choice: {auto['ms_per_step']:.1f} ms/step, {auto['fft_templates']} templates on "
-              f"the FFT form", file=sys.stderr, flush=True)
+              f"the FFT form, {auto['scan_templates']} on the early-abandoning scan", file=sys.stderr, flush=True)
 
     scan = None
     if a.scan_steps > 0 and a.prec != "fp32":

@@ -95,6 +95,7 @@ typedef struct {
     int64_t verified;        /* candidates left for the exact recomputation after the kNN bound was
                                 tightened from the candidates' own upper bounds */
     int32_t scan_templates;  /* templates that took the early-abandoning scan */
+    double scan_blocks;      /* their mean 32-sample blocks visited per window (sampled) */
 } tmx_stats;
 
 typedef struct {

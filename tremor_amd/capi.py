@@ -47,7 +47,7 @@ class Stats(ctypes.Structure):
                 ("candidates", ctypes.c_int64), ("retries", ctypes.c_int32), ("max_abs_rho_err", ctypes.c_float),
                 ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double),
                 ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64),
-                ("scan_templates", ctypes.c_int32)]
+                ("scan_templates", ctypes.c_int32), ("scan_blocks", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

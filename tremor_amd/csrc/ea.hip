@@ -28,7 +28,8 @@
 //               error gamma_{m/16 + 40}.
 // Over the blocks done, the 2-norm of the difference of the two term vectors
 // is at most T = (e_sig + 5u + 1e-9) sqrt(m) + sqrt(m) (dmu + dmu_ref) /
-// sigma_lo (1.001), hence with s this pass's running sum
+// sigma_lo (1.001) (dmu, dmu_ref: the two float means' distances from mu),
+// hence with s this pass's running sum
 //   d_ref >= (1 - gamma_ref) ((sqrt(s) (1 - gamma_acc - 1.01u) - T) / (1 + u))^2,
 // and the window is abandoned once s exceeds
 //   thr = ((sqrt(bound / (1 - gamma_ref)) (1 + u) + T) / (1 - gamma_acc - 1.01u))^2.
@@ -82,40 +83,46 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
         return;  // never searched (SkipSequential.cpp:256)
     const int nb = m / kBlock;
     const double md = static_cast<double>(m);
-    // Statistics from the prefix sums with their error bounds (filter.hpp).
+    const double invm = 1.0 / md;
+    // Statistics from the prefix sums with their error bounds (filter.hpp's,
+    // with sqrt(a) + sqrt(b) <= sqrt(2 (a + b)) and products by the rounded
+    // 1 / m, whose extra roundings the 8 u64 terms cover).
     const double2 pa = w.p[j], pb = w.p[j + m];
     const double S1 = pb.x - pa.x, S2 = pb.y - pa.y;
-    const double eS1 = kPrefixGamma * (sqrt(static_cast<double>(j + m) * fabs(pb.y)) +
-                                       sqrt(static_cast<double>(j) * fabs(pa.y))) +
+    const double eS1 = kPrefixGamma * sqrt(2.0 * (static_cast<double>(j + m) * fabs(pb.y) +
+                                                  static_cast<double>(j) * fabs(pa.y))) +
                        4.0 * u64 * fabs(S1);
     const double eS2 = kPrefixGamma * (fabs(pb.y) + fabs(pa.y)) + 4.0 * u64 * fabs(S2);
-    const double V = S2 - S1 * S1 / md;
-    const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) / md + 8.0 * u64 * (fabs(S2) + S1 * S1 / md);
-    const double vlo = V - eV;
-    if (!(vlo > md * 4e-24) || !(eV < 0.25 * V)) {
+    const double S1sq = S1 * S1 * invm;
+    const double V = S2 - S1sq;
+    const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) * invm * (1.0 + 4.0 * u64) + 8.0 * u64 * (fabs(S2) + S1sq);
+    if (!(V - eV > md * 4e-24) || !(eV < 0.25 * V)) {
         // Undecidable here (near-constant window or swamped prefix
         // differences): the exact recomputation decides.
-        if (mode == kModeSample)
+        if (mode == kModeSample) {
             atomicAdd(&visited[t], static_cast<unsigned long long>(nb + 1));
-        else
+            atomicAdd(&visited[T + t], 1ull);
+        } else {
             cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN, -INFINITY,
                      INFINITY);
-        if (mode == kModeSample)
-            atomicAdd(&visited[T + t], 1ull);
+        }
         return;
     }
-    const double sig = sqrt(V / md);
-    const double sig_lo = sqrt(vlo / md) * (1.0 - 4.0 * u64);
-    const double sig_hi = sqrt((V + eV) / md) * (1.0 + 4.0 * u64);
-    const double mu = S1 / md;
-    const double emu = eS1 / md + 2.0 * u64 * fabs(mu);
+    // sigma = sqrt(V_exact / m) with |V_exact - V| <= eV, rV = eV / V <= 1/4:
+    // |sigma / sig - 1| <= rV (sqrt(1 + x) - 1 <= x / 2, 1 - sqrt(1 - x) <= x).
+    const double sig = sqrt(V * invm);
+    const double rV = eV / V + 4.0 * u64;
+    const double mu = S1 * invm;
+    const double emu = eS1 * invm * (1.0 + 4.0 * u64) + 3.0 * u64 * fabs(mu);
     const float mean_f = static_cast<float>(mu);
-    const float inv_f = static_cast<float>(1.0 / sig);
-    const double e_sig = fmax(sig_hi / sig - 1.0, 1.0 - sig_lo / sig) + 2.0 * u;
+    const float inv_f = 1.0f / static_cast<float>(sig);  // 1 / sigma within (rV + 3u)
+    const double e_sig = rV * (1.0 + 4.0 * u) + 4.0 * u;
     const double dmu = u * fabs(mu) + emu;
-    const double dmu_ref = u * (fabs(mu) + emu) + 1e-11 * (fabs(mu) + emu + sig_hi);
+    const double dmu_ref = u * (fabs(mu) + emu) + 1e-11 * (fabs(mu) + emu + sig * (1.0 + rV));
     const double rm = sqrt(md);
-    const double Tm = (e_sig + 5.0 * u + 1e-9) * rm + rm * (dmu + dmu_ref) * 1.001 / sig_lo;
+    // 1 / sig_lo <= inv_f (1 + 3u) (1 + 2 rV) for rV <= 1/2.
+    const double inv_lo = static_cast<double>(inv_f) * (1.0 + 3.0 * u) * (1.0 + 2.0 * rV);
+    const double Tm = (e_sig + 5.0 * u + 1e-9) * rm + rm * (dmu + dmu_ref) * 1.001 * inv_lo;
     const double g_acc = gamma32(static_cast<double>(nb + m % kBlock + 34));
     const double g_ref = gamma32(floor(md / 16.0) + 40.0);
     const double bound = static_cast<double>(tmpl[t].bound);

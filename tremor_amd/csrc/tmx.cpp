@@ -120,6 +120,7 @@ struct CallPlan {
     bool has_threshold = false;
     std::vector<int32_t> scan_ids;  // templates on the early-abandoning scan (set after seeding)
     uint64_t scan_windows = 0;      // largest window count among them
+    double scan_blocks = 0.0;       // their mean sampled blocks per window
 };
 
 // Cost model (per template, seconds per window), in the role of the
@@ -160,8 +161,8 @@ double fft_seconds_per_window(int L)
 // as the reference decides after its sampled early-abandon test,
 // Tremor.cpp:2265-2286): a fixed cost per window (statistics) plus one per
 // 32-sample block visited, the blocks measured on a sample of windows.
-constexpr double kScanWindow = 3.0e-12;
-constexpr double kScanBlock = 5.6e-12;
+constexpr double kScanWindow = 9.0e-12;
+constexpr double kScanBlock = 3.6e-12;
 constexpr uint64_t kScanSample = 4096;
 
 double scan_seconds_per_window(double blocks) { return kScanWindow + kScanBlock * blocks; }
@@ -708,6 +709,7 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
 {
     plan.scan_ids.clear();
     plan.scan_windows = 0;
+    plan.scan_blocks = 0.0;
     if (cfg.scan_mode == TMX_SCAN_OFF)
         return 0;
     std::vector<int32_t> ids;
@@ -725,9 +727,8 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
     LANE_HIP(hipMemcpyAsync(L.scan_ids.p, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice, L.stream),
              "copy scan ids");
     std::vector<int32_t> chosen;
-    if (cfg.scan_mode == TMX_SCAN_ALWAYS) {
-        chosen = ids;
-    } else {
+    double blocks_sum = 0.0;
+    {
         const size_t vb = sizeof(unsigned long long) * 2 * static_cast<size_t>(plan.T);
         LANE_HIP(L.scan_visit.reserve(vb), "hipMalloc scan sample");
         LANE_HIP(launch_zero_f64(L.scan_visit.as<double>(), 2 * static_cast<uint64_t>(plan.T), L.stream),
@@ -743,8 +744,11 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
         for (int d : ids) {
             const double nw = static_cast<double>(std::max<unsigned long long>(1, visit[plan.T + d]));
             const double blocks = static_cast<double>(visit[d]) / nw;
-            if (!(planned_seconds_per_window(cfg, plan, d) < 0.9 * scan_seconds_per_window(blocks)))
+            if (cfg.scan_mode == TMX_SCAN_ALWAYS ||
+                !(planned_seconds_per_window(cfg, plan, d) < 0.9 * scan_seconds_per_window(blocks))) {
                 chosen.push_back(d);
+                blocks_sum += blocks;
+            }
         }
         if (chosen.empty())
             return 0;
@@ -752,6 +756,7 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
                                 L.stream),
                  "copy scan ids");
     }
+    plan.scan_blocks = blocks_sum / static_cast<double>(chosen.size());
     LANE_HIP(launch_mark_ea(L.tmpl.as<TmplDev>(), L.scan_ids.as<int32_t>(), static_cast<int>(chosen.size()),
                             L.stream),
              "mark scan templates");
@@ -802,6 +807,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     if ((rc = choose_scan(ctx, L, cfg, plan)))
         return rc;
     st.scan_templates = static_cast<int32_t>(plan.scan_ids.size());
+    st.scan_blocks = plan.scan_blocks;
     const double t2 = now_s();
     st.seconds_seed = t2 - t1;
 
