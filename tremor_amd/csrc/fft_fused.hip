@@ -13,11 +13,11 @@
 //      Im z[r] (unnormalised, like a c2r of length B), computed in LDS by
 //      Stockham passes of radix 16, 16, 16, 4 (the first one in registers);
 //   3. the window dot products dt = y[m - 1 + i] / B are read from LDS by a
-//      chunk filter: for each 64 consecutive windows, a rigorous lower bound
-//      of their reference distances from the largest dt and the statistics of
-//      the windows' common core (two FP64 prefix reads per chunk); only
-//      chunks that may hold a window within the bound are examined window by
-//      window (filter.hpp's interval) and emit candidates.
+//      chunk filter: for the windows of each 64 consecutive outputs, a
+//      rigorous lower bound of their reference distances from the largest dt
+//      and the statistics of the windows' common core (two FP64 prefix reads
+//      per chunk); only chunks that may hold a window within the bound are
+//      examined window by window (filter.hpp's interval) and emit candidates.
 //
 // Nothing of size B leaves the chip: per (piece, template) HBM traffic is the
 // piece spectrum (amortised over the batch) and the candidates.
@@ -43,7 +43,7 @@ constexpr int kM = 16384;            // complex points of the half-length transf
 constexpr int kB = 2 * kM;           // real length B
 constexpr int kT = 1024;             // threads per workgroup
 constexpr int kStep = kB - kB / 4 + 1;        // windows per piece
-constexpr int kChunks = (kStep + 63) / 64;    // 64-window chunks per piece
+constexpr int kChunks = (kStep + 63) / 64 + 1;  // 64-output chunks holding a piece's windows
 static_assert(kChunks <= kT, "one thread per chunk");
 constexpr int kLds = kM + (kM >> 4) + 2;      // padded complex slots (index M included)
 
@@ -349,42 +349,71 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
             const uint64_t nwin64 = W - base < static_cast<uint64_t>(kStep) ? W - base : static_cast<uint64_t>(kStep);
             const int nwin = static_cast<int>(nwin64);
             const int nch = (nwin + 63) / 64;
-            // Thread ch owns the 64-window chunk ch; its core's prefix pairs
-            // are loaded ahead of the transform.
-            const bool has_ch = tid < nch;
-            const int cnt = has_ch ? (nwin - 64 * tid < 64 ? nwin - 64 * tid : 64) : 0;
-            const int Lc = m - (cnt - 1);
-            const uint64_t ca = base + static_cast<uint64_t>(64 * tid + cnt - 1);
-            const uint64_t cb = base + static_cast<uint64_t>(64 * tid + m);
+            // Thread tid owns the chunk of outputs [64 (c0 + tid), + 64): the
+            // windows i = n - (m - 1) of its outputs n, clipped to the piece's
+            // [0, nwin).  Its core's prefix pairs are loaded ahead of the
+            // transform.
+            // (The chunk's bounds are recomputed after the transform rather
+            // than held across it.)
+            const auto chunk = [&](int &n_first, int &n_end) {
+                const int nlo = 64 * (((m - 1) >> 6) + opaque(tid));
+                n_first = nlo > m - 1 ? nlo : m - 1;
+                n_end = nlo + 64 < m - 1 + nwin ? nlo + 64 : m - 1 + nwin;
+            };
             double2 pa = make_double2(0.0, 0.0), pb = make_double2(0.0, 0.0);
-            if (has_ch && Lc >= 2) {
-                pa = w.p[ca];
-                pb = w.p[cb];
+            {
+                int nf, ne;
+                chunk(nf, ne);
+                const int cnt = nf < ne ? ne - nf : 0;
+                const int i_first = nf - (m - 1);
+                if (cnt > 0 && m - (cnt - 1) >= 2) {
+                    pa = w.p[base + static_cast<uint64_t>(i_first + cnt - 1)];
+                    pb = w.p[base + static_cast<uint64_t>(i_first + m)];
+                }
             }
             const __amdgpu_buffer_rsrc_t Hr = rsrc(qspec + static_cast<uint64_t>(ti) * static_cast<uint64_t>(bins),
                                                    bins * 8u);
             correlate_piece(data, X, Xn, Hr, Er);
 
-            // 3. Filter.  dt of window base + i is y[n] / B, n = m - 1 + i, at
-            // float n + 2 (n >> 5) of the padded array.
+            int n_first, n_end;
+            chunk(n_first, n_end);
+            const int c0 = (m - 1) >> 6;
+            const bool has_ch = n_first < n_end;
+            const int cnt = has_ch ? n_end - n_first : 0;
+            const int i_first = n_first - (m - 1);
+            const int Lc = m - (cnt - 1);
+            const uint64_t ca = base + static_cast<uint64_t>(i_first + cnt - 1);
+            const uint64_t cb = base + static_cast<uint64_t>(i_first + m);
+            // 3. Filter.  Output n (dt of window base + n - (m - 1), times B)
+            // is at float n + 2 (n >> 5) of the padded array.  The chunk's
+            // largest output, and a sum that is non-finite iff an output is.
             const float invB = 1.0f / static_cast<float>(kB);
-            const int n0 = m - 1 + 64 * tid;
-            float dmax = -INFINITY;
-            int arg = 0;
-            bool nonfinite = false;
-            if (has_ch) {
+            float vmax = -INFINITY, vsum = 0.0f;
+            if (cnt == 64) {
+                // Two padded rows of 32 floats; half the lanes take them in
+                // the other order (fewer LDS bank conflicts).
+                const int f0 = 68 * (c0 + tid);
+                const int sw = (tid >> 4) & 1;
+#pragma unroll 1
+                for (int hr = 0; hr < 2; hr++) {
+                    const float2 *row = reinterpret_cast<const float2 *>(fdata + f0 + 34 * (hr ^ sw));
 #pragma unroll 8
-                for (int l = 0; l < cnt; l++) {
-                    const int n = n0 + l;
-                    const float v = fdata[n + 2 * (n >> 5)];
-                    nonfinite |= !(v == v);
-                    if (v > dmax) {
-                        dmax = v;
-                        arg = l;
+                    for (int q = 0; q < 16; q++) {
+                        const float2 v = row[q];
+                        vmax = fmaxf(vmax, fmaxf(v.x, v.y));
+                        vsum += v.x;
+                        vsum += v.y;
                     }
                 }
-                dmax *= invB;
+            } else {
+                for (int n = n_first; n < n_end; n++) {
+                    const float v = fdata[n + 2 * (n >> 5)];
+                    vmax = fmaxf(vmax, v);
+                    vsum += v;
+                }
             }
+            const bool nonfinite = !(vsum - vsum == 0.0f);
+            const float dmax = vmax * invB;
             // Error terms of this (template, piece): FFT error of every dot
             // product plus the rounding of x~ = x0 - c (eta).
             const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
@@ -404,11 +433,11 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
                     if (!(lo > bound)) {
                         const unsigned long long slot = atomicAdd(nrec, 1ULL);
                         if (slot < rec_cap) {
-                            rec[slot] = ChunkRec{static_cast<uint32_t>(t), static_cast<uint32_t>(base + 64 * tid), cnt,
+                            rec[slot] = ChunkRec{static_cast<uint32_t>(t), static_cast<uint32_t>(base + i_first), cnt,
                                                  c, round_up_f(eabs), 0u};
                             float *dst = rec_dt + slot * 64;
                             for (int l = 0; l < cnt; l++) {
-                                const int n = n0 + l;
+                                const int n = n_first + l;
                                 dst[l] = fdata[n + 2 * (n >> 5)] * invB;
                             }
                         }
@@ -436,10 +465,14 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
                     }
                 }
                 if (lane == who && g > -INFINITY) {
+                    int arg = 0;
+                    for (int n = n_end - 1; n >= n_first; n--)
+                        if (fdata[n + 2 * (n >> 5)] == vmax)
+                            arg = n - n_first;
                     const unsigned long long slot = atomicAdd(nrec, 1ULL);
                     if (slot < rec_cap) {
                         rec[slot] = ChunkRec{static_cast<uint32_t>(t),
-                                             static_cast<uint32_t>(base + 64 * tid + arg), 1, c, round_up_f(eabs), 0u};
+                                             static_cast<uint32_t>(base + i_first + arg), 1, c, round_up_f(eabs), 0u};
                         rec_dt[slot * 64] = dmax;
                     }
                 }
