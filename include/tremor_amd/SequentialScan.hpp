@@ -54,6 +54,9 @@ public:
     void threshold(const float *queries, int query_count, float correlation, std::vector<Hit> &hits,
                    tmx_stats *stats = nullptr);
 
+    // Candidate forms (answers unchanged): enum tmx_fft_mode / tmx_scan_mode.
+    void set_forms(int fft_mode, int scan_mode);
+
     // Device bytes held beyond the raw samples (prefix statistics).
     double summary_memory_bytes() const { return summary_bytes_; }
     int window() const { return window_; }

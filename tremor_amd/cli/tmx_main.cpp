@@ -36,6 +36,8 @@ struct Options {
     int merge = 0, replication_groups = 0, paa = 16, workers = 4, index_threads = 4;
     std::string method = "sequential";
     int precision = TMX_PREC_BF16X3;
+    int fft_mode = TMX_FFT_AUTO;
+    int scan_mode = TMX_SCAN_AUTO;
     int device = 0;
 };
 
@@ -54,6 +56,8 @@ void usage(const char *prog)
         "  --no-benchmark-report      skip the timings CSV\n"
         "  --method sequential        (skip-sequential is accepted: same answers)\n"
         "  --precision bf16x3|fp16|fp32  candidate pass on the GPU (answers identical)\n"
+        "  --fft-mode auto|off|always     FFT form of the candidate pass (default auto: cost model)\n"
+        "  --scan-mode auto|off|always    early-abandoning scan form (default auto: cost model)\n"
         "  --device N                 HIP device\n"
         "  --search-workers/--index-threads/--paa-segments/--replication-groups accepted for compatibility\n",
         prog);
@@ -147,6 +151,12 @@ Options parse(int argc, char **argv)
             o.workers = parse_i(value(i, a), a);
         } else if (a == "--index-threads") {
             o.index_threads = parse_i(value(i, a), a);
+        } else if (a == "--fft-mode" || a == "--scan-mode") {
+            const std::string v = value(i, a);
+            const int mode = v == "auto" ? 0 : v == "off" ? 1 : v == "always" ? 2 : -1;
+            if (mode < 0)
+                throw std::invalid_argument(a + " must be auto, off or always");
+            (a == "--fft-mode" ? o.fft_mode : o.scan_mode) = mode;
         } else if (a == "--precision") {
             const std::string p = value(i, a);
             if (p == "bf16x3")
@@ -259,6 +269,7 @@ int run(const Options &o)
 
     tremor_amd::SequentialScan scan(o.workers, o.paa, o.merge, o.replication_groups, o.method != "sequential",
                                     o.precision, o.device);
+    scan.set_forms(o.fft_mode, o.scan_mode);
     const double t_build = now_s();
     scan.build(o.dataset, n, static_cast<int>(o.window));
     const double build_s = now_s() - t_build;

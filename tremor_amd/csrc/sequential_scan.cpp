@@ -50,6 +50,12 @@ void SequentialScan::check(int rc) const
         throw std::runtime_error(tmx_last_error(ctx_));
 }
 
+void SequentialScan::set_forms(int fft_mode, int scan_mode)
+{
+    check(tmx_set_fft_mode(ctx_, fft_mode));
+    check(tmx_set_scan_mode(ctx_, scan_mode));
+}
+
 void SequentialScan::build(const std::string &filename, idx_t total_samples, int window_length)
 {
     std::vector<float> raw(static_cast<size_t>(total_samples));
