@@ -9,10 +9,12 @@
 //     B[l][c]   = x~[J0 + 32 c + l]  (strided Hankel view of the waveform),
 // with l over K_t = roundup(m + 31, 16).  One 32x32 output tile holds 1024
 // consecutive windows of ONE template, so all its window statistics are for
-// one length, and every MFMA operand fragment is 8 consecutive samples
-// starting at a multiple of 8: aligned 16-byte loads, no shuffles.  A_t is
-// packed once per call in the exact per-lane fragment order (1 KB per
-// k-step), B is staged per workgroup in LDS from the float32 waveform,
+// one length, and every MFMA operand fragment is 8 consecutive samples: a
+// B fragment starts at a multiple of 8 (aligned 16-byte LDS reads, no
+// shuffles), an A fragment is 8 consecutive template values, read with one
+// unaligned 16-byte load from a zero-padded split copy of the template
+// (a few KB, L1-resident: no 32-fold expanded operand streams through the
+// caches).  B is staged per workgroup in LDS from the float32 waveform,
 // centred on the block mean c_b and split on the fly.
 //
 // Precision forms (FP32 accumulation in both):
@@ -94,8 +96,12 @@ __device__ inline uint16_t f16_bits(float v)
 
 // ---- operand packing --------------------------------------------------------
 
-// One thread per (template, k-step, lane): the lane's 8 Toeplitz entries
-// q[16 kappa + 8 h + e - s], split into parts.
+// Per template, the split template values in two 16-bit arrays (hi, then
+// lo) of P = kpad + 64 entries, entry 32 + i holding q[i] and every other
+// entry zero.  The lane (s, h) operand fragment of k-step kappa,
+// A[s][16 kappa + 8 h + e] = q[16 kappa + 8 h + e - s], e < 8, is then the 16
+// bytes at entry 32 + 16 kappa + 8 h - s: one (unaligned) vector load from an
+// array small enough to stay in the L1 cache.  One thread per entry.
 __global__ void k_pack(const TmplDev *__restrict__ tmpl, int count, const float *__restrict__ qz, int precision,
                        Frag *__restrict__ a)
 {
@@ -103,33 +109,23 @@ __global__ void k_pack(const TmplDev *__restrict__ tmpl, int count, const float 
     if (t >= count)
         return;
     const int m = tmpl[t].m;
-    const int ksteps = tmpl[t].kpad / 16;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int kappa = idx >> 6, lane = idx & 63;
-    if (kappa >= ksteps)
+    const int P = tmpl[t].kpad + 64;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P)
         return;
-    const float *q = qz + tmpl[t].q_off;
-    const int s = lane & 31, h = lane >> 5;
-    uint16_t hi[8], lo[8];
-    for (int e = 0; e < 8; e++) {
-        const int i = 16 * kappa + 8 * h + e - s;
-        const float v = (i >= 0 && i < m) ? q[i] : 0.0f;
-        if (precision == TMX_PREC_FP16) {
-            hi[e] = f16_bits(v * 128.0f);  // |q| * 2^7 <= 2^15: no overflow, tiny values stay normal
-            lo[e] = 0;
-        } else {
-            hi[e] = bf16_rne(v);
-            lo[e] = bf16_rne(v - bf16_to_f(hi[e]));
-        }
+    const int i = e - 32;
+    const float v = (i >= 0 && i < m) ? qz[tmpl[t].q_off + i] : 0.0f;
+    uint16_t hi, lo;
+    if (precision == TMX_PREC_FP16) {
+        hi = f16_bits(v * 128.0f);  // |q| * 2^7 <= 2^15: no overflow, tiny values stay normal
+        lo = 0;
+    } else {
+        hi = bf16_rne(v);
+        lo = bf16_rne(v - bf16_to_f(hi));
     }
-    Frag fh, fl;
-    for (int w = 0; w < 4; w++) {
-        fh.w[w] = static_cast<uint32_t>(hi[2 * w]) | (static_cast<uint32_t>(hi[2 * w + 1]) << 16);
-        fl.w[w] = static_cast<uint32_t>(lo[2 * w]) | (static_cast<uint32_t>(lo[2 * w + 1]) << 16);
-    }
-    const int64_t base = tmpl[t].a_off + static_cast<int64_t>(kappa) * 2 * 64;
-    a[base + lane] = fh;
-    a[base + 64 + lane] = fl;
+    uint16_t *base = reinterpret_cast<uint16_t *>(a + tmpl[t].a_off);
+    base[e] = hi;
+    base[P + e] = lo;
 }
 
 // ---- level-1 filter -------------------------------------------------------------
@@ -238,14 +234,21 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
 
     for (int tp = tb; tp < te; tp += kNM) {
         const int nt = te - tp < kNM ? te - tp : kNM;
+        // Operand arrays through buffer resources: the lane's entry offset
+        // (shared by both templates) in one register, the k-step and the lo
+        // part's offset in scalar registers.
         int ks[kNM];
-        const Frag *abase[kNM];
+        uint32_t lo_off[kNM];
+        __amdgpu_buffer_rsrc_t ar[kNM];
 #pragma unroll
         for (int i = 0; i < kNM; i++) {
             const int ti = tp + (i < nt ? i : 0);
             ks[i] = i < nt && !tmpl[ti].ea ? tmpl[ti].kpad / 16 : 0;
-            abase[i] = afrag + tmpl[ti].a_off + lane;
+            lo_off[i] = static_cast<uint32_t>(tmpl[ti].kpad + 64) * 2u;
+            ar[i] = __builtin_amdgcn_make_buffer_rsrc(const_cast<Frag *>(afrag + tmpl[ti].a_off), static_cast<short>(0),
+                                                      static_cast<int>(2u * lo_off[i]), 0x00020000);
         }
+        const uint32_t avoff = static_cast<uint32_t>(32 + 8 * h - col) * 2u;
         const int kt = (ks[0] > ks[kNM - 1] ? ks[0] : ks[kNM - 1]) * 16;
         f32x16 acc[kNM][kNN];
 #pragma unroll
@@ -292,16 +295,17 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
             }
             __syncthreads();
             // Template operands are prefetched one k-step ahead (registers), so
-            // the global-load latency overlaps the previous k-step's MFMAs.
+            // the load latency overlaps the previous k-step's MFMAs.
             Frag nh[kNM], nl[kNM];
             auto load_a = [&](int kappa) {
 #pragma unroll
                 for (int i = 0; i < kNM; i++) {
                     if (kappa < ks[i]) {
-                        const Frag *src = abase[i] + static_cast<int64_t>(kappa) * 128;
-                        nh[i] = src[0];
+                        const uint32_t so = 32u * static_cast<uint32_t>(kappa);
+                        nh[i] = __builtin_bit_cast(Frag, __builtin_amdgcn_raw_buffer_load_b128(ar[i], avoff, so, 0));
                         if (PREC != TMX_PREC_FP16)
-                            nl[i] = src[64];
+                            nl[i] = __builtin_bit_cast(
+                                Frag, __builtin_amdgcn_raw_buffer_load_b128(ar[i], avoff, so + lo_off[i], 0));
                     } else {
                         nh[i] = Frag{{0u, 0u, 0u, 0u}};
                         nl[i] = Frag{{0u, 0u, 0u, 0u}};
@@ -625,16 +629,16 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
 int mfma_group_size() { return kGroup; }
 uint64_t mfma_block_windows() { return kNW; }
 int mfma_kpad(int m) { return ((m + 31 + 15) / 16) * 16; }
-// Two parts (hi, lo) of 64 lanes per k-step, in 16-byte fragments.
-int64_t mfma_frag_count(int kpad) { return static_cast<int64_t>(kpad / 16) * 128; }
+// Two 16-bit arrays (hi, lo) of kpad + 64 entries, in 16-byte units.
+int64_t mfma_frag_count(int kpad) { return static_cast<int64_t>(kpad + 64) / 4; }
 
 hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, const float *qz, int precision, void *a_frag,
                             hipStream_t stream)
 {
     if (count <= 0)
         return hipSuccess;
-    // Longest template: 65536 + 31 -> 4098 k-steps * 64 lanes.
-    const int per_t = (65536 + 47) / 16 * 64;
+    // Longest template: kpad = 65536 + 32 (+ 64 padding entries).
+    const int per_t = 65536 + 32 + 64;
     const dim3 grid((per_t + 255) / 256, count);
     hipLaunchKernelGGL(k_pack, grid, dim3(256), 0, stream, tmpl, count, qz, precision, static_cast<Frag *>(a_frag));
     return hipGetLastError();
