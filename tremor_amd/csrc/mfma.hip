@@ -294,8 +294,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                 }
             }
             __syncthreads();
-            // Template operands are prefetched one k-step ahead (registers), so
-            // the load latency overlaps the previous k-step's MFMAs.
+            // Template operands: loaded at each k-step (the small operand
+            // arrays stay in the L1 cache, and the other wavefront of the
+            // SIMD covers the latency).
             Frag nh[kNM], nl[kNM];
             auto load_a = [&](int kappa) {
 #pragma unroll
@@ -312,19 +313,17 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                     }
                 }
             };
-            load_a(k0 >> 4);
             for (int kk = 0; kk < kc; kk += 16) {
                 const int kappa = (k0 + kk) >> 4;
+                load_a(kappa);
                 Frag ah[kNM], al[kNM];
 #pragma unroll
                 for (int i = 0; i < kNM; i++) {
                     ah[i] = nh[i];
                     al[i] = nl[i];
                 }
-                if (kk + 16 < kc)
-                    load_a(kappa + 1);
 #pragma unroll
-                for (int jn = 0; jn < kNN; jn++) {
+                for (int jn = 0; jn < (PREC == TMX_PREC_FP16 ? kNN : 0); jn++) {
                     const int cc = (wave * kNN + jn) * 32 + col;  // column within the block
                     const int e = swz(32 * cc + kk + 8 * h);
                     const Frag bh = *reinterpret_cast<const Frag *>(&lhi[e]);
@@ -337,20 +336,48 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                             __builtin_memcpy(&av, &ah[i], 16);
                             acc[i][jn] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc[i][jn], 0, 0, 0);
                         }
-                    } else {
-                        const Frag bl = *reinterpret_cast<const Frag *>(&llo[e]);
-                        bf16x8 bvh, bvl;
-                        __builtin_memcpy(&bvh, &bh, 16);
-                        __builtin_memcpy(&bvl, &bl, 16);
+                    }
+                }
+                if (PREC != TMX_PREC_FP16) {
+                    // Two column tiles at a time: the three products of each
+                    // accumulator (same order as always) interleaved with the
+                    // other three accumulators' products.
+#pragma unroll
+                    for (int jp = 0; jp < kNN; jp += 2) {
+                        bf16x8 bvh[2], bvl[2];
+#pragma unroll
+                        for (int q = 0; q < 2; q++) {
+                            const int cc = (wave * kNN + jp + q) * 32 + col;
+                            const int e = swz(32 * cc + kk + 8 * h);
+                            const Frag bh = *reinterpret_cast<const Frag *>(&lhi[e]);
+                            const Frag bl = *reinterpret_cast<const Frag *>(&llo[e]);
+                            __builtin_memcpy(&bvh[q], &bh, 16);
+                            __builtin_memcpy(&bvl[q], &bl, 16);
+                        }
+                        bf16x8 avh[kNM], avl[kNM];
 #pragma unroll
                         for (int i = 0; i < kNM; i++) {
-                            bf16x8 avh, avl;
-                            __builtin_memcpy(&avh, &ah[i], 16);
-                            __builtin_memcpy(&avl, &al[i], 16);
-                            acc[i][jn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(avl, bvh, acc[i][jn], 0, 0, 0);
-                            acc[i][jn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(avh, bvl, acc[i][jn], 0, 0, 0);
-                            acc[i][jn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(avh, bvh, acc[i][jn], 0, 0, 0);
+                            __builtin_memcpy(&avh[i], &ah[i], 16);
+                            __builtin_memcpy(&avl[i], &al[i], 16);
                         }
+#pragma unroll
+                        for (int q = 0; q < 2; q++)
+#pragma unroll
+                            for (int i = 0; i < kNM; i++)
+                                acc[i][jp + q] =
+                                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(avl[i], bvh[q], acc[i][jp + q], 0, 0, 0);
+#pragma unroll
+                        for (int q = 0; q < 2; q++)
+#pragma unroll
+                            for (int i = 0; i < kNM; i++)
+                                acc[i][jp + q] =
+                                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(avh[i], bvl[q], acc[i][jp + q], 0, 0, 0);
+#pragma unroll
+                        for (int q = 0; q < 2; q++)
+#pragma unroll
+                            for (int i = 0; i < kNM; i++)
+                                acc[i][jp + q] =
+                                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(avh[i], bvh[q], acc[i][jp + q], 0, 0, 0);
                     }
                 }
             }
