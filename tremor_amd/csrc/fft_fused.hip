@@ -535,6 +535,14 @@ __global__ __launch_bounds__(256) void k_chunk_windows(Waveform w, const TmplDev
 
 __global__ void k_zero_nrec(unsigned long long *p) { *p = 0ull; }
 
+// nrec[1] = max(nrec[1], nrec[0]): the largest queue count of a pass made of
+// several launches.
+__global__ void k_note_max(unsigned long long *nrec)
+{
+    if (nrec[0] > nrec[1])
+        nrec[1] = nrec[0];
+}
+
 }  // namespace
 
 // Table E[t] = e^{i pi t / M}, t in [0, 2M), for the fused pass.
@@ -543,9 +551,10 @@ int fft_fused_block() { return kB; }
 
 size_t fft_chunk_rec_bytes() { return sizeof(ChunkRec); }
 
-// rec / rec_dt: queue of rec_cap records (rec_dt: 64 floats each); *nrec gets
-// the number the pass produced (more than rec_cap: the pass must be rerun
-// with a larger queue in main mode; in seed mode the entries kept suffice).
+// rec / rec_dt: queue of rec_cap records (rec_dt: 64 floats each); nrec[0]
+// gets the number the launch produced, nrec[1] the largest such number since
+// it was last cleared (more than rec_cap: the pass must be rerun with a larger
+// queue in main mode; in seed mode the entries kept suffice).
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
                             const float2 *table, uint64_t P, int stride, double kfft, int mode, void *rec,
@@ -563,6 +572,7 @@ hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int 
                        pnorm, table, P, stride, kfft, mode, static_cast<ChunkRec *>(rec), rec_dt, nrec, rec_cap);
     hipLaunchKernelGGL(k_chunk_windows, dim3(static_cast<unsigned>(4 * grid)), dim3(256), 0, stream, w, tmpl,
                        static_cast<const ChunkRec *>(rec), rec_dt, nrec, rec_cap, mode, co);
+    hipLaunchKernelGGL(k_note_max, dim3(1), dim3(1), 0, stream, nrec);
     return hipGetLastError();
 }
 

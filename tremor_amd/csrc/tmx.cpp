@@ -161,6 +161,7 @@ double fft_seconds_per_window(int L)
 // as the reference decides after its sampled early-abandon test,
 // Tremor.cpp:2265-2286): a fixed cost per window (statistics) plus one per
 // 32-sample block visited, the blocks measured on a sample of windows.
+constexpr int kFusedChunk = 16;  // templates per fused FFT launch
 constexpr double kScanWindow = 9.0e-12;
 constexpr double kScanBlock = 3.6e-12;
 constexpr uint64_t kScanSample = 4096;
@@ -525,13 +526,20 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
             LANE_HIP(L.chunk_dt.reserve(sizeof(float) * 64 * want), "hipMalloc chunk queue");
             L.chunk_cap = want;
         }
-        LANE_HIP(L.chunk_n.reserve(sizeof(unsigned long long)), "hipMalloc chunk queue");
-        LANE_HIP(launch_fft_fused(w, L.tmpl.as<TmplDev>(), plan.Tg, F, f.spectra.as<float2>(), f.bins,
-                                  L.qspec.as<float2>(), L.qmax.as<float>(), f.pinfo.as<float4>(), f.pnorm.as<float2>(),
-                                  f.table.as<float2>(), f.P, stride, kfft, mode, L.chunk_rec.p,
-                                  L.chunk_dt.as<float>(), L.chunk_n.as<unsigned long long>(), L.chunk_cap,
-                                  cand_out(L), L.cus > 0 ? L.cus : 256, L.stream),
-                 "fused FFT pass");
+        LANE_HIP(L.chunk_n.reserve(2 * sizeof(unsigned long long)), "hipMalloc chunk queue");
+        LANE_HIP(launch_zero_u64(L.chunk_n.as<unsigned long long>() + 1, L.stream), "reset chunk counter");
+        // Launches of at most kFusedChunk templates: their spectra (8 (B/2 + 1)
+        // bytes each) stay in L2 while the pieces stream past.
+        for (int c0 = 0; c0 < F; c0 += kFusedChunk) {
+            const int Fc = std::min(kFusedChunk, F - c0);
+            LANE_HIP(launch_fft_fused(w, L.tmpl.as<TmplDev>(), plan.Tg + c0, Fc, f.spectra.as<float2>(), f.bins,
+                                      L.qspec.as<float2>() + static_cast<size_t>(c0) * f.bins, L.qmax.as<float>() + c0,
+                                      f.pinfo.as<float4>(), f.pnorm.as<float2>(), f.table.as<float2>(), f.P, stride,
+                                      kfft, mode, L.chunk_rec.p, L.chunk_dt.as<float>(),
+                                      L.chunk_n.as<unsigned long long>(), L.chunk_cap, cand_out(L),
+                                      L.cus > 0 ? L.cus : 256, L.stream),
+                     "fused FFT pass");
+        }
         return 0;
     }
     std::vector<FftPair> pairs;
@@ -626,7 +634,8 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     LANE_HIP(hipMemcpyAsync(&c, L.cand_count.p, sizeof(c), hipMemcpyDeviceToHost, L.stream), "read counter");
     const bool fused = f && f->fused && fft_any;
     if (fused)
-        LANE_HIP(hipMemcpyAsync(&nchunk, L.chunk_n.p, sizeof(nchunk), hipMemcpyDeviceToHost, L.stream),
+        LANE_HIP(hipMemcpyAsync(&nchunk, L.chunk_n.as<unsigned long long>() + 1, sizeof(nchunk),
+                                hipMemcpyDeviceToHost, L.stream),
                  "read chunk counter");
     LANE_HIP(hipStreamSynchronize(L.stream), "candidate pass");
     // A main pass whose chunk queue overflowed lost candidates: rerun.
