@@ -58,6 +58,21 @@ def _args():
     return ap.parse_args()
 
 
+def _measured_traffic(n, templates, k, prec):
+    """HBM bytes per main launch of the matrix-core kernel on this workload,
+    from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded under profiles/
+    (null for any other workload or precision)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_scan_mfma_c1_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("workload") != {"n": n, "templates": templates, "k": k, "precision": prec}:
+        return None
+    return {"bytes": rec["hbm_bytes_per_launch"], "unit": "B", "source": rec["source"]}
+
+
 def cpu_baseline(x, templates, k, budget_s):
     """The oracle's SkipSequentialScan::search_knn restatement (OpenMP,
     dynamic 2^16-window chunks, early abandoning) on a bounded sample:
@@ -430,7 +445,8 @@ def main():
                    "n": int(w.x.size), "templates": len(w.templates), "k": a.k, "precision": a.prec},
         "roofline": {"bound": "mfma" if a.prec != "fp32" else "fp32-vector", "achieved": achieved / 1e12,
                      "peak": PEAK[a.prec] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[a.prec],
-                     "traffic": None,
+                     "traffic": (_measured_traffic(w.x.size, len(w.templates), a.k, a.prec) or {}).get("bytes"),
+                     "traffic_source": (_measured_traffic(w.x.size, len(w.templates), a.k, a.prec) or {}).get("source"),
                      "kernel": "k_scan_mfma" if a.prec != "fp32" else "k_scan_fp32",
                      "kernel_ms": kms, "algorithmic_flops_per_launch": flops,
                      "note": "achieved counts 2m flops per comparison once; split-BF16 issues 3 MFMA products, "
