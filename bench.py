@@ -49,9 +49,9 @@ def _args():
     ap.add_argument("--single-duration", type=float, default=3.0,
                     help="c3: seconds of the same stream replayed one request per launch")
     ap.add_argument("--lanes", type=int, default=2)
-    ap.add_argument("--sweep", type=float, nargs="*", default=[80.0, 60.0, 40.0, 25.0],
+    ap.add_argument("--sweep", type=float, nargs="*", default=[60.0, 45.0, 35.0, 25.0],
                     help="c3: lower arrival rates to replay for the p99 <= 50 ms search")
-    ap.add_argument("--sweep-duration", type=float, default=4.0)
+    ap.add_argument("--sweep-duration", type=float, default=6.0)
     ap.add_argument("--sweep-batch", type=int, default=16)
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--long-subset", type=int, default=8, help="c4: templates of the GEMM and scan legs")

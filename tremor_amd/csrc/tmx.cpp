@@ -126,15 +126,17 @@ struct CallPlan {
 // Cost model (per template, seconds per window), in the role of the
 // reference's adaptive choice between scan and FFT (Tremor.cpp:2265-2286,
 // fft_fallback.hpp:55-60): the matrix-core form costs in proportion to the
-// padded template length plus a fixed epilogue; the fused FFT form (B = 2^15,
-// templates up to 8192 samples) a constant per window; the FFT form with
-// hipFFT transforms (longer templates) in proportion to B log2 B per piece of
-// (B - Lmax + 1) windows.  Constants measured on MI355X; FFT iff it is
-// estimated below 0.9 x the inner-product form, as in the reference.
-double gemm_seconds_per_window(int precision, int kpad)
+// padded template length, plus the staging and epilogue of each waveform
+// block shared by the templates of a group (at most 8, fewer in a small
+// batch); the fused FFT form (B = 2^15, templates up to 8192 samples) a
+// constant per window; the FFT form with hipFFT transforms (longer
+// templates) in proportion to B log2 B per piece of (B - Lmax + 1) windows.
+// Constants measured on MI355X; FFT iff it is estimated below 0.9 x the
+// inner-product form, as in the reference.
+double gemm_seconds_per_window(int precision, int kpad, int group)
 {
-    const double per_k = precision == TMX_PREC_FP16 ? 2.2e-15 : 4.6e-15;
-    return per_k * kpad + 2.1e-12;
+    const double per_k = precision == TMX_PREC_FP16 ? 1.64e-15 : 4.0e-15;
+    return per_k * kpad + 5.7e-12 / std::max(1, std::min(group, 8));
 }
 
 int fft_block_for(int L)
@@ -161,7 +163,8 @@ double fft_seconds_per_window(int L)
 // as the reference decides after its sampled early-abandon test,
 // Tremor.cpp:2265-2286): a fixed cost per window (statistics) plus one per
 // 32-sample block visited, the blocks measured on a sample of windows.
-constexpr int kFusedChunk = 16;  // templates per fused FFT launch
+constexpr int kFusedChunk = 16;    // templates per fused FFT launch
+constexpr int kFftMinLength = 256;  // shortest template the cost model sends to the FFT form
 constexpr double kScanWindow = 9.0e-12;
 constexpr double kScanBlock = 3.6e-12;
 constexpr uint64_t kScanSample = 4096;
@@ -176,7 +179,7 @@ double planned_seconds_per_window(const tmx_cfg &cfg, const CallPlan &plan, int 
         return fft_seconds_per_window(plan.fft_L);
     if (cfg.precision == TMX_PREC_FP32)
         return 8.0e-15 * t.m + 2.0e-12;
-    return gemm_seconds_per_window(cfg.precision, t.kpad);
+    return gemm_seconds_per_window(cfg.precision, t.kpad, plan.Tg);
 }
 
 int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count, CallPlan &plan,
@@ -244,9 +247,16 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
         if (cfg.fft_mode == TMX_FFT_ALWAYS) {
             plan.Tg = 0;
         } else {
+            // One FFT class per batch, sized by its longest FFT template (the
+            // first to move): every FFT template pays that class's cost.
+            // Templates shorter than kFftMinLength stay on the matrix cores:
+            // the FFT form's 64-window chunk filter needs a common core of a
+            // few hundred samples to prune (m = 128: 2.6x its cost at m = 300).
+            const int L_class = plan.tmpl[count - 1].m;
             while (plan.Tg > 0) {
                 const TmplDev &t = plan.tmpl[plan.Tg - 1];
-                if (!(fft_seconds_per_window(t.m) < 0.9 * gemm_seconds_per_window(cfg.precision, t.kpad)))
+                if (t.m < kFftMinLength ||
+                    !(fft_seconds_per_window(L_class) < 0.9 * gemm_seconds_per_window(cfg.precision, t.kpad, plan.Tg)))
                     break;
                 plan.Tg--;
             }
