@@ -171,7 +171,7 @@ struct Lane {
 struct FftClass {
     int B = 0, Lmax = 0, step = 0, bins = 0;
     uint64_t P = 0;
-    bool fused = false;  // B = 2^15: fused LDS pass (twiddle table, no rebased prefix pairs)
+    bool fused = false;  // B = 2^14 or 2^15: fused LDS pass (twiddle table, no rebased prefix pairs)
     DevMem spectra, pinfo, pnorm, rb, norm2, work_real, table;
     hipfftHandle fwd = 0;
     FftClass() = default;

@@ -1,7 +1,11 @@
-// Fused overlap-save FFT candidate pass for FFT length B = 2^15 (templates up
-// to Lmax = B / 4 = 8192 samples), in the role of the reference's FFT
-// fallback pass (fft_fallback.hpp:67-117) with its filter-then-exact-recheck
-// contract.  One workgroup of 1024 threads per CU; each takes pieces in turn.
+// Fused overlap-save FFT candidate pass for FFT lengths B = 2^15 and 2^14
+// (templates up to Lmax = B / 4 = 8192 / 4096 samples), in the role of the
+// reference's FFT fallback pass (fft_fallback.hpp:67-117) with its
+// filter-then-exact-recheck contract.  B = 2^15: one workgroup of 1024
+// threads per CU; B = 2^14: two of 512 (the LDS holds two transforms, so one
+// workgroup's barriers overlap the other's work).  Each workgroup takes
+// pieces in turn.  (Constants below are for B = 2^15; B = 2^14 halves the
+// thread count and the transform, and its last pass is radix 2.)
 // Per piece and template of the batch (the piece spectrum X_p, hipFFT r2c
 // cached per waveform, comes from HBM once per batch and from L2 after):
 //
@@ -39,13 +43,21 @@
 namespace tmx {
 namespace {
 
-constexpr int kM = 16384;            // complex points of the half-length transform
-constexpr int kB = 2 * kM;           // real length B
-constexpr int kT = 1024;             // threads per workgroup
-constexpr int kStep = kB - kB / 4 + 1;        // windows per piece
-constexpr int kChunks = (kStep + 63) / 64 + 1;  // 64-output chunks holding a piece's windows
-static_assert(kChunks <= kT, "one thread per chunk");
-constexpr int kLds = kM + (kM >> 4) + 2;      // padded complex slots (index M included)
+// Geometry of the fused pass for B = 2M = 2^(LOG2M + 1): 16 points per
+// thread; B = 2^15 (templates up to 8192) runs one workgroup of 1024 threads
+// per CU, B = 2^14 (templates up to 4096) two of 512 (half the LDS each).
+template <int LOG2M>
+struct Geom {
+    static constexpr int kM = 1 << LOG2M;                    // complex points of the half-length transform
+    static constexpr int kB = 2 * kM;                        // real length B
+    static constexpr int kT = kM / 16;                       // threads per workgroup
+    static constexpr int kStep = kB - kB / 4 + 1;            // windows per piece
+    static constexpr int kChunks = (kStep + 63) / 64 + 1;    // 64-output chunks holding a piece's windows
+    static constexpr int kLds = kM + (kM >> 4) + 2;          // padded complex slots (index M included)
+    static constexpr int kPerCU = LOG2M == 14 ? 1 : 2;       // workgroups per CU
+    static_assert(kChunks <= kT, "one thread per chunk");
+    static_assert(LOG2M == 13 || LOG2M == 14, "B = 2^14 or 2^15");
+};
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 
@@ -208,28 +220,31 @@ __device__ double chunk_dist_lo(double2 pa, double2 pb, uint64_t a, uint64_t b, 
 // Steps 1 and 2 for one (piece, template): on return data[pad(r)] holds
 // z[r] = y[2r] + i y[2r+1], the unnormalised circular correlation of the
 // piece with the template (all threads synchronised).
+template <int LOG2M>
 __device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, float2 Xn, __amdgpu_buffer_rsrc_t Hr,
                                                 __amdgpu_buffer_rsrc_t Er)
 {
+    using G = Geom<LOG2M>;
+    constexpr int kM = G::kM, kT = G::kT;
     const int tid = opaque(threadIdx.x);
     const uint32_t toff = static_cast<uint32_t>(tid) * 8u;
-    // 1. Y = X .* H into LDS (X: bins tid + 1024 r, and M for thread 0).
+    // 1. Y = X .* H into LDS (X: bins tid + kT r, and M for thread 0).
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        v[r] = cmul(X[r], bload(Hr, toff, 8192u * r));
-        data[pad(tid + 1024 * r)] = v[r];
+        v[r] = cmul(X[r], bload(Hr, toff, 8u * kT * r));
+        data[pad(tid + kT * r)] = v[r];
     }
     if (tid == 0)
         data[pad(kM)] = cmul(Xn, bload(Hr, 0, 8u * kM));
     __syncthreads();
     // 2a. Z formation (registers) = the first pass's inputs; the
-    // factor e^{i pi k / M} for k = tid + 1024 r is E[tid] e^{i pi r / 16}.
+    // factor e^{i pi k / M} for k = tid + kT r is E[tid] e^{i pi r / 16}.
     {
         const float2 e0 = bload(Er, toff, 0);
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const int k = tid + 1024 * r;
+            const int k = tid + kT * r;
             const float2 ym = data[pad(kM - k)];
             const float2 cy = make_float2(ym.x, -ym.y);
             const float2 A = cadd(v[r], cy);
@@ -248,7 +263,7 @@ __device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, f
     // Pass 2: radix 16, Ns = 16.
 #pragma unroll
     for (int r = 0; r < 16; r++)
-        v[r] = data[pad(tid + 1024 * r)];
+        v[r] = data[pad(tid + kT * r)];
     twiddle16(v, Er, tid & 15, 2 * kM / 256);
     dft16(v);
     __syncthreads();
@@ -259,7 +274,7 @@ __device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, f
     // Pass 3: radix 16, Ns = 256.
 #pragma unroll
     for (int r = 0; r < 16; r++)
-        v[r] = data[pad(tid + 1024 * r)];
+        v[r] = data[pad(tid + kT * r)];
     twiddle16(v, Er, tid & 255, 2 * kM / 4096);
     dft16(v);
     __syncthreads();
@@ -267,11 +282,11 @@ __device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, f
     for (int r = 0; r < 16; r++)
         data[pad((tid >> 8) * 4096 + (tid & 255) + 256 * r)] = v[r];
     __syncthreads();
-    // Pass 4: radix 4, Ns = 4096; each thread owns its four
+    // Pass 4, B = 2^15: radix 4, Ns = 4096; each thread owns its four
     // butterflies' slots, so no barrier between read and write.  The
     // twiddles e^{2 pi i r jj / M} for jj = tid + 1024 s are
     // E[2 r tid] e^{i pi r s / 8}.
-    {
+    if constexpr (LOG2M == 14) {
         const float2 e1 = bload(Er, 2 * toff, 0), e2 = bload(Er, 4 * toff, 0), e3 = bload(Er, 6 * toff, 0);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
@@ -286,6 +301,18 @@ __device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, f
             data[pad(jj + 4096)] = b;
             data[pad(jj + 8192)] = cc;
             data[pad(jj + 12288)] = d;
+        }
+    } else {
+        // Pass 4, B = 2^14: radix 2, Ns = 4096, butterflies jj = tid + 512 s
+        // with factor e^{2 pi i jj / M} = E[2 tid] e^{i pi s / 8}.
+        const float2 e1 = bload(Er, 2 * toff, 0);
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int jj = tid + 512 * s;
+            const float2 a = data[pad(jj)];
+            const float2 b = cmul(data[pad(jj + 4096)], s == 0 ? e1 : cmul(e1, make_float2(kZc[2 * s], kZs[2 * s])));
+            data[pad(jj)] = cadd(a, b);
+            data[pad(jj + 4096)] = csub(a, b);
         }
     }
     __syncthreads();
@@ -304,13 +331,16 @@ struct ChunkRec {
     uint32_t pad0;
 };
 
-__global__ __launch_bounds__(kT, 1) void k_fft_fused(
+template <int LOG2M>
+__global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves per SIMD: kPerCU workgroups per CU
     Waveform w, const TmplDev *__restrict__ tmpl, int t0, int F, const float2 *__restrict__ spectra, int bins,
     const float2 *__restrict__ qspec, const float *__restrict__ qmax, const float4 *__restrict__ pinfo,
     const float2 *__restrict__ pnorm, const float2 *__restrict__ E, uint64_t P, int stride, double kfft, int mode,
     ChunkRec *__restrict__ rec, float *__restrict__ rec_dt, unsigned long long *__restrict__ nrec, uint64_t rec_cap)
 {
-    __shared__ float2 data[kLds];
+    using G = Geom<LOG2M>;
+    constexpr int kM = G::kM, kB = G::kB, kT = G::kT, kStep = G::kStep;
+    __shared__ float2 data[G::kLds];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const float *fdata = reinterpret_cast<const float *>(data);
@@ -329,7 +359,7 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
             const uint32_t toff = static_cast<uint32_t>(opaque(tid)) * 8u;
 #pragma unroll
             for (int r = 0; r < 16; r++)
-                X[r] = bload(Xr, toff, 8192u * r);
+                X[r] = bload(Xr, toff, 8u * kT * r);
             if (tid == 0)
                 Xn = bload(Xr, 0, 8u * kM);
         }
@@ -373,7 +403,7 @@ __global__ __launch_bounds__(kT, 1) void k_fft_fused(
             }
             const __amdgpu_buffer_rsrc_t Hr = rsrc(qspec + static_cast<uint64_t>(ti) * static_cast<uint64_t>(bins),
                                                    bins * 8u);
-            correlate_piece(data, X, Xn, Hr, Er);
+            correlate_piece<LOG2M>(data, X, Xn, Hr, Er);
 
             int n_first, n_end;
             chunk(n_first, n_end);
@@ -546,8 +576,9 @@ __global__ void k_note_max(unsigned long long *nrec)
 }  // namespace
 
 // Table E[t] = e^{i pi t / M}, t in [0, 2M), for the fused pass.
-int fft_fused_table_len() { return 2 * kM; }
-int fft_fused_block() { return kB; }
+// The fused pass serves B = 2^14 and 2^15; its table E[t] = e^{i pi t / M},
+// t in [0, 2M), has B entries.
+bool fft_fused_supports(int B) { return B == Geom<13>::kB || B == Geom<14>::kB; }
 
 size_t fft_chunk_rec_bytes() { return sizeof(ChunkRec); }
 
@@ -558,19 +589,27 @@ size_t fft_chunk_rec_bytes() { return sizeof(ChunkRec); }
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
                             const float2 *table, uint64_t P, int stride, double kfft, int mode, void *rec,
-                            float *rec_dt, unsigned long long *nrec, uint64_t rec_cap, const CandOut &co, int grid,
+                            float *rec_dt, unsigned long long *nrec, uint64_t rec_cap, const CandOut &co, int cus,
                             hipStream_t stream)
 {
     if (F <= 0 || P == 0)
         return hipSuccess;
-    if (bins != kM + 1)
+    const bool small = bins == Geom<13>::kM + 1;
+    if (!small && bins != Geom<14>::kM + 1)
         return hipErrorInvalidValue;
+    const int grid = cus * (small ? Geom<13>::kPerCU : Geom<14>::kPerCU);
     const uint64_t nwork = (P + stride - 1) / stride;
     const unsigned g = static_cast<unsigned>(nwork < static_cast<uint64_t>(grid) ? nwork : grid);
     hipLaunchKernelGGL(k_zero_nrec, dim3(1), dim3(1), 0, stream, nrec);
-    hipLaunchKernelGGL(k_fft_fused, dim3(g), dim3(kT), 0, stream, w, tmpl, t0, F, spectra, bins, qspec, qmax, pinfo,
-                       pnorm, table, P, stride, kfft, mode, static_cast<ChunkRec *>(rec), rec_dt, nrec, rec_cap);
-    hipLaunchKernelGGL(k_chunk_windows, dim3(static_cast<unsigned>(4 * grid)), dim3(256), 0, stream, w, tmpl,
+    if (small)
+        hipLaunchKernelGGL(k_fft_fused<13>, dim3(g), dim3(Geom<13>::kT), 0, stream, w, tmpl, t0, F, spectra, bins,
+                           qspec, qmax, pinfo, pnorm, table, P, stride, kfft, mode, static_cast<ChunkRec *>(rec),
+                           rec_dt, nrec, rec_cap);
+    else
+        hipLaunchKernelGGL(k_fft_fused<14>, dim3(g), dim3(Geom<14>::kT), 0, stream, w, tmpl, t0, F, spectra, bins,
+                           qspec, qmax, pinfo, pnorm, table, P, stride, kfft, mode, static_cast<ChunkRec *>(rec),
+                           rec_dt, nrec, rec_cap);
+    hipLaunchKernelGGL(k_chunk_windows, dim3(static_cast<unsigned>(4 * cus)), dim3(256), 0, stream, w, tmpl,
                        static_cast<const ChunkRec *>(rec), rec_dt, nrec, rec_cap, mode, co);
     hipLaunchKernelGGL(k_note_max, dim3(1), dim3(1), 0, stream, nrec);
     return hipGetLastError();

@@ -55,8 +55,7 @@ int mfma_kpad(int m);
 hipError_t launch_piece_input(const float *x, uint64_t n, int step, int B, uint64_t p0, int cnt, float4 *pinfo,
                               float *out, double *norm2, hipStream_t stream);
 // fft_fused.hip
-int fft_fused_table_len();
-int fft_fused_block();
+bool fft_fused_supports(int B);
 size_t fft_chunk_rec_bytes();
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
@@ -128,8 +127,8 @@ struct CallPlan {
 // fft_fallback.hpp:55-60): the matrix-core form costs in proportion to the
 // padded template length, plus the staging and epilogue of each waveform
 // block shared by the templates of a group (at most 8, fewer in a small
-// batch); the fused FFT form (B = 2^15, templates up to 8192 samples) a
-// constant per window; the FFT form with hipFFT transforms (longer
+// batch); the fused FFT form (B = 2^14 for templates up to 4096 samples,
+// 2^15 up to 8192) a constant per window of the class; the FFT form with hipFFT transforms (longer
 // templates) in proportion to B log2 B per piece of (B - Lmax + 1) windows.
 // Constants measured on MI355X; FFT iff it is estimated below 0.9 x the
 // inner-product form, as in the reference.
@@ -141,7 +140,7 @@ double gemm_seconds_per_window(int precision, int kpad, int group)
 
 int fft_block_for(int L)
 {
-    int B = 1 << 15;
+    int B = 1 << 14;
     while (B < 4 * L)
         B *= 2;
     return B;
@@ -150,6 +149,8 @@ int fft_block_for(int L)
 double fft_seconds_per_window(int L)
 {
     const int B = fft_block_for(L);
+    if (B == (1 << 14))
+        return 3.4e-12;
     if (B == (1 << 15))
         return 4.5e-12;
     double lg = 0.0;
@@ -323,7 +324,9 @@ int build_block_prefix(tmx_ctx *ctx, Lane &L, int need)
 constexpr int kFftPieceBatch = 64;  // pieces / templates per forward transform launch
 
 // Piece spectra of the current waveform for templates up to L samples:
-// B = 2^15 doubled until >= 4 L (fft_fallback.cpp:124-126), Lmax = B / 4,
+// B = 2^14 doubled until >= 4 L (the reference starts at 2^15,
+// fft_fallback.cpp:124-126; 2^14 lets two fused workgroups share a CU),
+// Lmax = B / 4,
 // step = B - Lmax + 1; per piece the centre, the rebased prefix pairs of the
 // level-1 filter, ||x~||_2 and ||X||_inf.
 int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
@@ -338,7 +341,7 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
     f.step = B - f.Lmax + 1;
     f.bins = B / 2 + 1;
     f.P = (ctx->n + f.step - 1) / f.step;
-    f.fused = B == fft_fused_block();
+    f.fused = fft_fused_supports(B);
     const uint64_t Palloc = ((f.P + kFftPieceBatch - 1) / kFftPieceBatch) * kFftPieceBatch;
     LANE_HIP(f.spectra.reserve(sizeof(float2) * Palloc * f.bins), "hipMalloc FFT spectra");
     LANE_HIP(f.pinfo.reserve(sizeof(float4) * f.P), "hipMalloc FFT pieces");
@@ -349,7 +352,7 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
     LANE_HIP(f.work_real.reserve(sizeof(float) * static_cast<size_t>(kFftPieceBatch) * B), "hipMalloc FFT input");
     if (f.fused) {
         // E[t] = e^{i pi t / M}, M = B / 2, rounded from FP64.
-        const int len = fft_fused_table_len();
+        const int len = B;
         std::vector<float2> tab(len);
         const double M = static_cast<double>(B / 2);
         for (int t = 0; t < len; t++) {
