@@ -302,6 +302,31 @@ def test_fused_fft_many_templates_matches_mfma(oracle):
     assert np.array_equal(d[:2].view(np.uint32), do.view(np.uint32))
 
 
+@pytest.mark.parametrize("m", [4096, 4097])
+def test_fused_fft_class_boundary(oracle, m):
+    """Templates of 4096 samples take the B = 2^14 fused class (two
+    workgroups per CU, radix-2 last pass), 4097 the B = 2^15 class; both
+    must return the oracle's answers, alone and batched with a short
+    template."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(400_000, 41, events=20)
+    tm = synth.cut_templates(x, onsets, [m, 300], rng)
+    with Context(precision="bf16x3", fft="always") as ctx:
+        ctx.load_waveform(x)
+        _log("fft-class", m=m)
+        pos, d, st = ctx.knn(tm[:1], 7, 0)
+        print("  stats", st, flush=True)
+        assert st["fft_block"] == (16384 if m <= 4096 else 32768)
+        po, do, _ = oracle.scan_knn(x, tm[:1], 7, 0)
+        assert np.array_equal(pos, po)
+        assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+        r = ctx.threshold(tm, 0.6, 5)
+        tt, tp, td = oracle.scan_threshold(x, tm, 0.6, 5)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
 @pytest.mark.parametrize("rho", [0.9, 0.7, 0.5])
 def test_threshold_at_scale_shape(oracle, rho):
     """The threshold workload's shape (repeating-source clusters, lengths
