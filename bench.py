@@ -49,10 +49,12 @@ def _args():
     ap.add_argument("--single-duration", type=float, default=3.0,
                     help="c3: seconds of the same stream replayed one request per launch")
     ap.add_argument("--lanes", type=int, default=2)
-    ap.add_argument("--sweep", type=float, nargs="*", default=[60.0, 45.0, 35.0, 25.0],
+    ap.add_argument("--sweep", type=float, nargs="*", default=[80.0, 60.0, 45.0, 35.0],
                     help="c3: lower arrival rates to replay for the p99 <= 50 ms search")
     ap.add_argument("--sweep-duration", type=float, default=6.0)
-    ap.add_argument("--sweep-batch", type=int, default=16)
+    ap.add_argument("--sweep-batch", type=int, default=4)
+    ap.add_argument("--sweep-lanes", type=int, default=1,
+                    help="c3: lanes of the rate sweep (one lane keeps the tail shortest below saturation)")
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--long-subset", type=int, default=8, help="c4: templates of the GEMM and scan legs")
     return ap.parse_args()
@@ -311,7 +313,7 @@ def bench_serving(a):
     sweep = []
     for rate in a.sweep:
         st = synth.request_stream(w, rate, a.sweep_duration, seed=synth.BASE_SEED + 34 + int(rate))
-        r, _ = _serve(ctx, st, a.lanes, a.sweep_batch, f"rate {rate:g}")
+        r, _ = _serve(ctx, st, a.sweep_lanes, a.sweep_batch, f"rate {rate:g}")
         r["rate"] = rate
         sweep.append(r)
         print(f"[bench] rate {rate:g}/s: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms, "

@@ -125,6 +125,36 @@ def test_serving_threads_and_fft_requests(oracle):
             assert _same((results[i]["pos"], results[i]["d"]), _alone(ctx, reqs[i])), i
 
 
+def test_serving_restart_reuses_lanes(oracle):
+    """A stopped loop's lanes (their grown buffers and FFT plans of both fused
+    lengths) serve the next start, here over another waveform and with one
+    more lane; answers stay those of the synchronous path and the oracle."""
+    from tremor_amd.capi import Context
+    with Context(precision="bf16x3", fft="auto") as ctx:
+        for run, (n, seed, lanes) in enumerate(((300_000, 14, 2), (260_000, 15, 3))):
+            x, onsets, rng = _wave(n, seed, 30)
+            reqs = _requests(x, onsets, rng, 16, 300, 8192)
+            ctx.load_waveform(x)
+            _log("serving_restart", run=run, lanes=lanes, requests=len(reqs))
+            ctx.serve_start(lanes=lanes, max_batch=4)
+            tickets = [ctx.submit(r["template"], r["kind"], r["k"], r["rho"], r["merge"]) for r in reqs]
+            res = [ctx.wait(t, timeout=120.0) for t in tickets]
+            ctx.serve_stop()
+            assert all(r is not None and r["status"] == 0 for r in res), [r and r["error"] for r in res]
+            for i, r in enumerate(reqs):
+                _log("serving_restart_alone", run=run, i=i, m=r["template"].size, kind=r["kind"])
+                assert _same((res[i]["pos"], res[i]["d"]), _alone(ctx, r)), i
+            for i in range(0, len(reqs), 4):
+                r = reqs[i]
+                _log("serving_restart_oracle", run=run, i=i, kind=r["kind"])
+                if r["kind"] == "knn":
+                    op, od, _ = oracle.scan_knn(x, [r["template"]], r["k"], r["merge"])
+                    assert _same((res[i]["pos"], res[i]["d"]), (op[0], od[0])), i
+                else:
+                    _, op, od = oracle.scan_threshold(x, [r["template"]], r["rho"], r["merge"])
+                    assert _same((res[i]["pos"], res[i]["d"]), (op, od)), i
+
+
 def test_serving_errors():
     from tremor_amd.capi import Context, TmxError
     x, onsets, rng = _wave(50_000, 13, 5)

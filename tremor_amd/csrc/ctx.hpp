@@ -15,6 +15,7 @@
 
 #include <hipfft/hipfft.h>
 
+#include <atomic>
 #include <chrono>
 #include <cstddef>
 #include <memory>
@@ -90,8 +91,13 @@ struct Lane {
     DevMem scan_ids, scan_visit;
     uint64_t chunk_cap = 0;
     uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
-    hipfftHandle tfwd = 0, inv = 0;
-    int tfwd_B = 0, inv_B = 0, inv_batch = 0;
+    // template forward plans, one per FFT length (batches alternate between
+    // the classes, and a plan takes milliseconds to make)
+    static constexpr int kPlanSlots = 4;
+    hipfftHandle tfwd[kPlanSlots] = {};
+    int tfwd_B[kPlanSlots] = {};
+    hipfftHandle inv = 0;
+    int inv_B = 0, inv_batch = 0;
     // pinned host staging of the threshold hits (grown on demand)
     void *host_keys = nullptr;
     size_t host_keys_bytes = 0;
@@ -124,11 +130,15 @@ struct Lane {
             (void)hipHostFree(host_keys);
         host_keys = nullptr;
         host_keys_bytes = 0;
-        if (tfwd)
-            (void)hipfftDestroy(tfwd);
+        for (int i = 0; i < kPlanSlots; i++) {
+            if (tfwd[i])
+                (void)hipfftDestroy(tfwd[i]);
+            tfwd[i] = 0;
+            tfwd_B[i] = 0;
+        }
         if (inv)
             (void)hipfftDestroy(inv);
-        tfwd = inv = 0;
+        inv = 0;
         if (ev0)
             (void)hipEventDestroy(ev0);
         if (ev1)
@@ -226,6 +236,13 @@ struct tmx_ctx {
     std::vector<std::unique_ptr<tmx::FftClass>> fft;
     // serving loop (null when not running)
     std::unique_ptr<tmx::Server, tmx::ServerDelete> server;
+    // lanes of stopped serving loops, kept with their grown buffers and FFT
+    // plans for the next start (guarded by spare_mu)
+    std::mutex spare_mu;
+    std::vector<std::unique_ptr<tmx::Lane>> spare_lanes;
+    // largest candidate buffer and chunk queue (entries) any lane has needed:
+    // lanes reserve at least these, so one overflow and rerun teaches all
+    std::atomic<uint64_t> cand_hint{0}, chunk_hint{0};
 
     int fail(const std::string &msg, int code = -1)
     {

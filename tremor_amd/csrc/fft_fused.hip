@@ -378,7 +378,6 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
                 continue;
             const uint64_t nwin64 = W - base < static_cast<uint64_t>(kStep) ? W - base : static_cast<uint64_t>(kStep);
             const int nwin = static_cast<int>(nwin64);
-            const int nch = (nwin + 63) / 64;
             // Thread tid owns the chunk of outputs [64 (c0 + tid), + 64): the
             // windows i = n - (m - 1) of its outputs n, clipped to the piece's
             // [0, nwin).  Its core's prefix pairs are loaded ahead of the

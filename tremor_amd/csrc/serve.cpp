@@ -9,7 +9,9 @@
 // Requests that arrive while batches run wait only for the next free lane, so
 // lanes' batches overlap on the device and the batch size follows the load.
 // A batch that fails is retried request by request, so one bad request (for
-// example a threshold with more than 2^31 hits) fails alone.
+// example a threshold with more than 2^31 hits) fails alone.  A stopped loop
+// hands its lanes back to the context, and the next start reuses them with
+// the buffers and FFT plans they have grown.
 #include "tremor_amd/tmx.h"
 
 #include <algorithm>
@@ -156,7 +158,9 @@ void lane_main(tmx_ctx *ctx, Server *s, int li)
     }
 }
 
-void stop_server(Server *s)
+// Ends the lane threads once the queue has drained; the lanes go back to the
+// context's spares with their buffers and plans.
+void stop_server(tmx_ctx *ctx, Server *s)
 {
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -169,6 +173,11 @@ void stop_server(Server *s)
         if (t.joinable())
             t.join();
     s->threads.clear();
+    {
+        std::lock_guard<std::mutex> lk(ctx->spare_mu);
+        for (auto &L : s->lanes)
+            ctx->spare_lanes.push_back(std::move(L));
+    }
     s->lanes.clear();
     std::lock_guard<std::mutex> lk(s->mu);
     s->stopped = true;
@@ -201,7 +210,7 @@ void fill_result(const Request &r, tmx_result *out)
 void server_shutdown(tmx_ctx *ctx)
 {
     if (ctx->server) {
-        stop_server(ctx->server.get());
+        stop_server(ctx, ctx->server.get());
         ctx->server.reset();
     }
 }
@@ -259,7 +268,20 @@ int tmx_server_start(tmx_ctx *ctx, const tmx_server_cfg *cfg)
     if (hipSetDevice(ctx->cfg.device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
         return ctx->fail("cannot query device memory", -2);
     s->budget = c.mem_budget > 0 ? static_cast<size_t>(c.mem_budget) : free_b / 2;
-    for (int i = 0; i < c.lanes; i++) {
+    {
+        // Spare lanes of an earlier loop on this device first.
+        std::lock_guard<std::mutex> lk(ctx->spare_mu);
+        auto &sp = ctx->spare_lanes;
+        for (size_t i = 0; i < sp.size() && static_cast<int>(s->lanes.size()) < c.lanes;) {
+            if (sp[i]->device == ctx->cfg.device) {
+                s->lanes.push_back(std::move(sp[i]));
+                sp.erase(sp.begin() + static_cast<std::ptrdiff_t>(i));
+            } else {
+                i++;
+            }
+        }
+    }
+    while (static_cast<int>(s->lanes.size()) < c.lanes) {
         auto L = std::make_unique<Lane>();
         if (L->init(ctx->cfg.device) != hipSuccess)
             return ctx->fail("cannot create a lane stream", -2);
@@ -278,7 +300,7 @@ int tmx_server_stop(tmx_ctx *ctx)
         return -1;
     if (!ctx->server)
         return ctx->fail("the serving loop is not running");
-    stop_server(ctx->server.get());
+    stop_server(ctx, ctx->server.get());
     return 0;
 }
 

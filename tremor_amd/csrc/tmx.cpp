@@ -160,12 +160,14 @@ double fft_seconds_per_window(int L)
     return per_piece / static_cast<double>(B - L + 1);
 }
 
+constexpr int kFusedChunk = 16;    // templates per fused FFT launch
+constexpr uint64_t kChunkQueueMin = 1u << 20;  // chunk records a lane's fused pass starts with (280 B each)
+constexpr int kFftMinLength = 256;  // shortest template the cost model sends to the FFT form
+
 // The early-abandoning scan (a third form, chosen once the bounds are known,
 // as the reference decides after its sampled early-abandon test,
 // Tremor.cpp:2265-2286): a fixed cost per window (statistics) plus one per
 // 32-sample block visited, the blocks measured on a sample of windows.
-constexpr int kFusedChunk = 16;    // templates per fused FFT launch
-constexpr int kFftMinLength = 256;  // shortest template the cost model sends to the FFT form
 constexpr double kScanWindow = 9.0e-12;
 constexpr double kScanBlock = 3.6e-12;
 constexpr uint64_t kScanSample = 4096;
@@ -409,6 +411,43 @@ int build_resident(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &pl
 
 // ---- per-batch work on lane L (under the shared lock) ----------------------
 
+// The lane's forward plan of kFftPieceBatch templates at length f.B, made on
+// first use and kept; slots fill in order and, when all are taken, the oldest
+// plan is replaced.
+int template_plan(Lane &L, const FftClass &f, hipfftHandle *out)
+{
+    constexpr int S = Lane::kPlanSlots;
+    int slot = -1;
+    for (int i = 0; i < S && slot < 0; i++)
+        if (L.tfwd[i] && L.tfwd_B[i] == f.B)
+            slot = i;
+    if (slot < 0) {
+        for (int i = 0; i < S && slot < 0; i++)
+            if (!L.tfwd[i])
+                slot = i;
+        if (slot < 0) {
+            (void)hipfftDestroy(L.tfwd[0]);
+            for (int i = 0; i + 1 < S; i++) {
+                L.tfwd[i] = L.tfwd[i + 1];
+                L.tfwd_B[i] = L.tfwd_B[i + 1];
+            }
+            slot = S - 1;
+            L.tfwd[slot] = 0;
+        }
+        L.tfwd_B[slot] = 0;
+        int nn = f.B;
+        if (hipfftPlanMany(&L.tfwd[slot], 1, &nn, nullptr, 1, f.B, nullptr, 1, f.bins, HIPFFT_R2C, kFftPieceBatch) !=
+            HIPFFT_SUCCESS) {
+            L.tfwd[slot] = 0;
+            return L.fail("hipFFT template plan failed", -2);
+        }
+        L.tfwd_B[slot] = f.B;
+    }
+    hipfftSetStream(L.tfwd[slot], L.stream);
+    *out = L.tfwd[slot];
+    return 0;
+}
+
 // Spectra of the batch's FFT templates [Tg, T) (reversed, zero padded).
 int fft_prepare(Lane &L, const CallPlan &plan, const FftClass &f)
 {
@@ -429,23 +468,15 @@ int fft_prepare(Lane &L, const CallPlan &plan, const FftClass &f)
     LANE_HIP(hipMemcpyAsync(L.ids.p, ids.data(), sizeof(int32_t) * F, hipMemcpyHostToDevice, L.stream), "copy ids");
     LANE_HIP(hipMemcpyAsync(L.slot.p, slot.data(), sizeof(int32_t) * plan.T, hipMemcpyHostToDevice, L.stream),
              "copy slots");
-    if (!L.tfwd || L.tfwd_B != f.B) {
-        if (L.tfwd)
-            (void)hipfftDestroy(L.tfwd);
-        L.tfwd = 0;
-        int nn = f.B;
-        if (hipfftPlanMany(&L.tfwd, 1, &nn, nullptr, 1, f.B, nullptr, 1, f.bins, HIPFFT_R2C, kFftPieceBatch) !=
-            HIPFFT_SUCCESS)
-            return L.fail("hipFFT template plan failed", -2);
-        L.tfwd_B = f.B;
-    }
-    hipfftSetStream(L.tfwd, L.stream);
+    hipfftHandle plan_r2c = 0;
+    if (int rc = template_plan(L, f, &plan_r2c))
+        return rc;
     for (int i0 = 0; i0 < F; i0 += kFftPieceBatch) {
         const int cnt = std::min(kFftPieceBatch, F - i0);
         LANE_HIP(launch_tmpl_input(L.tmpl.as<TmplDev>(), L.ids.as<int32_t>() + i0, cnt, f.B, L.qz.as<float>(),
                                    L.qin.as<float>(), L.stream),
                  "FFT template input");
-        if (hipfftExecR2C(L.tfwd, L.qin.as<hipfftReal>(),
+        if (hipfftExecR2C(plan_r2c, L.qin.as<hipfftReal>(),
                           reinterpret_cast<hipfftComplex *>(L.qspec.as<float2>() + static_cast<size_t>(i0) * f.bins)) !=
             HIPFFT_SUCCESS)
             return L.fail("hipFFT r2c of the templates failed", -2);
@@ -482,6 +513,13 @@ int upload_and_prepare(Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const 
                  "MFMA operand packing");
     }
     return f ? fft_prepare(L, plan, *f) : 0;
+}
+
+void raise_hint(std::atomic<uint64_t> &hint, uint64_t v)
+{
+    uint64_t cur = hint.load();
+    while (cur < v && !hint.compare_exchange_weak(cur, v)) {
+    }
 }
 
 int ensure_cand(Lane &L, uint64_t cap, int T)
@@ -532,12 +570,19 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
         // would keep an infinite bound.
         const uint64_t nwork = (f.P + static_cast<uint64_t>(stride) - 1) / static_cast<uint64_t>(stride);
         const uint64_t seed_need = mode == kModeSeed ? nwork * static_cast<uint64_t>(F) * 16 : 0;
-        const uint64_t want = std::max<uint64_t>(std::max<uint64_t>(L.chunk_need, seed_need), 1u << 16);
+        // Main passes: room for kChunkQueueMin records (or every chunk a
+        // launch can produce, if fewer), more once a pass has overflowed.
+        const uint64_t most = static_cast<uint64_t>(std::min(F, kFusedChunk)) * nwork *
+                              ((static_cast<uint64_t>(f.step) + 63) / 64);
+        const uint64_t want = std::max({L.chunk_need, seed_need, std::min(most, kChunkQueueMin),
+                                        ctx->chunk_hint.load(), uint64_t{1024}});
         if (want > L.chunk_cap) {
             L.chunk_cap = 0;
             LANE_HIP(L.chunk_rec.reserve(fft_chunk_rec_bytes() * want), "hipMalloc chunk queue");
             LANE_HIP(L.chunk_dt.reserve(sizeof(float) * 64 * want), "hipMalloc chunk queue");
             L.chunk_cap = want;
+            if (L.chunk_need > 0)
+                raise_hint(ctx->chunk_hint, want);
         }
         LANE_HIP(L.chunk_n.reserve(2 * sizeof(unsigned long long)), "hipMalloc chunk queue");
         LANE_HIP(launch_zero_u64(L.chunk_n.as<unsigned long long>() + 1, L.stream), "reset chunk counter");
@@ -675,9 +720,11 @@ int run_main(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
         if (*count > (1ull << 31) - 1)
             return L.fail("more than 2^31 candidate windows; raise the bound or split the call", 1);
         (*retries)++;
-        rc = ensure_cand(L, *count + (*count >> 3) + 1024, plan.T);
+        const uint64_t cap = *count + (*count >> 3) + 1024;
+        rc = ensure_cand(L, cap, plan.T);
         if (rc)
             return rc;
+        raise_hint(ctx->cand_hint, cap);
     }
 }
 
@@ -799,7 +846,8 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     int rc = upload_and_prepare(L, cfg, plan, raw_packed, f);
     if (rc)
         return rc;
-    const uint64_t def_cap = cfg.cand_capacity > 0 ? static_cast<uint64_t>(cfg.cand_capacity) : (1ull << 22);
+    const uint64_t def_cap = cfg.cand_capacity > 0 ? static_cast<uint64_t>(cfg.cand_capacity)
+                                                   : std::max<uint64_t>(1ull << 22, ctx->cand_hint.load());
     if ((rc = ensure_cand(L, def_cap, T)))
         return rc;
 
