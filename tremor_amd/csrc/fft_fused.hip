@@ -15,7 +15,8 @@
 //      Z[k] = (Y[k] + conj Y[M-k]) + i e^{i pi k / M} (Y[k] - conj Y[M-k]),
 //      z = sum_k Z[k] e^{2 pi i r k / M} gives y[2r] = Re z[r], y[2r+1] =
 //      Im z[r] (unnormalised, like a c2r of length B), computed in LDS by
-//      Stockham passes of radix 16, 16, 16, 4 (the first one in registers);
+//      Stockham passes of radix 16, 16, 16, 4 (the first one in registers;
+//      the second and third passes' twiddles from small LDS tables);
 //   3. the window dot products dt = y[m - 1 + i] / B are read from LDS by a
 //      chunk filter: for the windows of each 64 consecutive outputs, a
 //      rigorous lower bound of their reference distances from the largest dt
@@ -24,7 +25,9 @@
 //      examined window by window (filter.hpp's interval) and emit candidates.
 //
 // Nothing of size B leaves the chip: per (piece, template) HBM traffic is the
-// piece spectrum (amortised over the batch) and the candidates.
+// piece spectrum (amortised over the batch) and the candidates.  The spectra
+// of the next (piece, template) pair are loaded into registers while the
+// current pair's chunks are filtered, so their latency is hidden.
 //
 // Error bound.  The inverse transform here is a sequence of 14 radix-2 levels
 // (grouped as radix 16/16/16/4) with twiddles from an FP64-rounded table
@@ -152,23 +155,37 @@ __device__ __forceinline__ void dft16(float2 *v)
         v[r] = o[r];
 }
 
-// v[r] *= w^{r k} for r = 1..15, w = e^{2 pi i / (16 Ns)}, from the table
-// E[t] = e^{i pi t / M} with index factor F = 2M / (16 Ns): six table values
-// (w^{s k}, w^{4 q k}) and one product each.
-__device__ __forceinline__ void twiddle16(float2 *v, __amdgpu_buffer_rsrc_t E, int k, int F)
+// Twiddles of the second and third passes, from LDS: T[j] = e^{2 pi i j / 256}
+// (j < 256) and the two-level pair A[j] = e^{2 pi i j / 64}, Bf[j] =
+// e^{2 pi i j / 4096} (j < 64), all entries of the FP64-rounded table E.
+constexpr int kTwT = 0, kTwA = 256, kTwB = 320, kTw = 384;
+
+// E[t] = e^{i pi t / M} has 2M entries; T, A and Bf are its entries at
+// strides 2M / 256, 2M / 64 and 2M / 4096.
+__device__ __forceinline__ void fill_twiddles(float2 *tw, __amdgpu_buffer_rsrc_t Er, int M, int tid, int nthreads)
 {
-    const uint32_t o = static_cast<uint32_t>(k * F) * 8u;
-    const float2 p1 = bload(E, o, 0), p2 = bload(E, 2 * o, 0), p3 = bload(E, 3 * o, 0);
-    v[1] = cmul(v[1], p1);
-    v[2] = cmul(v[2], p2);
-    v[3] = cmul(v[3], p3);
+    for (int j = tid; j < kTw; j += nthreads) {
+        const int t = j < kTwA ? j * (2 * M / 256) : j < kTwB ? (j - kTwA) * (2 * M / 64) : (j - kTwB) * (2 * M / 4096);
+        tw[j] = bload(Er, static_cast<uint32_t>(t) * 8u, 0);
+    }
+}
+
+// Pass 2 (Ns = 16, w = e^{2 pi i / 256}): v[r] *= w^{r k} = T[r k], r k <= 225.
+__device__ __forceinline__ void twiddle_pass2(float2 *v, const float2 *tw, int k)
+{
 #pragma unroll
-    for (int q = 1; q < 4; q++) {
-        const float2 cq = bload(E, 4 * q * o, 0);
-        v[4 * q] = cmul(v[4 * q], cq);
-        v[4 * q + 1] = cmul(v[4 * q + 1], cmul(cq, p1));
-        v[4 * q + 2] = cmul(v[4 * q + 2], cmul(cq, p2));
-        v[4 * q + 3] = cmul(v[4 * q + 3], cmul(cq, p3));
+    for (int r = 1; r < 16; r++)
+        v[r] = cmul(v[r], tw[kTwT + r * k]);
+}
+
+// Pass 3 (Ns = 256, w = e^{2 pi i / 4096}): v[r] *= w^{r k} = A[r k / 64]
+// Bf[r k mod 64] (one product of two table entries), r k <= 3825.
+__device__ __forceinline__ void twiddle_pass3(float2 *v, const float2 *tw, int k)
+{
+#pragma unroll
+    for (int r = 1; r < 16; r++) {
+        const int j = r * k;
+        v[r] = cmul(v[r], cmul(tw[kTwA + (j >> 6)], tw[kTwB + (j & 63)]));
     }
 }
 
@@ -221,22 +238,22 @@ __device__ double chunk_dist_lo(double2 pa, double2 pb, uint64_t a, uint64_t b, 
 // z[r] = y[2r] + i y[2r+1], the unnormalised circular correlation of the
 // piece with the template (all threads synchronised).
 template <int LOG2M>
-__device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, float2 Xn, __amdgpu_buffer_rsrc_t Hr,
-                                                __amdgpu_buffer_rsrc_t Er)
+__device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, float2 Xn, const float2 *H, float2 Hn,
+                                                const float2 *tw, __amdgpu_buffer_rsrc_t Er)
 {
     using G = Geom<LOG2M>;
     constexpr int kM = G::kM, kT = G::kT;
     const int tid = opaque(threadIdx.x);
     const uint32_t toff = static_cast<uint32_t>(tid) * 8u;
-    // 1. Y = X .* H into LDS (X: bins tid + kT r, and M for thread 0).
+    // 1. Y = X .* H into LDS (X, H: bins tid + kT r, and M for thread 0).
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        v[r] = cmul(X[r], bload(Hr, toff, 8u * kT * r));
+        v[r] = cmul(X[r], H[r]);
         data[pad(tid + kT * r)] = v[r];
     }
     if (tid == 0)
-        data[pad(kM)] = cmul(Xn, bload(Hr, 0, 8u * kM));
+        data[pad(kM)] = cmul(Xn, Hn);
     __syncthreads();
     // 2a. Z formation (registers) = the first pass's inputs; the
     // factor e^{i pi k / M} for k = tid + kT r is E[tid] e^{i pi r / 16}.
@@ -264,7 +281,7 @@ __device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, f
 #pragma unroll
     for (int r = 0; r < 16; r++)
         v[r] = data[pad(tid + kT * r)];
-    twiddle16(v, Er, tid & 15, 2 * kM / 256);
+    twiddle_pass2(v, tw, tid & 15);
     dft16(v);
     __syncthreads();
 #pragma unroll
@@ -275,7 +292,7 @@ __device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, f
 #pragma unroll
     for (int r = 0; r < 16; r++)
         v[r] = data[pad(tid + kT * r)];
-    twiddle16(v, Er, tid & 255, 2 * kM / 4096);
+    twiddle_pass3(v, tw, tid & 255);
     dft16(v);
     __syncthreads();
 #pragma unroll
@@ -347,22 +364,33 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
     constexpr double u = 5.9604644775390625e-08;
     const uint64_t nwork = (P + stride - 1) / stride;
     const __amdgpu_buffer_rsrc_t Er = rsrc(E, 2u * kM * 8u);
+    __shared__ float2 tw[kTw];
+    fill_twiddles(tw, Er, kM, tid, kT);
+    __syncthreads();
+
+    // Spectrum operands (bins tid + kT r, and M for thread 0) of the pair
+    // (piece, template) about to be transformed.  The first pair's are loaded
+    // here; every later pair's while the previous pair's chunks are filtered,
+    // so their latency hides behind the filter.
+    float2 X[16], H[16];
+    float2 Xn = make_float2(0.0f, 0.0f), Hn = make_float2(0.0f, 0.0f);
+    const auto load_spec = [&](float2 *dst, float2 &dst_n, const float2 *src) {
+        const __amdgpu_buffer_rsrc_t r = rsrc(src, bins * 8u);
+        const uint32_t toff = static_cast<uint32_t>(opaque(tid)) * 8u;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            dst[k] = bload(r, toff, 8u * kT * k);
+        if (tid == 0)
+            dst_n = bload(r, 0, 8u * kM);
+    };
+    if (blockIdx.x < nwork) {
+        load_spec(X, Xn, spectra + static_cast<uint64_t>(blockIdx.x) * stride * static_cast<uint64_t>(bins));
+        load_spec(H, Hn, qspec);
+    }
 
     for (uint64_t it = blockIdx.x; it < nwork; it += gridDim.x) {
         const uint64_t p = it * static_cast<uint64_t>(stride);
         const uint64_t base = p * static_cast<uint64_t>(kStep);
-        // The piece spectrum, once per piece, in registers for every template.
-        float2 X[16];
-        float2 Xn = make_float2(0.0f, 0.0f);
-        {
-            const __amdgpu_buffer_rsrc_t Xr = rsrc(spectra + p * static_cast<uint64_t>(bins), bins * 8u);
-            const uint32_t toff = static_cast<uint32_t>(opaque(tid)) * 8u;
-#pragma unroll
-            for (int r = 0; r < 16; r++)
-                X[r] = bload(Xr, toff, 8u * kT * r);
-            if (tid == 0)
-                Xn = bload(Xr, 0, 8u * kM);
-        }
         const float4 pi = pinfo[p];
         const float c = pi.x;
         const double amax = static_cast<double>(pi.w) * (1.0 + 1e-6);
@@ -371,12 +399,10 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
         for (int ti = 0; ti < F; ti++) {
             const int t = t0 + ti;
             const int m = tmpl[t].m;
-            if (static_cast<uint64_t>(m) > w.n || tmpl[t].ea)
-                continue;
-            const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
-            if (base >= W)
-                continue;
-            const uint64_t nwin64 = W - base < static_cast<uint64_t>(kStep) ? W - base : static_cast<uint64_t>(kStep);
+            const uint64_t W = static_cast<uint64_t>(m) <= w.n ? w.n - static_cast<uint64_t>(m) + 1 : 0;
+            const bool active = W > 0 && !tmpl[t].ea && base < W;
+            const uint64_t nwin64 = !active ? 0 : W - base < static_cast<uint64_t>(kStep) ? W - base
+                                                                                         : static_cast<uint64_t>(kStep);
             const int nwin = static_cast<int>(nwin64);
             // Thread tid owns the chunk of outputs [64 (c0 + tid), + 64): the
             // windows i = n - (m - 1) of its outputs n, clipped to the piece's
@@ -390,7 +416,7 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
                 n_end = nlo + 64 < m - 1 + nwin ? nlo + 64 : m - 1 + nwin;
             };
             double2 pa = make_double2(0.0, 0.0), pb = make_double2(0.0, 0.0);
-            {
+            if (active) {
                 int nf, ne;
                 chunk(nf, ne);
                 const int cnt = nf < ne ? ne - nf : 0;
@@ -399,10 +425,16 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
                     pa = w.p[base + static_cast<uint64_t>(i_first + cnt - 1)];
                     pb = w.p[base + static_cast<uint64_t>(i_first + m)];
                 }
+                correlate_piece<LOG2M>(data, X, Xn, H, Hn, tw, Er);
             }
-            const __amdgpu_buffer_rsrc_t Hr = rsrc(qspec + static_cast<uint64_t>(ti) * static_cast<uint64_t>(bins),
-                                                   bins * 8u);
-            correlate_piece<LOG2M>(data, X, Xn, Hr, Er);
+            // The next pair's operands: the next template's spectrum (the
+            // first one's after the last) and, after the last template, the
+            // next piece's.
+            load_spec(H, Hn, qspec + static_cast<uint64_t>(ti + 1 < F ? ti + 1 : 0) * static_cast<uint64_t>(bins));
+            if (ti + 1 == F && it + gridDim.x < nwork)
+                load_spec(X, Xn, spectra + (it + gridDim.x) * static_cast<uint64_t>(stride) * static_cast<uint64_t>(bins));
+            if (!active)
+                continue;
 
             int n_first, n_end;
             chunk(n_first, n_end);

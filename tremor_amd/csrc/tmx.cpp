@@ -150,9 +150,9 @@ double fft_seconds_per_window(int L)
 {
     const int B = fft_block_for(L);
     if (B == (1 << 14))
-        return 3.4e-12;
+        return 3.05e-12;
     if (B == (1 << 15))
-        return 4.5e-12;
+        return 3.4e-12;
     double lg = 0.0;
     while ((1 << static_cast<int>(lg)) < B)
         lg += 1.0;
