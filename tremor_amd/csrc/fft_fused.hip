@@ -64,6 +64,10 @@ struct Geom {
 
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 
+// Outputs per chunk of the filter (two padded LDS rows), and dot-product
+// slots per queued chunk record.
+constexpr int kChunkW = 64;
+
 // A copy of x the compiler cannot see through: index arithmetic derived from
 // it is redone where it is used instead of being hoisted out of the loops
 // (and spilled: the transform needs nearly every register).
@@ -189,6 +193,32 @@ __device__ __forceinline__ void twiddle_pass3(float2 *v, const float2 *tw, int k
     }
 }
 
+// Upper bound on sqrt(v), v >= 0, without an FP64 square root: the FP32 one
+// (V_SQRT_F32, 1 ulp) of v rounded to float, inflated by 2^-20 to cover both
+// roundings, plus 1e-18 for values below the float range; +inf from 1e37.
+__device__ __forceinline__ double sqrt_up(double v)
+{
+    if (!(v < 1e37))
+        return INFINITY;
+    return static_cast<double>(__builtin_amdgcn_sqrtf(static_cast<float>(v))) * (1.0 + 0x1p-20) + 1e-18;
+}
+
+// Per-template constants of chunk_dist_lo: 1 / m, an upper bound on sqrt(m),
+// the upper bound of gamma_{m/16 + 36} (1 / (1 - x) <= 1 + 2x for x <= 1/2),
+// and 1 / (m - 63), the core length of a full chunk.
+struct ChunkConst {
+    double invm, rm_up, gam, inv_full;
+};
+
+__device__ __forceinline__ ChunkConst chunk_const(int m)
+{
+    constexpr double u = 5.9604644775390625e-08;
+    const double md = static_cast<double>(m);
+    const double neff = floor(md / 16.0) + 36.0;
+    return ChunkConst{1.0 / md, sqrt_up(md), neff * u * (1.0 + 2.0 * neff * u) * (1.0 + 1e-12),
+                      m > kChunkW ? 1.0 / static_cast<double>(m - (kChunkW - 1)) : 0.0};
+}
+
 // Lower bound on the reference distance of every window j in [j0, j0 + cnt)
 // of a template of length m, given that each window's
 // N_j = dt_j - (mu_j - c) S is at most nmax: with the windows' common core
@@ -199,38 +229,44 @@ __device__ __forceinline__ void twiddle_pass3(float2 *v, const float2 *tw, int k
 // E_ref(dlo) (d - E_ref(d) increases wherever the result can exceed 0), with
 // E_ref as in filter.hpp and |mu_j| / sigma_j <= mu_abs / sigma_lo.
 // pa = P[a], pb = P[b] are the prefix pairs at the core's ends a = j0 + cnt - 1
-// and b = j0 + m (Lc = b - a).  -inf when the core cannot bound sigma.
-__device__ double chunk_dist_lo(double2 pa, double2 pb, uint64_t a, uint64_t b, int Lc, int m, double Q2,
-                                double nmax, double mu_abs)
+// and b = j0 + m (Lc = b - a, invL = the rounded 1 / Lc).  The differences and
+// V_C are FP64; the square roots and 1 / sigma_lo are FP32 estimates made
+// one-sided (sigma_lo rounded down, the error terms up), so the result is
+// still a lower bound.  -inf when the core cannot bound sigma.
+__device__ double chunk_dist_lo(double2 pa, double2 pb, uint64_t a, uint64_t b, int Lc, double invL,
+                                const ChunkConst &K, double Q2, int m, double nmax, double mu_abs)
 {
     constexpr double u = 5.9604644775390625e-08;
     constexpr double u64 = 1.1102230246251565e-16;
     if (Lc < 2 || !(nmax <= 1e300))
         return -INFINITY;  // no core, or a non-finite dot product: examine every window
     const double S1 = pb.x - pa.x, S2 = pb.y - pa.y;
-    const double eS1 = kPrefixGamma * (sqrt(static_cast<double>(b) * fabs(pb.y)) +
-                                       sqrt(static_cast<double>(a) * fabs(pa.y))) +
+    const double eS1 = kPrefixGamma * (sqrt_up(static_cast<double>(b) * fabs(pb.y)) +
+                                       sqrt_up(static_cast<double>(a) * fabs(pa.y))) +
                        4.0 * u64 * fabs(S1);
     const double eS2 = kPrefixGamma * (fabs(pb.y) + fabs(pa.y)) + 4.0 * u64 * fabs(S2);
-    const double L = static_cast<double>(Lc);
-    const double V = S2 - S1 * S1 / L;
-    const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) / L + 8.0 * u64 * (fabs(S2) + S1 * S1 / L);
+    // S1^2 / Lc as S1^2 invL: one rounding more than a division (10 u64).
+    const double S1L = S1 * S1 * invL;
+    const double V = S2 - S1L;
+    const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) * invL * (1.0 + 4.0 * u64) + 10.0 * u64 * (fabs(S2) + S1L);
     const double vlo = V - eV;
     if (!(vlo > 0.0))
         return -INFINITY;
+    // sigma_lo <= sqrt(vlo / m) and isig >= 1 / sigma_lo (V_RCP_F32, 1 ulp).
+    const float vm = static_cast<float>(vlo * K.invm);
+    if (!(vm > 1e-30f))
+        return -INFINITY;
+    const float sig_f = __builtin_amdgcn_sqrtf(fminf(vm, 1e38f)) * (1.0f - 0x1p-20f);
+    const double isig = static_cast<double>(__builtin_amdgcn_rcpf(sig_f)) * (1.0 + 0x1p-20);
+    const double gmax = nmax > 0.0 ? nmax * isig * (1.0 + 2.0 * u64) : 0.0;
     const double md = static_cast<double>(m);
-    const double sig_lo = sqrt(vlo / md) * (1.0 - 4.0 * u64);
-    const double gmax = nmax > 0.0 ? nmax / sig_lo * (1.0 + 4.0 * u64) : 0.0;
     const double base = Q2 + md;
     const double dlo = base - 2.0 * gmax - 8.0 * u64 * (base + 2.0 * gmax);
     if (!(dlo > 0.0))
         return dlo;
-    const double sq = sqrt(dlo);
-    const double rm = sqrt(md);
-    const double D1 = (u * 1.001) * (3.05 * rm + 1.03 * rm * mu_abs / sig_lo + sq);
-    const double neff = floor(md / 16.0) + 36.0;
-    const double gam = neff * u / (1.0 - neff * u);
-    const double Eref = (D1 * (2.0 * sq + D1) + gam * (sq + D1) * (sq + D1)) * 1.01;
+    const double sq = sqrt_up(dlo);
+    const double D1 = (u * 1.001) * (3.05 * K.rm_up + 1.03 * K.rm_up * mu_abs * isig + sq);
+    const double Eref = (D1 * (2.0 * sq + D1) + K.gam * (sq + D1) * (sq + D1)) * 1.01;
     return dlo - Eref;
 }
 
@@ -337,7 +373,7 @@ __device__ __forceinline__ void correlate_piece(float2 *data, const float2 *X, f
 
 
 // A chunk that may hold a window within the bound (main mode: its windows'
-// dot products follow in the dt queue, 64 slots per record), or a seed
+// dot products follow in the dt queue, kChunkW slots per record), or a seed
 // window (cnt = 1: the chunk's most promising window).
 struct ChunkRec {
     uint32_t t;   // template
@@ -411,9 +447,9 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
             // (The chunk's bounds are recomputed after the transform rather
             // than held across it.)
             const auto chunk = [&](int &n_first, int &n_end) {
-                const int nlo = 64 * (((m - 1) >> 6) + opaque(tid));
+                const int nlo = kChunkW * (((m - 1) >> 6) + opaque(tid));
                 n_first = nlo > m - 1 ? nlo : m - 1;
-                n_end = nlo + 64 < m - 1 + nwin ? nlo + 64 : m - 1 + nwin;
+                n_end = nlo + kChunkW < m - 1 + nwin ? nlo + kChunkW : m - 1 + nwin;
             };
             double2 pa = make_double2(0.0, 0.0), pb = make_double2(0.0, 0.0);
             if (active) {
@@ -450,7 +486,7 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
             // largest output, and a sum that is non-finite iff an output is.
             const float invB = 1.0f / static_cast<float>(kB);
             float vmax = -INFINITY, vsum = 0.0f;
-            if (cnt == 64) {
+            if (cnt == kChunkW) {
                 // Two padded rows of 32 floats; half the lanes take them in
                 // the other order (fewer LDS bank conflicts).
                 const int f0 = 68 * (c0 + tid);
@@ -488,15 +524,17 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
                     const double nslack = edot + amax * fabs(S);
                     const double mu_abs = fabs(static_cast<double>(c)) + amax;
                     const double bound = static_cast<double>(tmpl[t].bound);
+                    const ChunkConst K = chunk_const(m);
+                    const double invL = cnt == kChunkW ? K.inv_full : 1.0 / static_cast<double>(Lc);
                     const double lo = nonfinite ? -INFINITY
-                                                : chunk_dist_lo(pa, pb, ca, cb, Lc, m, Q2,
+                                                : chunk_dist_lo(pa, pb, ca, cb, Lc, invL, K, Q2, m,
                                                                 static_cast<double>(dmax) + nslack, mu_abs);
                     if (!(lo > bound)) {
                         const unsigned long long slot = atomicAdd(nrec, 1ULL);
                         if (slot < rec_cap) {
                             rec[slot] = ChunkRec{static_cast<uint32_t>(t), static_cast<uint32_t>(base + i_first), cnt,
                                                  c, round_up_f(eabs), 0u};
-                            float *dst = rec_dt + slot * 64;
+                            float *dst = rec_dt + slot * kChunkW;
                             for (int l = 0; l < cnt; l++) {
                                 const int n = n_first + l;
                                 dst[l] = fdata[n + 2 * (n >> 5)] * invB;
@@ -534,7 +572,7 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
                     if (slot < rec_cap) {
                         rec[slot] = ChunkRec{static_cast<uint32_t>(t),
                                              static_cast<uint32_t>(base + i_first + arg), 1, c, round_up_f(eabs), 0u};
-                        rec_dt[slot * 64] = dmax;
+                        rec_dt[slot * kChunkW] = dmax;
                     }
                 }
             }
@@ -567,7 +605,7 @@ __global__ __launch_bounds__(256) void k_chunk_windows(Waveform w, const TmplDev
         float est = 0.0f;
         double ilo = 0.0, ihi = 0.0;
         if (lane < R.cnt && w.x[j] != kMissing) {
-            const double dt = static_cast<double>(rec_dt[r * 64 + static_cast<uint64_t>(lane)]);
+            const double dt = static_cast<double>(rec_dt[r * kChunkW + static_cast<uint64_t>(lane)]);
             const DistInterval iv = dist_interval(w, j, m, Q2, S, R.c, dt, eta, static_cast<double>(R.eabs));
             if (mode == kModeMain) {
                 emit = !(iv.lo > static_cast<double>(tmpl[t].bound));  // NaN (undecidable): a candidate
@@ -612,8 +650,9 @@ __global__ void k_note_max(unsigned long long *nrec)
 bool fft_fused_supports(int B) { return B == Geom<13>::kB || B == Geom<14>::kB; }
 
 size_t fft_chunk_rec_bytes() { return sizeof(ChunkRec); }
+int fft_chunk_width() { return kChunkW; }
 
-// rec / rec_dt: queue of rec_cap records (rec_dt: 64 floats each); nrec[0]
+// rec / rec_dt: queue of rec_cap records (rec_dt: kChunkW floats each); nrec[0]
 // gets the number the launch produced, nrec[1] the largest such number since
 // it was last cleared (more than rec_cap: the pass must be rerun with a larger
 // queue in main mode; in seed mode the entries kept suffice).

@@ -57,6 +57,7 @@ hipError_t launch_piece_input(const float *x, uint64_t n, int step, int B, uint6
 // fft_fused.hip
 bool fft_fused_supports(int B);
 size_t fft_chunk_rec_bytes();
+int fft_chunk_width();  // windows per chunk record of the fused pass
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
                             const float2 *table, uint64_t P, int stride, double kfft, int mode, void *rec,
@@ -572,14 +573,14 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
         const uint64_t seed_need = mode == kModeSeed ? nwork * static_cast<uint64_t>(F) * 16 : 0;
         // Main passes: room for kChunkQueueMin records (or every chunk a
         // launch can produce, if fewer), more once a pass has overflowed.
-        const uint64_t most = static_cast<uint64_t>(std::min(F, kFusedChunk)) * nwork *
-                              ((static_cast<uint64_t>(f.step) + 63) / 64);
+        const uint64_t cw = static_cast<uint64_t>(fft_chunk_width());
+        const uint64_t most = static_cast<uint64_t>(std::min(F, kFusedChunk)) * nwork * ((f.step + cw - 1) / cw + 1);
         const uint64_t want = std::max({L.chunk_need, seed_need, std::min(most, kChunkQueueMin),
                                         ctx->chunk_hint.load(), uint64_t{1024}});
         if (want > L.chunk_cap) {
             L.chunk_cap = 0;
             LANE_HIP(L.chunk_rec.reserve(fft_chunk_rec_bytes() * want), "hipMalloc chunk queue");
-            LANE_HIP(L.chunk_dt.reserve(sizeof(float) * 64 * want), "hipMalloc chunk queue");
+            LANE_HIP(L.chunk_dt.reserve(sizeof(float) * fft_chunk_width() * want), "hipMalloc chunk queue");
             L.chunk_cap = want;
             if (L.chunk_need > 0)
                 raise_hint(ctx->chunk_hint, want);
