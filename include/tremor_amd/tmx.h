@@ -76,6 +76,9 @@ typedef struct {
     int32_t verbose;         /* 1 = print per-phase timings to stderr */
     int32_t fft_mode;        /* enum tmx_fft_mode; ignored for TMX_PREC_FP32 (plain scan) */
     int32_t scan_mode;       /* enum tmx_scan_mode */
+    int32_t verify_seq;      /* 1 = the exact recomputation takes every window's statistics by the
+                                sequential sums (tests, diagnostics); 0 = default (a rigorous
+                                parallel shortcut, sequential only where it cannot decide) */
 } tmx_cfg;
 
 typedef struct {

@@ -343,3 +343,34 @@ def test_threshold_at_scale_shape(oracle, rho):
     tt, tp, td = oracle.scan_threshold(w.x, w.templates, rho, 0)
     assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
     assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_verify_sequential_statistics(oracle):
+    """Long-template candidates take their window statistics from a parallel
+    FP64 sum with rigorous bounds on the reference's sequential sum (the
+    worst case, then the partial-sum bound), falling back to the sequential
+    sums by a wavefront; band-passed noise has window means near 0, where
+    the fallbacks happen.  Every window by the sequential sums (verify_seq)
+    must give the same answers, and both the oracle's."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(300_000, 43, events=25)
+    tm = synth.cut_templates(x, onsets, [520, 1500, 3000, 6000], rng)
+    x[rng.integers(0, x.size, size=40)] = synth.MISSING
+    with Context(precision="bf16x3") as a, Context(precision="bf16x3", verify_seq=True) as b:
+        a.load_waveform(x)
+        b.load_waveform(x)
+        _log("verify-seq-knn", k=50)
+        pa, da, _ = a.knn(tm, 50, 0)
+        pb, db, st = b.knn(tm, 50, 0)
+        print("  stats", st, flush=True)
+        assert np.array_equal(pa, pb) and np.array_equal(da.view(np.uint32), db.view(np.uint32))
+        po, do, _ = oracle.scan_knn(x, tm, 50, 0)
+        assert np.array_equal(pa, po) and np.array_equal(da.view(np.uint32), do.view(np.uint32))
+        _log("verify-seq-thr", rho=0.55)
+        ra = a.threshold(tm, 0.55, 0)
+        rb = b.threshold(tm, 0.55, 0)
+        assert np.array_equal(ra.pos, rb.pos) and np.array_equal(ra.d.view(np.uint32), rb.d.view(np.uint32))
+        tt, tp, td = oracle.scan_threshold(x, tm, 0.55, 0)
+        assert np.array_equal(ra.tmpl, tt) and np.array_equal(ra.pos, tp)
+        assert np.array_equal(ra.d.view(np.uint32), td.view(np.uint32))

@@ -37,7 +37,7 @@ REQ_THRESHOLD = 1
 class Cfg(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32), ("cand_capacity", ctypes.c_int64),
                 ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32), ("fft_mode", ctypes.c_int32),
-                ("scan_mode", ctypes.c_int32)]
+                ("scan_mode", ctypes.c_int32), ("verify_seq", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -161,7 +161,8 @@ class Context:
     """One device context: a resident waveform and search calls on it."""
 
     def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
-                 seed_target: int = 0, verbose: bool = False, fft: str = "auto", scan: str = "auto"):
+                 seed_target: int = 0, verbose: bool = False, fft: str = "auto", scan: str = "auto",
+                 verify_seq: bool = False):
         L = lib()
         cfg = Cfg()
         L.tmx_cfg_default(ctypes.byref(cfg))
@@ -172,6 +173,7 @@ class Context:
         cfg.verbose = 1 if verbose else 0
         cfg.fft_mode = FFT_MODES[fft]
         cfg.scan_mode = SCAN_MODES[scan]
+        cfg.verify_seq = 1 if verify_seq else 0
         h = ctypes.c_void_p()
         rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:

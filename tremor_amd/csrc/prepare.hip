@@ -14,6 +14,7 @@ namespace {
 
 constexpr int kPrepThreads = 256;
 constexpr int kMaxBlocks = 2048;  // templates up to 65536 samples
+constexpr int kStage = 4096;      // samples staged per chunk of the sequential sums
 
 #pragma clang fp contract(off)
 __global__ __launch_bounds__(kPrepThreads) void k_prepare(const float *__restrict__ raw_all, TmplDev *tmpl,
@@ -21,6 +22,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_prepare(const float *__restric
 {
     __shared__ double s_mean, s_sd;
     __shared__ float energy[kMaxBlocks];
+    __shared__ float stage[kStage];
     __shared__ double red_a[kPrepThreads], red_b[kPrepThreads];
     const int t = blockIdx.x;
     const int tid = threadIdx.x;
@@ -29,13 +31,26 @@ __global__ __launch_bounds__(kPrepThreads) void k_prepare(const float *__restric
     float *q = qz_all + tmpl[t].q_off;
     int32_t *order = order_all + tmpl[t].order_off;
 
-    if (tid == 0) {
-        double sum = 0.0, sum_sq = 0.0;
-        for (int i = 0; i < m; i++) {
-            const double v = static_cast<double>(raw[i]);
-            sum += v;
-            sum_sq = __builtin_fma(v, v, sum_sq);
+    // Sequential FP64 sums (their order fixes the rounding), over chunks of
+    // the samples staged in LDS by the whole workgroup, so the one summing
+    // thread never waits on a global load per sample.
+    double sum = 0.0, sum_sq = 0.0;
+    for (int c0 = 0; c0 < m; c0 += kStage) {
+        const int len = m - c0 < kStage ? m - c0 : kStage;
+        __syncthreads();  // the previous chunk has been summed
+        for (int i = tid; i < len; i += kPrepThreads)
+            stage[i] = raw[c0 + i];
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll 8
+            for (int i = 0; i < len; i++) {
+                const double v = static_cast<double>(stage[i]);
+                sum += v;
+                sum_sq = __builtin_fma(v, v, sum_sq);
+            }
         }
+    }
+    if (tid == 0) {
         const double mean = sum / static_cast<double>(m);
         double var = __builtin_fma(-mean, mean, sum_sq / static_cast<double>(m));
         if (var < 0.0)

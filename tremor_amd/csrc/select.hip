@@ -298,12 +298,19 @@ __global__ __launch_bounds__(64) void k_knn_select(const uint64_t *__restrict__ 
 }  // namespace
 
 // Groups the entries satisfying `pred` by template into `keys` (sorted within
-// each template); offsets has T + 1 slots.  Small inputs: a scatter into the
-// template segments and a segmented sort.  Large inputs: the pairs (key,
-// template) compacted behind sentinels and sorted by two stable device-wide
-// radix sorts, by key and then by template (a segmented sort gives one
-// workgroup to each segment, which a single large segment would serialise).
+// each template); offsets has T + 1 slots.  Many small segments: a scatter
+// into the template segments and a segmented sort.  Otherwise: the pairs
+// (key, template) compacted behind sentinels and sorted by two stable
+// device-wide radix sorts, by key and then by template (one sort for a single
+// template).  A segmented sort gives one workgroup to each segment, which
+// serialises a large one (22K keys of one template: 0.7 ms).
 constexpr uint64_t kSegmentedMax = 1u << 16;
+constexpr uint64_t kSegmentedAvg = 1024;  // entries per template, on average
+
+static bool use_segmented(uint64_t count, int T)
+{
+    return count <= kSegmentedMax && count <= kSegmentedAvg * static_cast<uint64_t>(T);
+}
 
 static int template_bits(int T)
 {
@@ -328,7 +335,7 @@ hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count,
     if (count == 0)
         return hipGetLastError();
     size_t bytes = sort_tmp_bytes;
-    if (count <= kSegmentedMax) {
+    if (use_segmented(count, T)) {
         hipLaunchKernelGGL(k_group_scatter, dim3(grid), dim3(kThreads), 0, stream, cand, val, count, tmpl, pred,
                            keykind, offsets, cursor, keys_tmp);
         const hipError_t err = hipGetLastError();
@@ -349,7 +356,7 @@ hipError_t launch_group_sort(const Cand *cand, const float *val, uint64_t count,
     // (keys -> keys_tmp), then back into keys.
     err = hipcub::DeviceRadixSort::SortPairs(sort_tmp, bytes, keys_tmp, keys, tv_a, tv_b, static_cast<int>(count), 0,
                                              64, stream);
-    if (err != hipSuccess)
+    if (err != hipSuccess || T == 1)
         return err;
     bytes = sort_tmp_bytes;
     err = hipcub::DeviceRadixSort::SortPairs(sort_tmp, bytes, tv_b, tv_a, keys, keys_tmp, static_cast<int>(count), 0,
@@ -367,7 +374,7 @@ size_t group_sort_tmp_bytes(uint64_t count, int T)
                                                      static_cast<int>(count < kSegmentedMax ? count : kSegmentedMax),
                                                      T, static_cast<const uint32_t *>(nullptr),
                                                      static_cast<const uint32_t *>(nullptr), 0, 64);
-    if (count > kSegmentedMax) {
+    if (!use_segmented(count, T)) {
         (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<const uint64_t *>(nullptr),
                                                  static_cast<uint64_t *>(nullptr),
                                                  static_cast<const uint32_t *>(nullptr),

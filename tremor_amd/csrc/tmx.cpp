@@ -748,10 +748,10 @@ int group_sort(Lane &L, const float *val, uint64_t count, int T, int pred, int k
     return 0;
 }
 
-int verify_and_err(tmx_ctx *ctx, Lane &L, uint64_t count, tmx_stats *st)
+int verify_and_err(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, uint64_t count, tmx_stats *st)
 {
     LANE_HIP(launch_verify(ctx->x.as<float>(), ctx->n, L.tmpl.as<TmplDev>(), L.qz.as<float>(), L.order.as<int32_t>(),
-                           L.cand.as<Cand>(), count, L.cand_d.as<float>(), L.stream),
+                           L.cand.as<Cand>(), count, L.cand_d.as<float>(), cfg.verify_seq != 0, L.stream),
              "verify kernel");
     LANE_HIP(launch_rho_err(L.cand.as<Cand>(), L.cand_val.as<float>(), L.cand_d.as<float>(), count,
                             L.tmpl.as<TmplDev>(), L.errbits.as<unsigned int>(), L.stream),
@@ -919,7 +919,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
         ncand = kept;
     }
     st.verified = static_cast<int64_t>(ncand);
-    if ((rc = verify_and_err(ctx, L, ncand, &st)))
+    if ((rc = verify_and_err(ctx, L, cfg, ncand, &st)))
         return rc;
     const double t4 = now_s();
     st.seconds_verify = t4 - t3;
@@ -1082,6 +1082,7 @@ int tmx_cfg_default(tmx_cfg *cfg)
     cfg->verbose = 0;
     cfg->fft_mode = TMX_FFT_AUTO;
     cfg->scan_mode = TMX_SCAN_AUTO;
+    cfg->verify_seq = 0;
     return 0;
 }
 

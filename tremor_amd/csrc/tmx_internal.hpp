@@ -167,7 +167,7 @@ constexpr int kScanWindows = 4096;  // windows per workgroup of the FP32 scan
 
 // Exact recomputation of candidates in the reference's order.
 hipError_t launch_verify(const float *x, uint64_t n, const TmplDev *tmpl, const float *qz, const int32_t *order,
-                         const Cand *cand, uint64_t count, float *d_out, hipStream_t stream);
+                         const Cand *cand, uint64_t count, float *d_out, bool force_seq, hipStream_t stream);
 
 // Exact reference stats of windows [first, first + count) for length m.
 hipError_t launch_window_stats(const float *x, int m, uint64_t first, uint64_t count, float *mean, float *sd,

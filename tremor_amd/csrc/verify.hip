@@ -14,10 +14,11 @@
 // samples and more): one wavefront per candidate.  The 16 FMA chains of the
 // distance then run on 16 lanes in the reference's order (bit-identical), and
 // the statistics come from a 64-lane FP64 sum with a rigorous bound on its
-// distance from the reference's sequential sum: when the whole interval of
-// possible reference values rounds to one float, that float is the
-// reference's; otherwise (values within ~1e-12 relative of a float rounding
-// boundary, about 1 window in 10^5) one lane runs the sequential code.
+// distance from the reference's sequential sum (first the worst case
+// gamma_{m-1} sum |v|, then, where that cannot decide, a bound from the
+// partial sums): when the whole interval of possible reference values rounds
+// to one float, that float is the reference's; otherwise the wavefront runs
+// the sequential sums themselves (wave_exact_stats).
 #include "tmx_internal.hpp"
 
 namespace tmx {
@@ -96,6 +97,56 @@ __device__ inline float exact_dist(const float *q, const float *r, int m, float 
 
 constexpr int kWaveMin = 512;
 
+// exact_stats by a wavefront, bit-identical (the same operations in the same
+// order on every lane): coalesced loads of 8 x 64 samples, then the
+// sequential FP64 sums over them, each sample read from its lane.  No lane
+// waits on a global load per sample.
+__device__ void wave_exact_stats(const float *w, int m, float &mean_f, float &sd_f)
+{
+    constexpr int kU = 8;
+    const int lane = threadIdx.x & 63;
+    const double md = static_cast<double>(m);
+    double sum = 0.0;
+    for (int c0 = 0; c0 < m; c0 += 64 * kU) {
+        float v[kU];
+#pragma unroll
+        for (int k = 0; k < kU; k++) {
+            const int d = c0 + 64 * k + lane;
+            v[k] = d < m ? w[d] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kU; k++) {
+            const int cnt = m - (c0 + 64 * k) < 64 ? m - (c0 + 64 * k) : 64;
+            for (int l = 0; l < cnt; l++) {
+                const float x = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v[k]), l));
+                sum += x == kMissing ? 0.0 : static_cast<double>(x);
+            }
+        }
+    }
+    const double mean = sum / md;
+    double squares = 0.0;
+    for (int c0 = 0; c0 < m; c0 += 64 * kU) {
+        float v[kU];
+#pragma unroll
+        for (int k = 0; k < kU; k++) {
+            const int d = c0 + 64 * k + lane;
+            v[k] = d < m ? w[d] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kU; k++) {
+            const int cnt = m - (c0 + 64 * k) < 64 ? m - (c0 + 64 * k) : 64;
+            for (int l = 0; l < cnt; l++) {
+                const float x = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v[k]), l));
+                const double t = (x == kMissing ? 0.0 : static_cast<double>(x)) - mean;
+                squares = __builtin_fma(t, t, squares);
+            }
+        }
+    }
+    const double sd = __builtin_sqrt(squares / md);
+    mean_f = static_cast<float>(mean);
+    sd_f = static_cast<float>(sd <= 1e-12 ? 0.0 : sd);
+}
+
 __device__ inline double wave_sum_f64(double v)
 {
 #pragma unroll
@@ -109,6 +160,44 @@ __device__ inline double gamma64(double k)
 {
     constexpr double u = 1.1102230246251565e-16;
     return k * u / (1.0 - k * u) * (1.0 + 1e-10);
+}
+
+// Bound on |sum_seq - S| for the reference's index-order FP64 sum sum_seq of
+// the window's samples (S the exact sum, A >= sum |v|), from its partial
+// sums: each addition's rounding is at most u times the partial sum it
+// produces, and |s_k| <= |S_k| + gamma_k A, so |sum_seq - S| <= u / (1 - u)
+// (sum_k |S_k| + gamma_m m A).  The exact prefix sums S_k are bounded by this
+// wavefront's own (rows of 64 scanned in FP64 on top of the sum of the rows
+// before: within gamma_{R+7} A of exact, R = ceil(m / 64)).  For sums that
+// wander like a random walk this is about sqrt(m) times below gamma_{m-1} A.
+__device__ double seq_sum_error(const float *w, int m, double A)
+{
+    constexpr double u = 1.1102230246251565e-16;
+    const int lane = threadIdx.x & 63;
+    const int R = (m + 63) / 64;
+    double run = 0.0, sabs = 0.0;
+    for (int r = 0; r < R; r++) {
+        const int d = 64 * r + lane;
+        double y = 0.0;
+        if (d < m) {
+            const float v = w[d];
+            y = v == kMissing ? 0.0 : static_cast<double>(v);
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double t = __shfl_up(y, off);
+            if (lane >= off)
+                y += t;
+        }
+        if (d < m)
+            sabs += fabs(run + y);
+        run += __shfl(y, 63);
+    }
+    sabs = __shfl(wave_sum_f64(sabs), 0);
+    const double md = static_cast<double>(m);
+    const double sum_abs = sabs * (1.0 + 2.0 * gamma64(static_cast<double>(R + 6))) +
+                           md * gamma64(static_cast<double>(R + 7)) * A;
+    return u / (1.0 - u) * (sum_abs + gamma64(md) * md * A) * (1.0 + 1e-10);
 }
 
 // The reference statistics (exact_stats) of one window by a wavefront; w[0]
@@ -131,14 +220,24 @@ __device__ bool wave_stats(const float *w, int m, float &mean_f, float &sd_f)
     // Parallel sum: ceil(m / 64) sequential terms then 6 tree levels.
     const double gp = gamma64(static_cast<double>((m + 63) / 64 + 6));
     const double A = a * (1.0 + 2.0 * gp);                      // >= sum |v|
-    const double Es = (gamma64(static_cast<double>(m - 1)) + gp) * A;  // |sum_seq - s|
+    double Es = (gamma64(static_cast<double>(m - 1)) + gp) * A;  // |sum_seq - s|
     // mean_ref = fl(sum_seq / m) lies in [ml, mh].
     double ml = (s - Es) / md, mh = (s + Es) / md;
     ml -= fabs(ml) * 4.0 * u;
     mh += fabs(mh) * 4.0 * u;
     mean_f = static_cast<float>(ml);
-    if (static_cast<float>(mh) != mean_f)
-        return false;
+    if (static_cast<float>(mh) != mean_f) {
+        // Undecided (common for means near 0, where the float spacing is
+        // fine): the partial-sum bound instead.
+        Es = seq_sum_error(w, m, A) + gp * A;
+        ml = (s - Es) / md;
+        mh = (s + Es) / md;
+        ml -= fabs(ml) * 4.0 * u;
+        mh += fabs(mh) * 4.0 * u;
+        mean_f = static_cast<float>(ml);
+        if (static_cast<float>(mh) != mean_f)
+            return false;
+    }
     // Squares about the midpoint, then the reference's sum about mean_ref.
     const double mu = 0.5 * (ml + mh);
     const double W = 0.5 * (mh - ml) * (1.0 + 4.0 * u);
@@ -221,7 +320,7 @@ __global__ __launch_bounds__(kThreads) void k_verify_wave(const float *__restric
                                                           const float *__restrict__ qz,
                                                           const int32_t *__restrict__ order,
                                                           const Cand *__restrict__ cand, uint64_t count,
-                                                          float *__restrict__ d_out)
+                                                          float *__restrict__ d_out, int force_seq)
 {
     const uint64_t i = (static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -235,12 +334,8 @@ __global__ __launch_bounds__(kThreads) void k_verify_wave(const float *__restric
     if (static_cast<uint64_t>(c.j) + static_cast<uint64_t>(m) <= n && x[c.j] != kMissing) {
         const float *w = x + c.j;
         float mean, sd;
-        if (!wave_stats(w, m, mean, sd)) {
-            if (lane == 0)
-                exact_stats(w, m, mean, sd);
-            mean = __shfl(mean, 0);
-            sd = __shfl(sd, 0);
-        }
+        if (force_seq || !wave_stats(w, m, mean, sd))
+            wave_exact_stats(w, m, mean, sd);
         out = wave_dist(qz + tmpl[c.t].q_off, w, m, mean, sd, order + tmpl[c.t].order_off);
     }
     if (lane == 0)
@@ -310,7 +405,7 @@ __global__ __launch_bounds__(kThreads) void k_dist_batch(const float *__restrict
 }  // namespace
 
 hipError_t launch_verify(const float *x, uint64_t n, const TmplDev *tmpl, const float *qz, const int32_t *order,
-                         const Cand *cand, uint64_t count, float *d_out, hipStream_t stream)
+                         const Cand *cand, uint64_t count, float *d_out, bool force_seq, hipStream_t stream)
 {
     if (count == 0)
         return hipSuccess;
@@ -322,7 +417,7 @@ hipError_t launch_verify(const float *x, uint64_t n, const TmplDev *tmpl, const 
         const uint64_t g = wgrid - g0 < (1u << 31) - 1 ? wgrid - g0 : (1u << 31) - 1;
         const uint64_t first = g0 * (kThreads / 64);
         hipLaunchKernelGGL(k_verify_wave, dim3(static_cast<unsigned>(g)), dim3(kThreads), 0, stream, x, n, tmpl, qz,
-                           order, cand + first, count - first, d_out + first);
+                           order, cand + first, count - first, d_out + first, force_seq ? 1 : 0);
     }
     return hipGetLastError();
 }
