@@ -49,7 +49,7 @@ def _args():
     ap.add_argument("--single-duration", type=float, default=3.0,
                     help="c3: seconds of the same stream replayed one request per launch")
     ap.add_argument("--lanes", type=int, default=2)
-    ap.add_argument("--sweep", type=float, nargs="*", default=[80.0, 60.0, 45.0, 35.0],
+    ap.add_argument("--sweep", type=float, nargs="*", default=[140.0, 120.0, 100.0, 80.0, 60.0],
                     help="c3: lower arrival rates to replay for the p99 <= 50 ms search")
     ap.add_argument("--sweep-duration", type=float, default=6.0)
     ap.add_argument("--sweep-batch", type=int, default=4)
