@@ -530,14 +530,26 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
                                                 : chunk_dist_lo(pa, pb, ca, cb, Lc, invL, K, Q2, m,
                                                                 static_cast<double>(dmax) + nslack, mu_abs);
                     if (!(lo > bound)) {
-                        const unsigned long long slot = atomicAdd(nrec, 1ULL);
+                        const unsigned long long slot = wave_claim(nrec);
                         if (slot < rec_cap) {
                             rec[slot] = ChunkRec{static_cast<uint32_t>(t), static_cast<uint32_t>(base + i_first), cnt,
                                                  c, round_up_f(eabs), 0u};
                             float *dst = rec_dt + slot * kChunkW;
-                            for (int l = 0; l < cnt; l++) {
-                                const int n = n_first + l;
-                                dst[l] = fdata[n + 2 * (n >> 5)] * invB;
+                            if (cnt == kChunkW) {
+                                // Two padded rows of 32 outputs, 16 bytes per store.
+                                const float2 *row = reinterpret_cast<const float2 *>(fdata + 68 * (c0 + tid));
+                                float4 *d4 = reinterpret_cast<float4 *>(dst);
+#pragma unroll 4
+                                for (int q = 0; q < 16; q++) {
+                                    const float2 a = row[(q >> 3) * 17 + 2 * (q & 7)];
+                                    const float2 b = row[(q >> 3) * 17 + 2 * (q & 7) + 1];
+                                    d4[q] = make_float4(a.x * invB, a.y * invB, b.x * invB, b.y * invB);
+                                }
+                            } else {
+                                for (int l = 0; l < cnt; l++) {
+                                    const int n = n_first + l;
+                                    dst[l] = fdata[n + 2 * (n >> 5)] * invB;
+                                }
                             }
                         }
                     }
