@@ -163,7 +163,7 @@ double fft_seconds_per_window(int L)
 
 constexpr int kFusedChunk = 16;    // templates per fused FFT launch
 constexpr uint64_t kChunkQueueMin = 1u << 20;  // chunk records a lane's fused pass starts with (280 B each)
-constexpr int kFftMinLength = 256;  // shortest template the cost model sends to the FFT form
+constexpr int kFftMinLength = 160;  // shortest template the cost model sends to the FFT form
 
 // The early-abandoning scan (a third form, chosen once the bounds are known,
 // as the reference decides after its sampled early-abandon test,
@@ -254,8 +254,10 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
             // One FFT class per batch, sized by its longest FFT template (the
             // first to move): every FFT template pays that class's cost.
             // Templates shorter than kFftMinLength stay on the matrix cores:
-            // the FFT form's 64-window chunk filter needs a common core of a
-            // few hundred samples to prune (m = 128: 2.6x its cost at m = 300).
+            // the FFT form's 64-window chunk filter needs a common core of
+            // about a hundred samples to prune (single templates at 2^31
+            // samples: m = 160 takes 8.7-11.8 ms on the FFT form against
+            // 13.5-16.5 on the matrix cores; m = 128 is mixed).
             const int L_class = plan.tmpl[count - 1].m;
             while (plan.Tg > 0) {
                 const TmplDev &t = plan.tmpl[plan.Tg - 1];
