@@ -462,6 +462,8 @@ def main():
         "candidates_per_step": last["candidates"],
         "max_abs_rho_err": last["max_abs_rho_err"],
         "phase_ms": {k: 1e3 * last[k] for k in ("seconds_seed", "seconds_main", "seconds_verify", "seconds_select")},
+        "autotune": {"templates_per_workgroup": last["tuned_group"],
+                     "note": "timed on the first (warmup) call among 4, 8, 16 and kept for the context"},
     }
     print(json.dumps(out), flush=True)
 

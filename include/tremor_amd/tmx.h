@@ -79,6 +79,9 @@ typedef struct {
     int32_t verify_seq;      /* 1 = the exact recomputation takes every window's statistics by the
                                 sequential sums (tests, diagnostics); 0 = default (a rigorous
                                 parallel shortcut, sequential only where it cannot decide) */
+    int32_t autotune;        /* 1 = default: the templates per matrix-core workgroup are timed on a
+                                sampled pass the first time a batch shape is seen and the fastest
+                                kept for the context; 0 = fixed (8) */
 } tmx_cfg;
 
 typedef struct {
@@ -99,6 +102,8 @@ typedef struct {
                                 tightened from the candidates' own upper bounds */
     int32_t scan_templates;  /* templates that took the early-abandoning scan */
     double scan_blocks;      /* their mean 32-sample blocks visited per window (sampled) */
+    int32_t tuned_group;     /* templates per matrix-core workgroup used by this call */
+    double seconds_tune;     /* time spent timing group sizes in this call (first call of a shape) */
 } tmx_stats;
 
 typedef struct {

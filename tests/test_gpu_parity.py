@@ -374,3 +374,36 @@ def test_verify_sequential_statistics(oracle):
         tt, tp, td = oracle.scan_threshold(x, tm, 0.55, 0)
         assert np.array_equal(ra.tmpl, tt) and np.array_equal(ra.pos, tp)
         assert np.array_equal(ra.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_autotuned_group_size(oracle):
+    """The matrix-core form's templates per workgroup are timed on the first
+    batch of a shape (4, 8 or 16) and kept for the context; the answers do
+    not depend on the choice: identical with the autotuner off (8), and the
+    oracle's."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(400_000, 47, events=40)
+    lengths = [int(v) for v in rng.integers(200, 1200, size=24)]
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    with Context(precision="bf16x3", fft="off", scan="off") as a, \
+            Context(precision="bf16x3", fft="off", scan="off", autotune=False) as b:
+        a.load_waveform(x)
+        b.load_waveform(x)
+        _log("autotune-first")
+        pa, da, st = a.knn(tm, 10, 0)
+        print("  stats", st, flush=True)
+        assert st["tuned_group"] in (4, 8, 16) and st["seconds_tune"] > 0.0
+        _log("autotune-cached")
+        pa2, da2, st2 = a.knn(tm, 10, 0)
+        assert st2["tuned_group"] == st["tuned_group"] and st2["seconds_tune"] == 0.0
+        pb, db, stb = b.knn(tm, 10, 0)
+        assert stb["tuned_group"] == 8 and stb["seconds_tune"] == 0.0
+        for p, d in ((pa2, da2), (pb, db)):
+            assert np.array_equal(pa, p) and np.array_equal(da.view(np.uint32), d.view(np.uint32))
+        po, do, _ = oracle.scan_knn(x, tm, 10, 0)
+        assert np.array_equal(pa, po) and np.array_equal(da.view(np.uint32), do.view(np.uint32))
+        r = a.threshold(tm, 0.7, 0)
+        tt, tp, td = oracle.scan_threshold(x, tm, 0.7, 0)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))

@@ -37,7 +37,7 @@ REQ_THRESHOLD = 1
 class Cfg(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32), ("cand_capacity", ctypes.c_int64),
                 ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32), ("fft_mode", ctypes.c_int32),
-                ("scan_mode", ctypes.c_int32), ("verify_seq", ctypes.c_int32)]
+                ("scan_mode", ctypes.c_int32), ("verify_seq", ctypes.c_int32), ("autotune", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -47,7 +47,8 @@ class Stats(ctypes.Structure):
                 ("candidates", ctypes.c_int64), ("retries", ctypes.c_int32), ("max_abs_rho_err", ctypes.c_float),
                 ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double),
                 ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64),
-                ("scan_templates", ctypes.c_int32), ("scan_blocks", ctypes.c_double)]
+                ("scan_templates", ctypes.c_int32), ("scan_blocks", ctypes.c_double),
+                ("tuned_group", ctypes.c_int32), ("seconds_tune", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -162,7 +163,7 @@ class Context:
 
     def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
                  seed_target: int = 0, verbose: bool = False, fft: str = "auto", scan: str = "auto",
-                 verify_seq: bool = False):
+                 verify_seq: bool = False, autotune: bool = True):
         L = lib()
         cfg = Cfg()
         L.tmx_cfg_default(ctypes.byref(cfg))
@@ -174,6 +175,7 @@ class Context:
         cfg.fft_mode = FFT_MODES[fft]
         cfg.scan_mode = SCAN_MODES[scan]
         cfg.verify_seq = 1 if verify_seq else 0
+        cfg.autotune = 1 if autotune else 0
         h = ctypes.c_void_p()
         rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:

@@ -17,6 +17,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <map>
 #include <cstddef>
 #include <memory>
 #include <mutex>
@@ -243,6 +244,10 @@ struct tmx_ctx {
     // largest candidate buffer and chunk queue (entries) any lane has needed:
     // lanes reserve at least these, so one overflow and rerun teaches all
     std::atomic<uint64_t> cand_hint{0}, chunk_hint{0};
+    // autotuner: templates per matrix-core workgroup chosen per batch shape
+    // (precision, template count, mean padded length), guarded by tune_mu
+    std::mutex tune_mu;
+    std::map<uint32_t, int> tuned_group;
 
     int fail(const std::string &msg, int code = -1)
     {

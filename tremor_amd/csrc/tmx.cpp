@@ -186,6 +186,16 @@ double planned_seconds_per_window(const tmx_cfg &cfg, const CallPlan &plan, int 
     return gemm_seconds_per_window(cfg.precision, t.kpad, plan.Tg);
 }
 
+// Consecutive templates (length-sorted) of the matrix-core form in groups of
+// G: one workgroup per (block, group).
+void set_groups(CallPlan &plan, int G)
+{
+    plan.group_begin.clear();
+    for (int d = 0; d < plan.Tg; d += G)
+        plan.group_begin.push_back(d);
+    plan.group_begin.push_back(plan.Tg);
+}
+
 int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count, CallPlan &plan,
                std::vector<float> &raw_packed)
 {
@@ -270,11 +280,7 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
     }
     for (int d = plan.Tg; d < count; d++)
         plan.fft_L = std::max(plan.fft_L, plan.tmpl[d].m);
-    const int G = cfg.precision == TMX_PREC_FP32 ? 1 : mfma_group_size();
-    plan.group_begin.clear();
-    for (int d = 0; d < plan.Tg; d += G)
-        plan.group_begin.push_back(d);
-    plan.group_begin.push_back(plan.Tg);
+    set_groups(plan, cfg.precision == TMX_PREC_FP32 ? 1 : mfma_group_size());
     return 0;
 }
 
@@ -647,6 +653,100 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
     return 0;
 }
 
+// ---- autotuner: templates per matrix-core workgroup -------------------------
+//
+// The group size trades workgroups (a larger grid) against the per-workgroup
+// work shared by a group (the block's largest |x~|, its statistics reads and
+// staging).  The first batch of a shape - precision, matrix-core template
+// count and mean padded length, each in powers of two - times every
+// candidate on a strided sample of blocks of the main pass, with the batch's
+// real bounds (after seeding), and the context keeps the fastest for that
+// shape.  Answers do not depend on the group size.
+
+uint64_t waveform_blocks(tmx_ctx *ctx, const tmx_cfg &cfg);
+
+constexpr int kTuneGroups[] = {4, 8, 16};
+constexpr uint64_t kTuneBlocks = 512;  // blocks per timed sample (every S-th block)
+
+uint32_t tune_key(const tmx_cfg &cfg, const CallPlan &plan)
+{
+    double ksum = 0.0;
+    for (int d = 0; d < plan.Tg; d++)
+        ksum += static_cast<double>(plan.tmpl[d].kpad);
+    const int kmean = static_cast<int>(ksum / std::max(plan.Tg, 1));
+    uint32_t tb = 0, kb = 0;
+    while ((1 << tb) < plan.Tg)
+        tb++;
+    while ((1 << kb) < kmean)
+        kb++;
+    return static_cast<uint32_t>(cfg.precision) << 16 | tb << 8 | kb;
+}
+
+int upload_groups(Lane &L, const CallPlan &plan)
+{
+    LANE_HIP(L.groups.reserve(sizeof(int32_t) * plan.group_begin.size()), "hipMalloc groups");
+    LANE_HIP(hipMemcpyAsync(L.groups.p, plan.group_begin.data(), sizeof(int32_t) * plan.group_begin.size(),
+                            hipMemcpyHostToDevice, L.stream),
+             "copy groups");
+    return 0;
+}
+
+int autotune_groups(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
+{
+    if (cfg.precision == TMX_PREC_FP32) {
+        st.tuned_group = 1;
+        return 0;
+    }
+    st.tuned_group = mfma_group_size();
+    int inner = 0;
+    for (int d = 0; d < plan.Tg; d++)
+        inner += plan.tmpl[d].ea ? 0 : 1;
+    // Up to the smallest candidate every size gives one group: nothing to choose.
+    if (!cfg.autotune || inner <= kTuneGroups[0])
+        return 0;
+    const uint32_t key = tune_key(cfg, plan);
+    int G = 0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->tune_mu);
+        const auto it = ctx->tuned_group.find(key);
+        if (it != ctx->tuned_group.end())
+            G = it->second;
+    }
+    if (G == 0) {
+        const double t0 = now_s();
+        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
+        const uint64_t blocks = waveform_blocks(ctx, cfg);
+        const int S = static_cast<int>(std::max<uint64_t>(1, blocks / kTuneBlocks));
+        float best = 0.0f;
+        for (const int g : kTuneGroups) {
+            set_groups(plan, g);
+            if (int rc = upload_groups(L, plan))
+                return rc;
+            const int groups = static_cast<int>(plan.group_begin.size()) - 1;
+            LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
+            LANE_HIP(hipEventRecord(L.ev0, L.stream), "event");
+            LANE_HIP(launch_scan_mfma(w, L.tmpl.as<TmplDev>(), plan.T, L.groups.as<int32_t>(), groups, cfg.precision,
+                                      L.afrag.p, ctx->rb.as<float2>(), ctx->binfo.as<float4>(), ctx->rb_len,
+                                      kModeMain, S, cand_out(L), L.stream),
+                     "MFMA scan (tuning)");
+            LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
+            LANE_HIP(hipEventSynchronize(L.ev1), "tuning");
+            float ms = 0.0f;
+            LANE_HIP(hipEventElapsedTime(&ms, L.ev0, L.ev1), "event time");
+            if (G == 0 || ms < best) {
+                best = ms;
+                G = g;
+            }
+        }
+        std::lock_guard<std::mutex> lk(ctx->tune_mu);
+        ctx->tuned_group[key] = G;
+        st.seconds_tune = now_s() - t0;
+    }
+    st.tuned_group = G;
+    set_groups(plan, G);
+    return upload_groups(L, plan);
+}
+
 // Piece stride of the FFT form's seed pass.  The default target (1024
 // entries) is raised for large k on long waveforms, where it samples too few
 // pieces for a tight k-th bound: up to one piece in 150 and 48 entries per
@@ -892,8 +992,10 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
         return rc;
     st.scan_templates = static_cast<int32_t>(plan.scan_ids.size());
     st.scan_blocks = plan.scan_blocks;
+    st.seconds_seed = now_s() - t1;
+    if ((rc = autotune_groups(ctx, L, cfg, plan, st)))
+        return rc;
     const double t2 = now_s();
-    st.seconds_seed = t2 - t1;
 
     uint64_t ncand = 0;
     if ((rc = run_main(ctx, L, cfg, plan, f, &ncand, &st.retries)))
@@ -1096,6 +1198,7 @@ int tmx_cfg_default(tmx_cfg *cfg)
     cfg->fft_mode = TMX_FFT_AUTO;
     cfg->scan_mode = TMX_SCAN_AUTO;
     cfg->verify_seq = 0;
+    cfg->autotune = 1;
     return 0;
 }
 
