@@ -290,10 +290,11 @@ def bench_serving(a):
     ctx.load_waveform(w.x)
     load_s = time.time() - t
     # Derived structures (block prefix pairs for templates up to 8192, FFT
-    # piece spectra of both fused classes) are built by a first search of
-    # each form.
+    # piece spectra of every fused overlap-save shape) are built by a first
+    # search of each form.
     t = time.time()
-    warm = [synth.cut_templates(w.x, w.events, [m], np.random.default_rng(1))[0] for m in (8192, 2000, 128)]
+    warm = [synth.cut_templates(w.x, w.events, [m], np.random.default_rng(1))[0]
+            for m in (8192, 4000, 2000, 1000, 128)]
     for tm in warm:
         ctx.knn([tm], 1, 0)
     warm_s = time.time() - t

@@ -104,6 +104,8 @@ typedef struct {
     double scan_blocks;      /* their mean 32-sample blocks visited per window (sampled) */
     int32_t tuned_group;     /* templates per matrix-core workgroup used by this call */
     double seconds_tune;     /* time spent timing group sizes in this call (first call of a shape) */
+    int32_t fft_lmax;        /* longest template of the FFT form's piece shape (pieces of fft_block
+                                samples every fft_block - fft_lmax + 1 windows; 0: none) */
 } tmx_stats;
 
 typedef struct {

@@ -302,11 +302,13 @@ def test_fused_fft_many_templates_matches_mfma(oracle):
     assert np.array_equal(d[:2].view(np.uint32), do.view(np.uint32))
 
 
-@pytest.mark.parametrize("m", [4096, 4097])
-def test_fused_fft_class_boundary(oracle, m):
-    """Templates of 4096 samples take the B = 2^14 fused class (two
-    workgroups per CU, radix-2 last pass), 4097 the B = 2^15 class; both
-    must return the oracle's answers, alone and batched with a short
+@pytest.mark.parametrize("m,shape", [(1024, (16384, 1024)), (1025, (16384, 2048)), (2048, (16384, 2048)),
+                                     (2049, (32768, 4096)), (4096, (32768, 4096)), (4097, (32768, 8192))])
+def test_fused_fft_class_boundary(oracle, m, shape):
+    """Every overlap-save shape of the fused FFT form at its boundary:
+    B = 2^14 (two workgroups per CU, radix-2 last pass) with pieces for
+    templates up to 1024 and 2048 samples, B = 2^15 for up to 4096 and 8192;
+    each must return the oracle's answers, alone and batched with a short
     template."""
     from tremor_amd import synth
     from tremor_amd.capi import Context
@@ -317,7 +319,7 @@ def test_fused_fft_class_boundary(oracle, m):
         _log("fft-class", m=m)
         pos, d, st = ctx.knn(tm[:1], 7, 0)
         print("  stats", st, flush=True)
-        assert st["fft_block"] == (16384 if m <= 4096 else 32768)
+        assert (st["fft_block"], st["fft_lmax"]) == shape
         po, do, _ = oracle.scan_knn(x, tm[:1], 7, 0)
         assert np.array_equal(pos, po)
         assert np.array_equal(d.view(np.uint32), do.view(np.uint32))

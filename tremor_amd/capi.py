@@ -48,7 +48,7 @@ class Stats(ctypes.Structure):
                 ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double),
                 ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64),
                 ("scan_templates", ctypes.c_int32), ("scan_blocks", ctypes.c_double),
-                ("tuned_group", ctypes.c_int32), ("seconds_tune", ctypes.c_double)]
+                ("tuned_group", ctypes.c_int32), ("seconds_tune", ctypes.c_double), ("fft_lmax", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -177,8 +177,13 @@ struct Lane {
     }
 };
 
-// Piece spectra of the resident waveform for one FFT length B (overlap-save:
-// pieces of B samples every step = B - Lmax + 1, Lmax = B / 4).
+// Overlap-save shape: pieces of B samples every B - Lmax + 1 windows, for
+// templates up to Lmax samples.
+struct FftShape {
+    int B, Lmax;
+};
+
+// Piece spectra of the resident waveform for one shape (B, Lmax).
 struct FftClass {
     int B = 0, Lmax = 0, step = 0, bins = 0;
     uint64_t P = 0;

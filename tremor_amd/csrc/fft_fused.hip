@@ -1,5 +1,6 @@
 // Fused overlap-save FFT candidate pass for FFT lengths B = 2^15 and 2^14
-// (templates up to Lmax = B / 4 = 8192 / 4096 samples), in the role of the
+// (pieces of B samples every step = B - Lmax + 1 windows, templates up to
+// Lmax samples, Lmax from B / 16 to B / 4), in the role of the
 // reference's FFT fallback pass (fft_fallback.hpp:67-117) with its
 // filter-then-exact-recheck contract.  B = 2^15: one workgroup of 1024
 // threads per CU; B = 2^14: two of 512 (the LDS holds two transforms, so one
@@ -54,11 +55,10 @@ struct Geom {
     static constexpr int kM = 1 << LOG2M;                    // complex points of the half-length transform
     static constexpr int kB = 2 * kM;                        // real length B
     static constexpr int kT = kM / 16;                       // threads per workgroup
-    static constexpr int kStep = kB - kB / 4 + 1;            // windows per piece
-    static constexpr int kChunks = (kStep + 63) / 64 + 1;    // 64-output chunks holding a piece's windows
+    static constexpr int kStepMax = kB - kB / 16 + 1;        // windows per piece, templates up to B / 16
     static constexpr int kLds = kM + (kM >> 4) + 2;          // padded complex slots (index M included)
     static constexpr int kPerCU = LOG2M == 14 ? 1 : 2;       // workgroups per CU
-    static_assert(kChunks <= kT, "one thread per chunk");
+    static_assert((kStepMax + 63) / 64 + 1 <= kT, "one thread per 64-output chunk of a piece");
     static_assert(LOG2M == 13 || LOG2M == 14, "B = 2^14 or 2^15");
 };
 
@@ -388,11 +388,12 @@ template <int LOG2M>
 __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves per SIMD: kPerCU workgroups per CU
     Waveform w, const TmplDev *__restrict__ tmpl, int t0, int F, const float2 *__restrict__ spectra, int bins,
     const float2 *__restrict__ qspec, const float *__restrict__ qmax, const float4 *__restrict__ pinfo,
-    const float2 *__restrict__ pnorm, const float2 *__restrict__ E, uint64_t P, int stride, double kfft, int mode,
-    ChunkRec *__restrict__ rec, float *__restrict__ rec_dt, unsigned long long *__restrict__ nrec, uint64_t rec_cap)
+    const float2 *__restrict__ pnorm, const float2 *__restrict__ E, uint64_t P, int step, int stride, double kfft,
+    int mode, ChunkRec *__restrict__ rec, float *__restrict__ rec_dt, unsigned long long *__restrict__ nrec,
+    uint64_t rec_cap)
 {
     using G = Geom<LOG2M>;
-    constexpr int kM = G::kM, kB = G::kB, kT = G::kT, kStep = G::kStep;
+    constexpr int kM = G::kM, kB = G::kB, kT = G::kT;
     __shared__ float2 data[G::kLds];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
 
     for (uint64_t it = blockIdx.x; it < nwork; it += gridDim.x) {
         const uint64_t p = it * static_cast<uint64_t>(stride);
-        const uint64_t base = p * static_cast<uint64_t>(kStep);
+        const uint64_t base = p * static_cast<uint64_t>(step);
         const float4 pi = pinfo[p];
         const float c = pi.x;
         const double amax = static_cast<double>(pi.w) * (1.0 + 1e-6);
@@ -437,8 +438,8 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
             const int m = tmpl[t].m;
             const uint64_t W = static_cast<uint64_t>(m) <= w.n ? w.n - static_cast<uint64_t>(m) + 1 : 0;
             const bool active = W > 0 && !tmpl[t].ea && base < W;
-            const uint64_t nwin64 = !active ? 0 : W - base < static_cast<uint64_t>(kStep) ? W - base
-                                                                                         : static_cast<uint64_t>(kStep);
+            const uint64_t nwin64 = !active ? 0 : W - base < static_cast<uint64_t>(step) ? W - base
+                                                                                        : static_cast<uint64_t>(step);
             const int nwin = static_cast<int>(nwin64);
             // Thread tid owns the chunk of outputs [64 (c0 + tid), + 64): the
             // windows i = n - (m - 1) of its outputs n, clipped to the piece's
@@ -670,7 +671,7 @@ int fft_chunk_width() { return kChunkW; }
 // queue in main mode; in seed mode the entries kept suffice).
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
-                            const float2 *table, uint64_t P, int stride, double kfft, int mode, void *rec,
+                            const float2 *table, uint64_t P, int step, int stride, double kfft, int mode, void *rec,
                             float *rec_dt, unsigned long long *nrec, uint64_t rec_cap, const CandOut &co, int cus,
                             hipStream_t stream)
 {
@@ -679,18 +680,21 @@ hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int 
     const bool small = bins == Geom<13>::kM + 1;
     if (!small && bins != Geom<14>::kM + 1)
         return hipErrorInvalidValue;
+    // Pieces of step windows: every 64-output chunk of one needs a thread.
+    if (step < 1 || step > (small ? Geom<13>::kStepMax : Geom<14>::kStepMax))
+        return hipErrorInvalidValue;
     const int grid = cus * (small ? Geom<13>::kPerCU : Geom<14>::kPerCU);
     const uint64_t nwork = (P + stride - 1) / stride;
     const unsigned g = static_cast<unsigned>(nwork < static_cast<uint64_t>(grid) ? nwork : grid);
     hipLaunchKernelGGL(k_zero_nrec, dim3(1), dim3(1), 0, stream, nrec);
     if (small)
         hipLaunchKernelGGL(k_fft_fused<13>, dim3(g), dim3(Geom<13>::kT), 0, stream, w, tmpl, t0, F, spectra, bins,
-                           qspec, qmax, pinfo, pnorm, table, P, stride, kfft, mode, static_cast<ChunkRec *>(rec),
-                           rec_dt, nrec, rec_cap);
+                           qspec, qmax, pinfo, pnorm, table, P, step, stride, kfft, mode,
+                           static_cast<ChunkRec *>(rec), rec_dt, nrec, rec_cap);
     else
         hipLaunchKernelGGL(k_fft_fused<14>, dim3(g), dim3(Geom<14>::kT), 0, stream, w, tmpl, t0, F, spectra, bins,
-                           qspec, qmax, pinfo, pnorm, table, P, stride, kfft, mode, static_cast<ChunkRec *>(rec),
-                           rec_dt, nrec, rec_cap);
+                           qspec, qmax, pinfo, pnorm, table, P, step, stride, kfft, mode,
+                           static_cast<ChunkRec *>(rec), rec_dt, nrec, rec_cap);
     hipLaunchKernelGGL(k_chunk_windows, dim3(static_cast<unsigned>(4 * cus)), dim3(256), 0, stream, w, tmpl,
                        static_cast<const ChunkRec *>(rec), rec_dt, nrec, rec_cap, mode, co);
     hipLaunchKernelGGL(k_note_max, dim3(1), dim3(1), 0, stream, nrec);

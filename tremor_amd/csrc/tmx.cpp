@@ -60,7 +60,7 @@ size_t fft_chunk_rec_bytes();
 int fft_chunk_width();  // windows per chunk record of the fused pass
 hipError_t launch_fft_fused(const Waveform &w, const TmplDev *tmpl, int t0, int F, const float2 *spectra, int bins,
                             const float2 *qspec, const float *qmax, const float4 *pinfo, const float2 *pnorm,
-                            const float2 *table, uint64_t P, int stride, double kfft, int mode, void *rec,
+                            const float2 *table, uint64_t P, int step, int stride, double kfft, int mode, void *rec,
                             float *rec_dt, unsigned long long *nrec, uint64_t rec_cap, const CandOut &co, int grid,
                             hipStream_t stream);
 hipError_t launch_piece_norm(const double *norm2, float2 *pnorm, uint64_t P, hipStream_t stream);
@@ -139,21 +139,40 @@ double gemm_seconds_per_window(int precision, int kpad, int group)
     return per_k * kpad + 5.7e-12 / std::max(1, std::min(group, 8));
 }
 
-int fft_block_for(int L)
+// Overlap-save shape of the FFT form for a batch whose longest FFT template
+// has L samples: pieces of B samples every B - Lmax + 1 windows.  The fused
+// lengths take the smallest Lmax that holds L (a piece's transform costs the
+// same whatever Lmax, so a shorter Lmax gives more windows per piece); longer
+// templates double B from 2^16 until B >= 4 L, with Lmax = B / 4 (the
+// reference's rule, fft_fallback.cpp:124-126, which starts at B = 2^15).
+FftShape fft_shape_for(int L)
 {
-    int B = 1 << 14;
+    if (L <= 1024)
+        return FftShape{1 << 14, 1024};
+    if (L <= 2048)
+        return FftShape{1 << 14, 2048};
+    if (L <= 4096)
+        return FftShape{1 << 15, 4096};
+    if (L <= 8192)
+        return FftShape{1 << 15, 8192};
+    int B = 1 << 16;
     while (B < 4 * L)
         B *= 2;
-    return B;
+    return FftShape{B, B / 4};
 }
 
+// Fused lengths: the per-window cost measured at Lmax = B / 4 (3.05e-12 s for
+// B = 2^14, 3.4e-12 for 2^15), times the ratio of windows per piece.
 double fft_seconds_per_window(int L)
 {
-    const int B = fft_block_for(L);
+    const FftShape sh = fft_shape_for(L);
+    const int B = sh.B;
+    const double step = static_cast<double>(B - sh.Lmax + 1);
+    const double step_quarter = static_cast<double>(B - B / 4 + 1);
     if (B == (1 << 14))
-        return 3.05e-12;
+        return 3.05e-12 * step_quarter / step;
     if (B == (1 << 15))
-        return 3.4e-12;
+        return 3.4e-12 * step_quarter / step;
     double lg = 0.0;
     while ((1 << static_cast<int>(lg)) < B)
         lg += 1.0;
@@ -295,10 +314,10 @@ int block_prefix_len(const CallPlan &plan)
     return ((nw + kmax + 1023) / 1024) * 1024;
 }
 
-FftClass *find_fft_class(tmx_ctx *ctx, int B)
+FftClass *find_fft_class(tmx_ctx *ctx, FftShape sh)
 {
     for (auto &c : ctx->fft)
-        if (c->B == B)
+        if (c->B == sh.B && c->Lmax == sh.Lmax)
             return c.get();
     return nullptr;
 }
@@ -308,7 +327,7 @@ bool resident_ready(tmx_ctx *ctx, const tmx_cfg &cfg, const CallPlan &plan)
     if (cfg.precision != TMX_PREC_FP32 && plan.Tg > 0 &&
         !(ctx->rb_n == ctx->n && ctx->rb_len >= block_prefix_len(plan)))
         return false;
-    if (plan.T > plan.Tg && !find_fft_class(ctx, fft_block_for(plan.fft_L)))
+    if (plan.T > plan.Tg && !find_fft_class(ctx, fft_shape_for(plan.fft_L)))
         return false;
     return true;
 }
@@ -334,21 +353,20 @@ int build_block_prefix(tmx_ctx *ctx, Lane &L, int need)
 
 constexpr int kFftPieceBatch = 64;  // pieces / templates per forward transform launch
 
-// Piece spectra of the current waveform for templates up to L samples:
-// B = 2^14 doubled until >= 4 L (the reference starts at 2^15,
-// fft_fallback.cpp:124-126; 2^14 lets two fused workgroups share a CU),
-// Lmax = B / 4,
-// step = B - Lmax + 1; per piece the centre, the rebased prefix pairs of the
-// level-1 filter, ||x~||_2 and ||X||_inf.
+// Piece spectra of the current waveform for templates up to Lt samples, in
+// the shape fft_shape_for(Lt) (B, Lmax; step = B - Lmax + 1); per piece the
+// centre, the rebased prefix pairs of the level-1 filter (hipFFT lengths),
+// ||x~||_2 and ||X||_inf.
 int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
 {
-    const int B = fft_block_for(Lt);
-    if (find_fft_class(ctx, B))
+    const FftShape sh = fft_shape_for(Lt);
+    if (find_fft_class(ctx, sh))
         return 0;
+    const int B = sh.B;
     auto cls = std::make_unique<FftClass>();
     FftClass &f = *cls;
     f.B = B;
-    f.Lmax = B / 4;
+    f.Lmax = sh.Lmax;
     f.step = B - f.Lmax + 1;
     f.bins = B / 2 + 1;
     f.P = (ctx->n + f.step - 1) / f.step;
@@ -601,7 +619,7 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
             const int Fc = std::min(kFusedChunk, F - c0);
             LANE_HIP(launch_fft_fused(w, L.tmpl.as<TmplDev>(), plan.Tg + c0, Fc, f.spectra.as<float2>(), f.bins,
                                       L.qspec.as<float2>() + static_cast<size_t>(c0) * f.bins, L.qmax.as<float>() + c0,
-                                      f.pinfo.as<float4>(), f.pnorm.as<float2>(), f.table.as<float2>(), f.P, stride,
+                                      f.pinfo.as<float4>(), f.pnorm.as<float2>(), f.table.as<float2>(), f.P, f.step, stride,
                                       kfft, mode, L.chunk_rec.p, L.chunk_dt.as<float>(),
                                       L.chunk_n.as<unsigned long long>(), L.chunk_cap, cand_out(L),
                                       L.cus > 0 ? L.cus : 256, L.stream),
@@ -956,7 +974,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
                  const std::vector<float> &raw_packed, std::vector<QueryOut> &out, tmx_stats &st)
 {
     const int T = plan.T;
-    const FftClass *f = plan.T > plan.Tg ? find_fft_class(ctx, fft_block_for(plan.fft_L)) : nullptr;
+    const FftClass *f = plan.T > plan.Tg ? find_fft_class(ctx, fft_shape_for(plan.fft_L)) : nullptr;
     int rc = upload_and_prepare(L, cfg, plan, raw_packed, f);
     if (rc)
         return rc;
@@ -1011,6 +1029,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     for (int d = plan.Tg; d < plan.T; d++)
         st.fft_templates += plan.tmpl[d].ea ? 0 : 1;
     st.fft_block = f ? f->B : 0;
+    st.fft_lmax = f ? f->Lmax : 0;
     const double t3 = now_s();
     st.seconds_main = t3 - t2;
     st.candidates = static_cast<int64_t>(ncand);
