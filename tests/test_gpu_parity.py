@@ -263,16 +263,19 @@ def test_fused_fft_form(fft_ctx, oracle, merge):
 
 
 def test_cost_model_routes_long_templates(oracle):
+    """Sixteen short templates share full matrix-core groups (cheaper there
+    than on the FFT form); the long one takes the FFT form in its own
+    shape."""
     from tremor_amd import synth
     from tremor_amd.capi import Context
     x, onsets, rng = _wave(300_000, 12, events=20)
-    tm = synth.cut_templates(x, onsets, [256, 512, 30000], rng)
+    tm = synth.cut_templates(x, onsets, [int(v) for v in np.linspace(256, 400, 16)] + [30000], rng)
     with Context(precision="bf16x3", fft="auto") as ctx:
         ctx.load_waveform(x)
         _log("auto-route")
         pos, d, st = ctx.knn(tm, 5, 0)
         print("  stats", st, flush=True)
-        assert st["fft_templates"] == 1
+        assert st["fft_templates"] == 1 and st["fft_block"] == 131072
         po, do, _ = oracle.scan_knn(x, tm, 5, 0)
         assert np.array_equal(pos, po)
         assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
@@ -407,5 +410,30 @@ def test_autotuned_group_size(oracle):
         assert np.array_equal(pa, po) and np.array_equal(da.view(np.uint32), do.view(np.uint32))
         r = a.threshold(tm, 0.7, 0)
         tt, tp, td = oracle.scan_threshold(x, tm, 0.7, 0)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_fft_shape_groups_in_one_batch(oracle):
+    """One batch whose FFT templates fall in every overlap-save shape (the
+    four fused ones and a hipFFT length): each group runs in its own shape,
+    with its own spectra, seed stride and template spectra, and the answers
+    are the oracle's."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(400_000, 53, events=30)
+    tm = synth.cut_templates(x, onsets, [300, 1500, 3000, 6000, 9000, 700, 2500], rng)
+    with Context(precision="bf16x3", fft="always") as ctx:
+        ctx.load_waveform(x)
+        for k, merge in ((5, 0), (3, 40)):
+            _log("fft-groups-knn", k=k, merge=merge)
+            pos, d, st = ctx.knn(tm, k, merge)
+            print("  stats", st, flush=True)
+            assert st["fft_templates"] == len(tm) and st["fft_block"] == 65536
+            po, do, _ = oracle.scan_knn(x, tm, k, merge)
+            assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
+        _log("fft-groups-thr")
+        r = ctx.threshold(tm, 0.6, 3)
+        tt, tp, td = oracle.scan_threshold(x, tm, 0.6, 3)
         assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
         assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))

@@ -114,6 +114,15 @@ struct CallPlan {
     double flops = 0.0;             // sum of 2 m (n - m + 1)
     int Tg = 0;                     // templates [0, Tg) take the inner-product form, [Tg, T) the FFT form
     int fft_L = 0;                  // longest FFT template
+    // The FFT templates by overlap-save shape: [t0, t1) consecutive (sorted
+    // by length), their spectra from qspec_off (float2 units) in the lane's
+    // template-spectrum buffer.
+    struct FftGroup {
+        int t0, t1;
+        FftShape shape;
+        size_t qspec_off;
+    };
+    std::vector<FftGroup> fft;
     int max_want = 0;               // largest kNN bound set (k or 2k - 1)
     int max_k = 0;                  // largest k
     int64_t out_slots = 0;          // sum of k over the kNN templates
@@ -181,6 +190,7 @@ double fft_seconds_per_window(int L)
 }
 
 constexpr int kFusedChunk = 16;    // templates per fused FFT launch
+constexpr int kFftPieceBatch = 64;  // pieces / templates per forward transform launch
 constexpr uint64_t kChunkQueueMin = 1u << 20;  // chunk records a lane's fused pass starts with (280 B each)
 constexpr int kFftMinLength = 160;  // shortest template the cost model sends to the FFT form
 
@@ -199,7 +209,7 @@ double planned_seconds_per_window(const tmx_cfg &cfg, const CallPlan &plan, int 
 {
     const TmplDev &t = plan.tmpl[d];
     if (d >= plan.Tg)
-        return fft_seconds_per_window(plan.fft_L);
+        return fft_seconds_per_window(t.m);
     if (cfg.precision == TMX_PREC_FP32)
         return 8.0e-15 * t.m + 2.0e-12;
     return gemm_seconds_per_window(cfg.precision, t.kpad, plan.Tg);
@@ -280,18 +290,17 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
         if (cfg.fft_mode == TMX_FFT_ALWAYS) {
             plan.Tg = 0;
         } else {
-            // One FFT class per batch, sized by its longest FFT template (the
-            // first to move): every FFT template pays that class's cost.
+            // Each FFT template runs in the overlap-save shape of its own
+            // length (fft_shape_for), so pays that shape's cost.
             // Templates shorter than kFftMinLength stay on the matrix cores:
             // the FFT form's 64-window chunk filter needs a common core of
             // about a hundred samples to prune (single templates at 2^31
             // samples: m = 160 takes 8.7-11.8 ms on the FFT form against
             // 13.5-16.5 on the matrix cores; m = 128 is mixed).
-            const int L_class = plan.tmpl[count - 1].m;
             while (plan.Tg > 0) {
                 const TmplDev &t = plan.tmpl[plan.Tg - 1];
                 if (t.m < kFftMinLength ||
-                    !(fft_seconds_per_window(L_class) < 0.9 * gemm_seconds_per_window(cfg.precision, t.kpad, plan.Tg)))
+                    !(fft_seconds_per_window(t.m) < 0.9 * gemm_seconds_per_window(cfg.precision, t.kpad, plan.Tg)))
                     break;
                 plan.Tg--;
             }
@@ -299,6 +308,19 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
     }
     for (int d = plan.Tg; d < count; d++)
         plan.fft_L = std::max(plan.fft_L, plan.tmpl[d].m);
+    plan.fft.clear();
+    size_t qoff = 0;
+    for (int d = plan.Tg; d < count;) {
+        const FftShape sh = fft_shape_for(plan.tmpl[d].m);
+        int e = d + 1;
+        while (e < count && fft_shape_for(plan.tmpl[e].m).B == sh.B && fft_shape_for(plan.tmpl[e].m).Lmax == sh.Lmax)
+            e++;
+        plan.fft.push_back(CallPlan::FftGroup{d, e, sh, qoff});
+        // r2c transforms write whole batches of kFftPieceBatch templates
+        qoff += static_cast<size_t>((e - d + kFftPieceBatch - 1) / kFftPieceBatch) * kFftPieceBatch *
+                static_cast<size_t>(sh.B / 2 + 1);
+        d = e;
+    }
     set_groups(plan, cfg.precision == TMX_PREC_FP32 ? 1 : mfma_group_size());
     return 0;
 }
@@ -327,8 +349,9 @@ bool resident_ready(tmx_ctx *ctx, const tmx_cfg &cfg, const CallPlan &plan)
     if (cfg.precision != TMX_PREC_FP32 && plan.Tg > 0 &&
         !(ctx->rb_n == ctx->n && ctx->rb_len >= block_prefix_len(plan)))
         return false;
-    if (plan.T > plan.Tg && !find_fft_class(ctx, fft_shape_for(plan.fft_L)))
-        return false;
+    for (const auto &g : plan.fft)
+        if (!find_fft_class(ctx, g.shape))
+            return false;
     return true;
 }
 
@@ -351,7 +374,6 @@ int build_block_prefix(tmx_ctx *ctx, Lane &L, int need)
     return 0;
 }
 
-constexpr int kFftPieceBatch = 64;  // pieces / templates per forward transform launch
 
 // Piece spectra of the current waveform for templates up to Lt samples, in
 // the shape fft_shape_for(Lt) (B, Lmax; step = B - Lmax + 1); per piece the
@@ -428,8 +450,8 @@ int build_resident(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &pl
         if (rc)
             return rc;
     }
-    if (plan.T > plan.Tg) {
-        const int rc = build_fft_class(ctx, L, plan.fft_L);
+    for (const auto &g : plan.fft) {
+        const int rc = build_fft_class(ctx, L, plan.tmpl[g.t1 - 1].m);
         if (rc)
             return rc;
     }
@@ -475,48 +497,63 @@ int template_plan(Lane &L, const FftClass &f, hipfftHandle *out)
     return 0;
 }
 
-// Spectra of the batch's FFT templates [Tg, T) (reversed, zero padded).
-int fft_prepare(Lane &L, const CallPlan &plan, const FftClass &f)
+// Spectra of the batch's FFT templates [Tg, T) (reversed, zero padded to the
+// B of each one's shape group); slot[t] = t's index within its group, qmax
+// indexed by t - Tg.
+int fft_prepare(Lane &L, const CallPlan &plan, const std::vector<const FftClass *> &fcls)
 {
     const int F = plan.T - plan.Tg;
     if (F <= 0)
         return 0;
-    const int Falloc = ((F + kFftPieceBatch - 1) / kFftPieceBatch) * kFftPieceBatch;
-    LANE_HIP(L.qspec.reserve(sizeof(float2) * static_cast<size_t>(Falloc) * f.bins), "hipMalloc template spectra");
+    const CallPlan::FftGroup &last = plan.fft.back();
+    const size_t qlen = last.qspec_off + static_cast<size_t>((last.t1 - last.t0 + kFftPieceBatch - 1) /
+                                                               kFftPieceBatch) *
+                                             kFftPieceBatch * static_cast<size_t>(last.shape.B / 2 + 1);
+    int Bmax = 0;
+    for (const auto &g : plan.fft)
+        Bmax = std::max(Bmax, g.shape.B);
+    LANE_HIP(L.qspec.reserve(sizeof(float2) * qlen), "hipMalloc template spectra");
     LANE_HIP(L.qmax.reserve(sizeof(float) * F), "hipMalloc template spectra");
-    LANE_HIP(L.qin.reserve(sizeof(float) * static_cast<size_t>(kFftPieceBatch) * f.B), "hipMalloc template input");
+    LANE_HIP(L.qin.reserve(sizeof(float) * static_cast<size_t>(kFftPieceBatch) * Bmax), "hipMalloc template input");
     std::vector<int32_t> ids(F), slot(plan.T, -1);
-    for (int i = 0; i < F; i++) {
-        ids[i] = plan.Tg + i;
-        slot[plan.Tg + i] = i;
-    }
+    for (const auto &g : plan.fft)
+        for (int t = g.t0; t < g.t1; t++) {
+            ids[t - plan.Tg] = t;
+            slot[t] = t - g.t0;
+        }
     LANE_HIP(L.ids.reserve(sizeof(int32_t) * F), "hipMalloc FFT ids");
     LANE_HIP(L.slot.reserve(sizeof(int32_t) * plan.T), "hipMalloc FFT ids");
     LANE_HIP(hipMemcpyAsync(L.ids.p, ids.data(), sizeof(int32_t) * F, hipMemcpyHostToDevice, L.stream), "copy ids");
     LANE_HIP(hipMemcpyAsync(L.slot.p, slot.data(), sizeof(int32_t) * plan.T, hipMemcpyHostToDevice, L.stream),
              "copy slots");
-    hipfftHandle plan_r2c = 0;
-    if (int rc = template_plan(L, f, &plan_r2c))
-        return rc;
-    for (int i0 = 0; i0 < F; i0 += kFftPieceBatch) {
-        const int cnt = std::min(kFftPieceBatch, F - i0);
-        LANE_HIP(launch_tmpl_input(L.tmpl.as<TmplDev>(), L.ids.as<int32_t>() + i0, cnt, f.B, L.qz.as<float>(),
-                                   L.qin.as<float>(), L.stream),
-                 "FFT template input");
-        if (hipfftExecR2C(plan_r2c, L.qin.as<hipfftReal>(),
-                          reinterpret_cast<hipfftComplex *>(L.qspec.as<float2>() + static_cast<size_t>(i0) * f.bins)) !=
-            HIPFFT_SUCCESS)
-            return L.fail("hipFFT r2c of the templates failed", -2);
+    for (size_t gi = 0; gi < plan.fft.size(); gi++) {
+        const CallPlan::FftGroup &g = plan.fft[gi];
+        const FftClass &f = *fcls[gi];
+        hipfftHandle plan_r2c = 0;
+        if (int rc = template_plan(L, f, &plan_r2c))
+            return rc;
+        const int Fg = g.t1 - g.t0;
+        float2 *qs = L.qspec.as<float2>() + g.qspec_off;
+        for (int i0 = 0; i0 < Fg; i0 += kFftPieceBatch) {
+            const int cnt = std::min(kFftPieceBatch, Fg - i0);
+            LANE_HIP(launch_tmpl_input(L.tmpl.as<TmplDev>(), L.ids.as<int32_t>() + (g.t0 - plan.Tg) + i0, cnt, f.B,
+                                       L.qz.as<float>(), L.qin.as<float>(), L.stream),
+                     "FFT template input");
+            if (hipfftExecR2C(plan_r2c, L.qin.as<hipfftReal>(),
+                              reinterpret_cast<hipfftComplex *>(qs + static_cast<size_t>(i0) * f.bins)) !=
+                HIPFFT_SUCCESS)
+                return L.fail("hipFFT r2c of the templates failed", -2);
+        }
+        LANE_HIP(launch_spec_max(qs, Fg, f.bins, L.qmax.as<float>() + (g.t0 - plan.Tg), 1, L.stream),
+                 "template spectral maxima");
     }
-    LANE_HIP(launch_spec_max(L.qspec.as<float2>(), F, f.bins, L.qmax.as<float>(), 1, L.stream),
-             "template spectral maxima");
     return 0;
 }
 
 // Uploads the batch's templates, prepares them on the device, packs the
 // matrix-core operands and the FFT template spectra.
 int upload_and_prepare(Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const std::vector<float> &raw_packed,
-                       const FftClass *f)
+                       const std::vector<const FftClass *> &fcls)
 {
     const int T = plan.T;
     LANE_HIP(L.raw.reserve(sizeof(float) * (plan.raw_len + 64)), "hipMalloc templates");
@@ -539,7 +576,7 @@ int upload_and_prepare(Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const 
         LANE_HIP(launch_mfma_pack(L.tmpl.as<TmplDev>(), plan.Tg, L.qz.as<float>(), cfg.precision, L.afrag.p, L.stream),
                  "MFMA operand packing");
     }
-    return f ? fft_prepare(L, plan, *f) : 0;
+    return fft_prepare(L, plan, fcls);
 }
 
 void raise_hint(std::atomic<uint64_t> &hint, uint64_t v)
@@ -579,50 +616,68 @@ CandOut cand_out(Lane &L)
                    L.cand_count.as<unsigned long long>(), L.cand_cap};
 }
 
-int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f, int mode, int stride)
+// The chunk queue of the fused passes of one candidate pass: room for every
+// launch's records (launches of at most kFusedChunk templates, each followed
+// by the per-window pass that empties the queue), and the overflow counter
+// nrec[1] cleared once for the whole pass.
+int reserve_chunk_queue(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const std::vector<const FftClass *> &fcls,
+                        int mode, const std::vector<int> &strides)
 {
-    const int F = plan.T - plan.Tg;
-    if (F <= 0)
-        return 0;
+    uint64_t want = std::max<uint64_t>({L.chunk_need, ctx->chunk_hint.load(), uint64_t{1024}});
+    for (size_t gi = 0; gi < plan.fft.size(); gi++) {
+        const FftClass &f = *fcls[gi];
+        if (!f.fused)
+            continue;
+        const int Fc = std::min(plan.fft[gi].t1 - plan.fft[gi].t0, kFusedChunk);
+        const uint64_t stride = static_cast<uint64_t>(strides[gi]);
+        const uint64_t nwork = (f.P + stride - 1) / stride;
+        // Seed passes queue at most one record per wave (16 per piece and
+        // template) and must keep them all: a template without seed entries
+        // would keep an infinite bound.  Main passes: room for kChunkQueueMin
+        // records (or every chunk a launch can produce, if fewer), more once
+        // a pass has overflowed.
+        const uint64_t cw = static_cast<uint64_t>(fft_chunk_width());
+        const uint64_t most = static_cast<uint64_t>(Fc) * nwork * ((f.step + cw - 1) / cw + 1);
+        want = std::max(want, mode == kModeSeed ? nwork * static_cast<uint64_t>(Fc) * 16 : std::min(most, kChunkQueueMin));
+    }
+    if (want > L.chunk_cap) {
+        L.chunk_cap = 0;
+        LANE_HIP(L.chunk_rec.reserve(fft_chunk_rec_bytes() * want), "hipMalloc chunk queue");
+        LANE_HIP(L.chunk_dt.reserve(sizeof(float) * fft_chunk_width() * want), "hipMalloc chunk queue");
+        L.chunk_cap = want;
+        if (L.chunk_need > 0)
+            raise_hint(ctx->chunk_hint, want);
+    }
+    LANE_HIP(L.chunk_n.reserve(2 * sizeof(unsigned long long)), "hipMalloc chunk queue");
+    LANE_HIP(launch_zero_u64(L.chunk_n.as<unsigned long long>() + 1, L.stream), "reset chunk counter");
+    return 0;
+}
+
+// The FFT form of a pass for one shape group, over every `stride`-th piece
+// (the chunk queue of the fused lengths reserved beforehand).
+int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::FftGroup &g, const FftClass &f,
+                 int mode, int stride)
+{
+    const int F = g.t1 - g.t0;
     bool any = false;
-    for (int i = 0; i < F && !any; i++)
-        any = static_cast<uint64_t>(plan.tmpl[plan.Tg + i].m) <= ctx->n;
+    for (int t = g.t0; t < g.t1 && !any; t++)
+        any = static_cast<uint64_t>(plan.tmpl[t].m) <= ctx->n && !plan.tmpl[t].ea;
     if (!any)
         return 0;
     Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
     const double kfft = fft_error_factor(f.B);
+    const float2 *qs = L.qspec.as<float2>() + g.qspec_off;
+    float *qmax = L.qmax.as<float>() + (g.t0 - plan.Tg);
     if (f.fused) {
-        // Seed passes queue at most one record per wave (16 per piece and
-        // template) and must keep them all: a template without seed entries
-        // would keep an infinite bound.
-        const uint64_t nwork = (f.P + static_cast<uint64_t>(stride) - 1) / static_cast<uint64_t>(stride);
-        const uint64_t seed_need = mode == kModeSeed ? nwork * static_cast<uint64_t>(F) * 16 : 0;
-        // Main passes: room for kChunkQueueMin records (or every chunk a
-        // launch can produce, if fewer), more once a pass has overflowed.
-        const uint64_t cw = static_cast<uint64_t>(fft_chunk_width());
-        const uint64_t most = static_cast<uint64_t>(std::min(F, kFusedChunk)) * nwork * ((f.step + cw - 1) / cw + 1);
-        const uint64_t want = std::max({L.chunk_need, seed_need, std::min(most, kChunkQueueMin),
-                                        ctx->chunk_hint.load(), uint64_t{1024}});
-        if (want > L.chunk_cap) {
-            L.chunk_cap = 0;
-            LANE_HIP(L.chunk_rec.reserve(fft_chunk_rec_bytes() * want), "hipMalloc chunk queue");
-            LANE_HIP(L.chunk_dt.reserve(sizeof(float) * fft_chunk_width() * want), "hipMalloc chunk queue");
-            L.chunk_cap = want;
-            if (L.chunk_need > 0)
-                raise_hint(ctx->chunk_hint, want);
-        }
-        LANE_HIP(L.chunk_n.reserve(2 * sizeof(unsigned long long)), "hipMalloc chunk queue");
-        LANE_HIP(launch_zero_u64(L.chunk_n.as<unsigned long long>() + 1, L.stream), "reset chunk counter");
         // Launches of at most kFusedChunk templates: their spectra (8 (B/2 + 1)
         // bytes each) stay in L2 while the pieces stream past.
         for (int c0 = 0; c0 < F; c0 += kFusedChunk) {
             const int Fc = std::min(kFusedChunk, F - c0);
-            LANE_HIP(launch_fft_fused(w, L.tmpl.as<TmplDev>(), plan.Tg + c0, Fc, f.spectra.as<float2>(), f.bins,
-                                      L.qspec.as<float2>() + static_cast<size_t>(c0) * f.bins, L.qmax.as<float>() + c0,
-                                      f.pinfo.as<float4>(), f.pnorm.as<float2>(), f.table.as<float2>(), f.P, f.step, stride,
-                                      kfft, mode, L.chunk_rec.p, L.chunk_dt.as<float>(),
-                                      L.chunk_n.as<unsigned long long>(), L.chunk_cap, cand_out(L),
-                                      L.cus > 0 ? L.cus : 256, L.stream),
+            LANE_HIP(launch_fft_fused(w, L.tmpl.as<TmplDev>(), g.t0 + c0, Fc, f.spectra.as<float2>(), f.bins,
+                                      qs + static_cast<size_t>(c0) * f.bins, qmax + c0, f.pinfo.as<float4>(),
+                                      f.pnorm.as<float2>(), f.table.as<float2>(), f.P, f.step, stride, kfft, mode,
+                                      L.chunk_rec.p, L.chunk_dt.as<float>(), L.chunk_n.as<unsigned long long>(),
+                                      L.chunk_cap, cand_out(L), L.cus > 0 ? L.cus : 256, L.stream),
                      "fused FFT pass");
         }
         return 0;
@@ -630,7 +685,7 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
     std::vector<FftPair> pairs;
     for (uint64_t p = 0; p < f.P; p += static_cast<uint64_t>(stride))
         for (int i = 0; i < F; i++) {
-            const int t = plan.Tg + i;
+            const int t = g.t0 + i;
             const uint64_t m = static_cast<uint64_t>(plan.tmpl[t].m);
             if (m <= ctx->n && !plan.tmpl[t].ea && p * f.step < ctx->n - m + 1)
                 pairs.push_back(FftPair{static_cast<uint32_t>(t), static_cast<uint32_t>(p)});
@@ -657,14 +712,14 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const FftClass &f,
     for (size_t b0 = 0; b0 < pairs.size(); b0 += NB) {
         const int cnt = static_cast<int>(std::min<size_t>(NB, pairs.size() - b0));
         const FftPair *pp = L.pairs.as<FftPair>() + b0;
-        LANE_HIP(launch_product(pp, cnt, f.spectra.as<float2>(), L.qspec.as<float2>(), L.slot.as<int32_t>(), f.bins,
-                                L.Y.as<float2>(), L.stream),
+        LANE_HIP(launch_product(pp, cnt, f.spectra.as<float2>(), qs, L.slot.as<int32_t>(), f.bins, L.Y.as<float2>(),
+                                L.stream),
                  "FFT product");
         if (hipfftExecC2R(L.inv, reinterpret_cast<hipfftComplex *>(L.Y.as<float2>()), L.y.as<hipfftReal>()) !=
             HIPFFT_SUCCESS)
             return L.fail("hipFFT c2r failed", -2);
         LANE_HIP(launch_fft_filter(w, L.tmpl.as<TmplDev>(), pp, cnt, L.y.as<float>(), f.B, f.step, f.rb.as<float2>(),
-                                   f.B + 1, f.pinfo.as<float4>(), f.pnorm.as<float2>(), L.qmax.as<float>(),
+                                   f.B + 1, f.pinfo.as<float4>(), f.pnorm.as<float2>(), qmax,
                                    L.slot.as<int32_t>(), kfft, mode, cand_out(L), L.stream),
                  "FFT filter");
     }
@@ -787,8 +842,8 @@ int fft_seed_stride(const FftClass *f, uint64_t target, int max_want, bool defau
 
 // One candidate pass (seed or main) over every template; returns the entry
 // count through *count (may exceed the capacity).
-int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const FftClass *f, int mode,
-             int block_stride, uint64_t *count, int fft_stride = 1)
+int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const std::vector<const FftClass *> &fcls,
+             int mode, int block_stride, uint64_t *count, const std::vector<int> *fft_strides = nullptr)
 {
     Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
     LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
@@ -809,10 +864,22 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
                      "MFMA scan");
         }
     }
-    if (f && fft_any) {
-        const int rc = run_fft_pass(ctx, L, plan, *f, mode, fft_stride);
-        if (rc)
-            return rc;
+    bool fused = false;
+    if (fft_any) {
+        const std::vector<int> ones(plan.fft.size(), 1);
+        const std::vector<int> &strides = fft_strides ? *fft_strides : ones;
+        for (const FftClass *f : fcls)
+            fused |= f->fused;
+        if (fused) {
+            const int rc = reserve_chunk_queue(ctx, L, plan, fcls, mode, strides);
+            if (rc)
+                return rc;
+        }
+        for (size_t gi = 0; gi < plan.fft.size(); gi++) {
+            const int rc = run_fft_pass(ctx, L, plan, plan.fft[gi], *fcls[gi], mode, strides[gi]);
+            if (rc)
+                return rc;
+        }
     }
     if (mode == kModeMain && !plan.scan_ids.empty())
         LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
@@ -822,7 +889,6 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
     unsigned long long c = 0, nchunk = 0;
     LANE_HIP(hipMemcpyAsync(&c, L.cand_count.p, sizeof(c), hipMemcpyDeviceToHost, L.stream), "read counter");
-    const bool fused = f && f->fused && fft_any;
     if (fused)
         LANE_HIP(hipMemcpyAsync(&nchunk, L.chunk_n.as<unsigned long long>() + 1, sizeof(nchunk),
                                 hipMemcpyDeviceToHost, L.stream),
@@ -836,11 +902,11 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
 }
 
 // Main pass with buffer growth on overflow.
-int run_main(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const FftClass *f, uint64_t *count,
-             int *retries)
+int run_main(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan,
+             const std::vector<const FftClass *> &fcls, uint64_t *count, int *retries)
 {
     for (;;) {
-        int rc = run_pass(ctx, L, cfg, plan, f, kModeMain, 1, count);
+        int rc = run_pass(ctx, L, cfg, plan, fcls, kModeMain, 1, count);
         if (rc)
             return rc;
         if (L.chunk_need > 0) {
@@ -974,8 +1040,10 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
                  const std::vector<float> &raw_packed, std::vector<QueryOut> &out, tmx_stats &st)
 {
     const int T = plan.T;
-    const FftClass *f = plan.T > plan.Tg ? find_fft_class(ctx, fft_shape_for(plan.fft_L)) : nullptr;
-    int rc = upload_and_prepare(L, cfg, plan, raw_packed, f);
+    std::vector<const FftClass *> fcls;
+    for (const auto &g : plan.fft)
+        fcls.push_back(find_fft_class(ctx, g.shape));
+    int rc = upload_and_prepare(L, cfg, plan, raw_packed, fcls);
     if (rc)
         return rc;
     const uint64_t def_cap = cfg.cand_capacity > 0 ? static_cast<uint64_t>(cfg.cand_capacity)
@@ -998,9 +1066,12 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
         uint64_t stride = (blocks * 4) / block_target;
         if (stride < 1)
             stride = 1;
+        std::vector<int> fft_strides;
+        for (const FftClass *f : fcls)
+            fft_strides.push_back(fft_seed_stride(f, target, plan.max_want, cfg.seed_target <= 0));
         uint64_t nseed = 0;
-        if ((rc = run_pass(ctx, L, cfg, plan, f, kModeSeed, static_cast<int>(std::min<uint64_t>(stride, 1u << 30)),
-                           &nseed, fft_seed_stride(f, target, plan.max_want, cfg.seed_target <= 0))))
+        if ((rc = run_pass(ctx, L, cfg, plan, fcls, kModeSeed, static_cast<int>(std::min<uint64_t>(stride, 1u << 30)),
+                           &nseed, &fft_strides)))
             return rc;
         if (nseed > L.cand_cap)
             nseed = L.cand_cap;  // any subset of seed entries still gives a valid bound
@@ -1021,15 +1092,15 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     const double t2 = now_s();
 
     uint64_t ncand = 0;
-    if ((rc = run_main(ctx, L, cfg, plan, f, &ncand, &st.retries)))
+    if ((rc = run_main(ctx, L, cfg, plan, fcls, &ncand, &st.retries)))
         return rc;
     st.kernel_ms_main = L.last_pass_ms;
     st.flops_main = plan.flops;
     st.fft_templates = 0;
     for (int d = plan.Tg; d < plan.T; d++)
         st.fft_templates += plan.tmpl[d].ea ? 0 : 1;
-    st.fft_block = f ? f->B : 0;
-    st.fft_lmax = f ? f->Lmax : 0;
+    st.fft_block = fcls.empty() ? 0 : fcls.back()->B;
+    st.fft_lmax = fcls.empty() ? 0 : fcls.back()->Lmax;
     const double t3 = now_s();
     st.seconds_main = t3 - t2;
     st.candidates = static_cast<int64_t>(ncand);
@@ -1437,7 +1508,7 @@ int prepare_one(tmx_ctx *ctx, const float *tmpl, int32_t m)
     std::vector<float> raw_packed;
     int rc = build_plan(ctx, L, cfg, &q, 1, plan, raw_packed);
     if (!rc)
-        rc = upload_and_prepare(L, cfg, plan, raw_packed, nullptr);
+        rc = upload_and_prepare(L, cfg, plan, raw_packed, std::vector<const FftClass *>{});
     if (rc)
         ctx->fail(L.err, rc);
     return rc;
