@@ -437,3 +437,27 @@ def test_fft_shape_groups_in_one_batch(oracle):
         tt, tp, td = oracle.scan_threshold(x, tm, 0.6, 3)
         assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
         assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_threshold_rows_outlive_result():
+    """Threshold rows are views of the library's buffers, freed with the last
+    array: arrays kept after the result object is gone still hold the rows
+    (compared with a second call after heavy allocation)."""
+    import gc
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(200_000, 59, events=30)
+    tm = synth.cut_templates(x, onsets, [200, 700], rng)
+    with Context(precision="bf16x3") as ctx:
+        ctx.load_waveform(x)
+        _log("rows-lifetime")
+        r = ctx.threshold(tm, 0.3, 0)
+        pos, d, tt = r.pos, r.d, r.tmpl
+        assert pos.size > 1000
+        del r
+        gc.collect()
+        junk = [np.ones(1 << 20, np.float64) for _ in range(64)]
+        r2 = ctx.threshold(tm, 0.3, 0)
+        del junk
+        assert np.array_equal(pos, r2.pos) and np.array_equal(tt, r2.tmpl)
+        assert np.array_equal(d.view(np.uint32), r2.d.view(np.uint32))

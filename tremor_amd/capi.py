@@ -158,6 +158,26 @@ class ThresholdResult:
     stats: dict
 
 
+class _HitsOwner:
+    """Holds a tmx_hits result until every array viewing it is gone."""
+
+    def __init__(self, L, hits):
+        self._L = L
+        self._hits = hits
+
+    def __del__(self):
+        if self._hits is not None:
+            self._L.tmx_hits_free(self._hits)
+            self._hits = None
+
+
+def _view(ptr, ctype, n, owner):
+    """A numpy array over n elements at ptr, keeping owner alive."""
+    buf = (ctype * n).from_address(ctypes.addressof(ptr.contents))
+    buf._owner = owner
+    return np.ctypeslib.as_array(buf)
+
+
 class Context:
     """One device context: a resident waveform and search calls on it."""
 
@@ -240,20 +260,18 @@ class Context:
         st = Stats()
         self._check(self._L.tmx_threshold(self._h, _p(flat, _f32p), _p(lengths, _i32p), _p(offsets, _u64p), T,
                                           float(rho), merge, ctypes.byref(out), ctypes.byref(st)))
-        try:
-            h = out.contents
-            n = int(h.count)
-            if n:
-                res = ThresholdResult(np.ctypeslib.as_array(h.tmpl, (n,)).copy(),
-                                      np.ctypeslib.as_array(h.counter, (n,)).copy(),
-                                      np.ctypeslib.as_array(h.pos, (n,)).copy(),
-                                      np.ctypeslib.as_array(h.d, (n,)).copy(), st.as_dict())
-            else:
-                res = ThresholdResult(np.zeros(0, np.int32), np.zeros(0, np.int64), np.zeros(0, np.uint64),
-                                      np.zeros(0, np.float32), st.as_dict())
-        finally:
+        h = out.contents
+        n = int(h.count)
+        if not n:
             self._L.tmx_hits_free(out)
-        return res
+            return ThresholdResult(np.zeros(0, np.int32), np.zeros(0, np.int64), np.zeros(0, np.uint64),
+                                   np.zeros(0, np.float32), st.as_dict())
+        # The rows stay in the library's buffers (no copy of what can be
+        # gigabytes); they are freed with the last of the four arrays.
+        owner = _HitsOwner(self._L, out)
+        return ThresholdResult(_view(h.tmpl, ctypes.c_int32, n, owner), _view(h.counter, ctypes.c_int64, n, owner),
+                               _view(h.pos, ctypes.c_uint64, n, owner), _view(h.d, ctypes.c_float, n, owner),
+                               st.as_dict())
 
     def distance_batch(self, template, starts):
         t = np.ascontiguousarray(template, dtype=np.float32)

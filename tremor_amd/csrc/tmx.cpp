@@ -22,6 +22,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ctx.hpp"
@@ -577,6 +578,29 @@ int upload_and_prepare(Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const 
                  "MFMA operand packing");
     }
     return fft_prepare(L, plan, fcls);
+}
+
+// fn(i) for i in [0, n) on up to 16 host threads when the work (rows to
+// move) is large enough to pay for them; items must be independent.
+template <class Fn>
+void parallel_items(int n, size_t work, const Fn &fn)
+{
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nt = work < (1u << 20) ? 1 : static_cast<int>(std::min<unsigned>(hw, static_cast<unsigned>(n)));
+    if (nt <= 1) {
+        for (int i = 0; i < n; i++)
+            fn(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int k = 0; k < nt; k++)
+        th.emplace_back([&] {
+            for (int i = next++; i < n; i = next++)
+                fn(i);
+        });
+    for (auto &t : th)
+        t.join();
 }
 
 void raise_hint(std::atomic<uint64_t> &hint, uint64_t v)
@@ -1169,13 +1193,13 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     }
     LANE_HIP(hipStreamSynchronize(L.stream), "results");
     out.assign(T, QueryOut{});
-    for (int d = 0; d < T; d++) {
+    parallel_items(T, offs.empty() ? 0 : offs[T], [&](int d) {
         const TmplDev &t = plan.tmpl[d];
         QueryOut &o = out[plan.perm[d]];
         if (t.k > 0) {
             o.pos.assign(hp.begin() + t.out_off, hp.begin() + t.out_off + t.k);
             o.d.assign(hd.begin() + t.out_off, hd.begin() + t.out_off + t.k);
-            continue;
+            return;
         }
         // Threshold hits by position; greedy merge of the reference
         // (SkipSequential.cpp:279-290): drop a hit closer than merge + 1 to
@@ -1196,7 +1220,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
             last = pos;
             has_last = true;
         }
-    }
+    });
     st.seconds_select = now_s() - t4;
     return 0;
 }
@@ -1461,23 +1485,26 @@ int tmx_threshold(tmx_ctx *ctx, const float *templates, const int32_t *lengths, 
     std::vector<QueryOut> res;
     if ((rc = direct_batch(ctx, q, res, stats)))
         return rc;
-    size_t rows = 0;
-    for (const auto &r : res)
-        rows += r.pos.size();
+    std::vector<size_t> first(count + 1, 0);
+    for (int i = 0; i < count; i++)
+        first[i + 1] = first[i] + res[i].pos.size();
+    const size_t rows = first[count];
     tmx_hits *h = static_cast<tmx_hits *>(std::calloc(1, sizeof(tmx_hits)));
     const size_t hn = rows ? rows : 1;
     h->tmpl = static_cast<int32_t *>(std::malloc(sizeof(int32_t) * hn));
     h->counter = static_cast<int64_t *>(std::malloc(sizeof(int64_t) * hn));
     h->pos = static_cast<uint64_t *>(std::malloc(sizeof(uint64_t) * hn));
     h->d = static_cast<float *>(std::malloc(sizeof(float) * hn));
-    size_t r = 0;
-    for (int i = 0; i < count; i++)
-        for (size_t e = 0; e < res[i].pos.size(); e++, r++) {
-            h->tmpl[r] = i;
-            h->counter[r] = static_cast<int64_t>(e);
-            h->pos[r] = res[i].pos[e];
-            h->d[r] = res[i].d[e];
+    parallel_items(count, rows, [&](int i) {
+        const size_t r0 = first[i];
+        const size_t ni = res[i].pos.size();
+        for (size_t e = 0; e < ni; e++) {
+            h->tmpl[r0 + e] = i;
+            h->counter[r0 + e] = static_cast<int64_t>(e);
         }
+        std::memcpy(h->pos + r0, res[i].pos.data(), sizeof(uint64_t) * ni);
+        std::memcpy(h->d + r0, res[i].d.data(), sizeof(float) * ni);
+    });
     h->count = static_cast<int64_t>(rows);
     *out = h;
     return 0;
