@@ -461,3 +461,30 @@ def test_threshold_rows_outlive_result():
         del junk
         assert np.array_equal(pos, r2.pos) and np.array_equal(tt, r2.tmpl)
         assert np.array_equal(d.view(np.uint32), r2.d.view(np.uint32))
+
+
+def test_fft_shapes_short_waveform(oracle):
+    """The FFT form on a waveform shorter than one piece of every shape (a
+    single partial piece each), templates close to the waveform's length,
+    missing samples and a constant stretch: the oracle's answers."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    rng = np.random.default_rng(61)
+    x = synth.bandpassed_noise(9000, rng)
+    synth.inject_events(x, 6, rng)
+    x[rng.integers(0, x.size, size=15)] = synth.MISSING
+    x[4000:4300] = 0.25
+    tm = [x[100:400].copy(), x[1000:2500].copy(), x[2000:5000].copy(), x[500:8600].copy(),
+          rng.standard_normal(8990).astype(np.float32)]
+    with Context(precision="bf16x3", fft="always") as ctx:
+        ctx.load_waveform(x)
+        for k, merge in ((4, 0), (2, 25)):
+            _log("fft-short", k=k, merge=merge)
+            pos, d, st = ctx.knn(tm, k, merge)
+            print("  stats", st, flush=True)
+            po, do, _ = oracle.scan_knn(x, tm, k, merge)
+            assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
+        r = ctx.threshold(tm, 0.3, 0)
+        tt, tp, td = oracle.scan_threshold(x, tm, 0.3, 0)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
