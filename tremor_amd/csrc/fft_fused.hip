@@ -56,13 +56,13 @@ struct Geom {
     static constexpr int kB = 2 * kM;                        // real length B
     static constexpr int kT = kM / 16;                       // threads per workgroup
     static constexpr int kStepMax = kB - kB / 16 + 1;        // windows per piece, templates up to B / 16
-    static constexpr int kLds = kM + (kM >> 4) + 2;          // padded complex slots (index M included)
+    static constexpr int kLds = kM + (kM >> 3) + 4;          // padded complex slots (index M included)
     static constexpr int kPerCU = LOG2M == 14 ? 1 : 2;       // workgroups per CU
     static_assert((kStepMax + 63) / 64 + 1 <= kT, "one thread per 64-output chunk of a piece");
     static_assert(LOG2M == 13 || LOG2M == 14, "B = 2^14 or 2^15");
 };
 
-__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int pad(int i) { return i + 2 * (i >> 4); }
 
 // Outputs per chunk of the filter (two padded LDS rows), and dot-product
 // slots per queued chunk record.
@@ -483,29 +483,31 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
             const uint64_t ca = base + static_cast<uint64_t>(i_first + cnt - 1);
             const uint64_t cb = base + static_cast<uint64_t>(i_first + m);
             // 3. Filter.  Output n (dt of window base + n - (m - 1), times B)
-            // is at float n + 2 (n >> 5) of the padded array.  The chunk's
+            // is at float n + 4 (n >> 5) of the padded array.  The chunk's
             // largest output, and a sum that is non-finite iff an output is.
             const float invB = 1.0f / static_cast<float>(kB);
             float vmax = -INFINITY, vsum = 0.0f;
             if (cnt == kChunkW) {
                 // Two padded rows of 32 floats; half the lanes take them in
                 // the other order (fewer LDS bank conflicts).
-                const int f0 = 68 * (c0 + tid);
+                const int f0 = 72 * (c0 + tid);
                 const int sw = (tid >> 4) & 1;
 #pragma unroll 1
                 for (int hr = 0; hr < 2; hr++) {
-                    const float2 *row = reinterpret_cast<const float2 *>(fdata + f0 + 34 * (hr ^ sw));
+                    const float4 *row = reinterpret_cast<const float4 *>(fdata + f0 + 36 * (hr ^ sw));
 #pragma unroll 8
-                    for (int q = 0; q < 16; q++) {
-                        const float2 v = row[q];
-                        vmax = fmaxf(vmax, fmaxf(v.x, v.y));
+                    for (int q = 0; q < 8; q++) {
+                        const float4 v = row[q];
+                        vmax = fmaxf(vmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
                         vsum += v.x;
                         vsum += v.y;
+                        vsum += v.z;
+                        vsum += v.w;
                     }
                 }
             } else {
                 for (int n = n_first; n < n_end; n++) {
-                    const float v = fdata[n + 2 * (n >> 5)];
+                    const float v = fdata[n + 4 * (n >> 5)];
                     vmax = fmaxf(vmax, v);
                     vsum += v;
                 }
@@ -538,18 +540,18 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
                             float *dst = rec_dt + slot * kChunkW;
                             if (cnt == kChunkW) {
                                 // Two padded rows of 32 outputs, 16 bytes per store.
-                                const float2 *row = reinterpret_cast<const float2 *>(fdata + 68 * (c0 + tid));
+                                const float2 *row = reinterpret_cast<const float2 *>(fdata + 72 * (c0 + tid));
                                 float4 *d4 = reinterpret_cast<float4 *>(dst);
 #pragma unroll 4
                                 for (int q = 0; q < 16; q++) {
-                                    const float2 a = row[(q >> 3) * 17 + 2 * (q & 7)];
-                                    const float2 b = row[(q >> 3) * 17 + 2 * (q & 7) + 1];
+                                    const float2 a = row[(q >> 3) * 18 + 2 * (q & 7)];
+                                    const float2 b = row[(q >> 3) * 18 + 2 * (q & 7) + 1];
                                     d4[q] = make_float4(a.x * invB, a.y * invB, b.x * invB, b.y * invB);
                                 }
                             } else {
                                 for (int l = 0; l < cnt; l++) {
                                     const int n = n_first + l;
-                                    dst[l] = fdata[n + 2 * (n >> 5)] * invB;
+                                    dst[l] = fdata[n + 4 * (n >> 5)] * invB;
                                 }
                             }
                         }
@@ -579,7 +581,7 @@ __global__ __launch_bounds__(Geom<LOG2M>::kT, 4) void k_fft_fused(  // 4 waves p
                 if (lane == who && g > -INFINITY) {
                     int arg = 0;
                     for (int n = n_end - 1; n >= n_first; n--)
-                        if (fdata[n + 2 * (n >> 5)] == vmax)
+                        if (fdata[n + 4 * (n >> 5)] == vmax)
                             arg = n - n_first;
                     const unsigned long long slot = atomicAdd(nrec, 1ULL);
                     if (slot < rec_cap) {
