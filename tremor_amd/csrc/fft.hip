@@ -7,16 +7,23 @@
 // The FFT dot products only filter the windows: every window whose distance
 // may be within the bound, allowing for the FFT error, is recomputed exactly.
 //
-// Transforms are hipFFT (FP32, unnormalised); the kernels here prepare the
-// pieces and templates, take the spectral norms for the error bound, form
-// the products and run the fused candidate filter on the inverse transform.
+// Transforms are hipFFT (FP32, unnormalised): r2c forward, and the real
+// inverse of length B as a complex inverse of length B / 2 whose input the
+// product kernel forms (k_product_z).  The kernels here prepare the pieces
+// and templates, take the spectral norms for the error bound, form the
+// products and run the fused candidate filter on the inverse transform.
 //
 // Error bound.  With X = F x~ (piece), H = F h (reversed template), y = F^-1
 // (X .* H) / B, and forward / inverse transforms of relative 2-norm error
 // eps = c log2(B) u each, the product rounding 2u, every output satisfies
 //   |y_fft - y| <= (2 eps + 2u) ||x~||_2 ||H||_inf + eps ||h||_2 ||X||_inf
 // (2-norm bounds of the error vector, hence of each element).  c = 10 with
-// a further factor 4: the transform library is treated as a black box.
+// a further factor 4: the transform library is treated as a black box.  The
+// inverse is the Z step (a table twiddle rounded from FP64, one complex
+// product and three additions per bin: a few u of relative 2-norm error,
+// since (Y[k], conj Y[M-k]) -> (Z[k], conj Z[M-k]) is sqrt 2 times a unitary
+// map) followed by a transform of length B / 2, within c log2(B) u =
+// c log2(B / 2) u + c u.
 #include <hipfft/hipfft.h>
 
 #include "filter.hpp"
@@ -114,18 +121,36 @@ __global__ void k_piece_norm(const double *__restrict__ norm2, float2 *__restric
         pnorm[p].x = static_cast<float>(sqrt(norm2[p]) * (1.0 + 1e-6));
 }
 
-// Y[b] = X[p] .* H[t] for the pairs of one batch.
-__global__ void k_product(const FftPair *__restrict__ pairs, const float2 *__restrict__ spectra,
-                          const float2 *__restrict__ qspec, const int32_t *__restrict__ slot, int bins,
-                          float2 *__restrict__ Y)
+// The inverse real transform of Y = X[p] .* H[t] (B/2 + 1 bins, B = 2M) as an
+// M-point complex inverse transform: with
+//   Z[k] = (Y[k] + conj Y[M-k]) + i e^{i pi k / M} (Y[k] - conj Y[M-k]),  k < M,
+// z = sum_k Z[k] e^{2 pi i r k / M} gives y[2r] = Re z[r], y[2r+1] = Im z[r],
+// the unnormalised real inverse of length B (the layout a c2r writes).  One
+// thread per bin pair (k, M - k), so the product is formed once per bin;
+// E[k] = e^{i pi k / M} comes from the FP64-rounded table.
+__global__ void k_product_z(const FftPair *__restrict__ pairs, const float2 *__restrict__ spectra,
+                            const float2 *__restrict__ qspec, const int32_t *__restrict__ slot, int bins,
+                            const float2 *__restrict__ E, float2 *__restrict__ Z)
 {
     const FftPair pr = pairs[blockIdx.y];
+    const int M = bins - 1;
     const float2 *X = spectra + static_cast<uint64_t>(pr.p) * bins;
     const float2 *H = qspec + static_cast<uint64_t>(slot[pr.t]) * bins;
-    float2 *out = Y + static_cast<uint64_t>(blockIdx.y) * bins;
-    for (int k = blockIdx.x * kThreads + threadIdx.x; k < bins; k += gridDim.x * kThreads) {
-        const float2 a = X[k], b = H[k];
-        out[k] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    float2 *out = Z + static_cast<uint64_t>(blockIdx.y) * M;
+    const auto cmul = [](float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    // Z[k] from Y[k] and Y[M - k]: A + i E[k] D with A = Yk + conj Ym, D = Yk - conj Ym.
+    const auto zval = [&](float2 yk, float2 ym, float2 e) {
+        const float2 A = make_float2(yk.x + ym.x, yk.y - ym.y);
+        const float2 D = make_float2(yk.x - ym.x, yk.y + ym.y);
+        const float2 ED = cmul(e, D);
+        return make_float2(A.x - ED.y, A.y + ED.x);
+    };
+    for (int k = blockIdx.x * kThreads + threadIdx.x; k <= M / 2; k += gridDim.x * kThreads) {
+        const int km = M - k;
+        const float2 yk = cmul(X[k], H[k]), ym = cmul(X[km], H[km]);
+        out[k] = zval(yk, ym, E[k]);
+        if (k > 0 && km != k)
+            out[km] = zval(ym, yk, E[km]);
     }
 }
 
@@ -289,13 +314,15 @@ hipError_t launch_tmpl_input(const TmplDev *tmpl, const int32_t *ids, int count,
     return hipGetLastError();
 }
 
-hipError_t launch_product(const FftPair *pairs, int npairs, const float2 *spectra, const float2 *qspec,
-                          const int32_t *slot, int bins, float2 *Y, hipStream_t stream)
+hipError_t launch_product_z(const FftPair *pairs, int npairs, const float2 *spectra, const float2 *qspec,
+                            const int32_t *slot, int bins, const float2 *E, float2 *Z, hipStream_t stream)
 {
     if (npairs <= 0)
         return hipSuccess;
-    const int gx = (bins + kThreads * 8 - 1) / (kThreads * 8);
-    hipLaunchKernelGGL(k_product, dim3(gx, npairs), dim3(kThreads), 0, stream, pairs, spectra, qspec, slot, bins, Y);
+    const int half = (bins - 1) / 2 + 1;
+    const int gx = (half + kThreads * 8 - 1) / (kThreads * 8);
+    hipLaunchKernelGGL(k_product_z, dim3(gx, npairs), dim3(kThreads), 0, stream, pairs, spectra, qspec, slot, bins, E,
+                       Z);
     return hipGetLastError();
 }
 

@@ -68,8 +68,8 @@ hipError_t launch_piece_norm(const double *norm2, float2 *pnorm, uint64_t P, hip
 hipError_t launch_spec_max(const float2 *spec, int rows, int bins, float *out, int stride_out, hipStream_t stream);
 hipError_t launch_tmpl_input(const TmplDev *tmpl, const int32_t *ids, int count, int B, const float *qz, float *out,
                              hipStream_t stream);
-hipError_t launch_product(const FftPair *pairs, int npairs, const float2 *spectra, const float2 *qspec,
-                          const int32_t *slot, int bins, float2 *Y, hipStream_t stream);
+hipError_t launch_product_z(const FftPair *pairs, int npairs, const float2 *spectra, const float2 *qspec,
+                            const int32_t *slot, int bins, const float2 *E, float2 *Z, hipStream_t stream);
 hipError_t launch_fft_filter(const Waveform &w, const TmplDev *tmpl, const FftPair *pairs, int npairs, const float *y,
                              int B, int step, const float2 *rb, int rb_len, const float4 *pinfo, const float2 *pnorm,
                              const float *qmax, const int32_t *slot, double kfft, int mode, const CandOut &co,
@@ -402,8 +402,9 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
     if (!f.fused)
         LANE_HIP(f.rb.reserve(sizeof(float2) * f.P * static_cast<uint64_t>(B + 1)), "hipMalloc FFT prefix pairs");
     LANE_HIP(f.work_real.reserve(sizeof(float) * static_cast<size_t>(kFftPieceBatch) * B), "hipMalloc FFT input");
-    if (f.fused) {
-        // E[t] = e^{i pi t / M}, M = B / 2, rounded from FP64.
+    {
+        // E[t] = e^{i pi t / M}, M = B / 2, rounded from FP64 (the fused pass's
+        // twiddles and both forms' real-inverse factors).
         const int len = B;
         std::vector<float2> tab(len);
         const double M = static_cast<double>(B / 2);
@@ -721,8 +722,10 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::Ff
         if (L.inv)
             (void)hipfftDestroy(L.inv);
         L.inv = 0;
-        int nn = f.B;
-        if (hipfftPlanMany(&L.inv, 1, &nn, nullptr, 1, f.bins, nullptr, 1, f.B, HIPFFT_C2R, NB) != HIPFFT_SUCCESS)
+        // The real inverse of length B as a complex inverse of length B / 2
+        // (k_product_z forms its input).
+        int nn = f.B / 2;
+        if (hipfftPlanMany(&L.inv, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_C2C, NB) != HIPFFT_SUCCESS)
             return L.fail("hipFFT inverse plan failed", -2);
         L.inv_batch = NB;
         L.inv_B = f.B;
@@ -736,12 +739,12 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::Ff
     for (size_t b0 = 0; b0 < pairs.size(); b0 += NB) {
         const int cnt = static_cast<int>(std::min<size_t>(NB, pairs.size() - b0));
         const FftPair *pp = L.pairs.as<FftPair>() + b0;
-        LANE_HIP(launch_product(pp, cnt, f.spectra.as<float2>(), qs, L.slot.as<int32_t>(), f.bins, L.Y.as<float2>(),
-                                L.stream),
+        LANE_HIP(launch_product_z(pp, cnt, f.spectra.as<float2>(), qs, L.slot.as<int32_t>(), f.bins,
+                                  f.table.as<float2>(), L.Y.as<float2>(), L.stream),
                  "FFT product");
-        if (hipfftExecC2R(L.inv, reinterpret_cast<hipfftComplex *>(L.Y.as<float2>()), L.y.as<hipfftReal>()) !=
-            HIPFFT_SUCCESS)
-            return L.fail("hipFFT c2r failed", -2);
+        if (hipfftExecC2C(L.inv, reinterpret_cast<hipfftComplex *>(L.Y.as<float2>()),
+                          reinterpret_cast<hipfftComplex *>(L.y.as<float>()), HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+            return L.fail("hipFFT inverse failed", -2);
         LANE_HIP(launch_fft_filter(w, L.tmpl.as<TmplDev>(), pp, cnt, L.y.as<float>(), f.B, f.step, f.rb.as<float2>(),
                                    f.B + 1, f.pinfo.as<float4>(), f.pnorm.as<float2>(), qmax,
                                    L.slot.as<int32_t>(), kfft, mode, cand_out(L), L.stream),
