@@ -97,8 +97,10 @@ struct Lane {
     static constexpr int kPlanSlots = 4;
     hipfftHandle tfwd[kPlanSlots] = {};
     int tfwd_B[kPlanSlots] = {};
-    hipfftHandle inv = 0;
-    int inv_B = 0, inv_batch = 0;
+    // inverse plans of the hipFFT form, one per FFT length (a call with
+    // templates in several length classes alternates between them)
+    hipfftHandle inv[kPlanSlots] = {};
+    int inv_B[kPlanSlots] = {};
     // pinned host staging of the threshold hits (grown on demand)
     void *host_keys = nullptr;
     size_t host_keys_bytes = 0;
@@ -137,9 +139,12 @@ struct Lane {
             tfwd[i] = 0;
             tfwd_B[i] = 0;
         }
-        if (inv)
-            (void)hipfftDestroy(inv);
-        inv = 0;
+        for (int i = 0; i < kPlanSlots; i++) {
+            if (inv[i])
+                (void)hipfftDestroy(inv[i]);
+            inv[i] = 0;
+            inv_B[i] = 0;
+        }
         if (ev0)
             (void)hipEventDestroy(ev0);
         if (ev1)

@@ -462,41 +462,52 @@ int build_resident(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &pl
 
 // ---- per-batch work on lane L (under the shared lock) ----------------------
 
-// The lane's forward plan of kFftPieceBatch templates at length f.B, made on
-// first use and kept; slots fill in order and, when all are taken, the oldest
-// plan is replaced.
-int template_plan(Lane &L, const FftClass &f, hipfftHandle *out)
+// A plan cache of Lane::kPlanSlots slots keyed by FFT length: plans are made
+// on first use and kept; slots fill in order and, when all are taken, the
+// oldest plan is replaced.  make(&handle) creates the plan for length B.
+template <typename Make>
+int cached_plan(Lane &L, hipfftHandle *plans, int *keys, int B, Make make, const char *what, hipfftHandle *out)
 {
     constexpr int S = Lane::kPlanSlots;
     int slot = -1;
     for (int i = 0; i < S && slot < 0; i++)
-        if (L.tfwd[i] && L.tfwd_B[i] == f.B)
+        if (plans[i] && keys[i] == B)
             slot = i;
     if (slot < 0) {
         for (int i = 0; i < S && slot < 0; i++)
-            if (!L.tfwd[i])
+            if (!plans[i])
                 slot = i;
         if (slot < 0) {
-            (void)hipfftDestroy(L.tfwd[0]);
+            (void)hipfftDestroy(plans[0]);
             for (int i = 0; i + 1 < S; i++) {
-                L.tfwd[i] = L.tfwd[i + 1];
-                L.tfwd_B[i] = L.tfwd_B[i + 1];
+                plans[i] = plans[i + 1];
+                keys[i] = keys[i + 1];
             }
             slot = S - 1;
-            L.tfwd[slot] = 0;
+            plans[slot] = 0;
         }
-        L.tfwd_B[slot] = 0;
-        int nn = f.B;
-        if (hipfftPlanMany(&L.tfwd[slot], 1, &nn, nullptr, 1, f.B, nullptr, 1, f.bins, HIPFFT_R2C, kFftPieceBatch) !=
-            HIPFFT_SUCCESS) {
-            L.tfwd[slot] = 0;
-            return L.fail("hipFFT template plan failed", -2);
+        keys[slot] = 0;
+        if (make(&plans[slot]) != HIPFFT_SUCCESS) {
+            plans[slot] = 0;
+            return L.fail(what, -2);
         }
-        L.tfwd_B[slot] = f.B;
+        keys[slot] = B;
     }
-    hipfftSetStream(L.tfwd[slot], L.stream);
-    *out = L.tfwd[slot];
+    hipfftSetStream(plans[slot], L.stream);
+    *out = plans[slot];
     return 0;
+}
+
+// The lane's forward plan of kFftPieceBatch templates at length f.B.
+int template_plan(Lane &L, const FftClass &f, hipfftHandle *out)
+{
+    return cached_plan(
+        L, L.tfwd, L.tfwd_B, f.B,
+        [&](hipfftHandle *h) {
+            int nn = f.B;
+            return hipfftPlanMany(h, 1, &nn, nullptr, 1, f.B, nullptr, 1, f.bins, HIPFFT_R2C, kFftPieceBatch);
+        },
+        "hipFFT template plan failed", out);
 }
 
 // Spectra of the batch's FFT templates [Tg, T) (reversed, zero padded to the
@@ -718,19 +729,17 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::Ff
     if (pairs.empty())
         return 0;
     const int NB = static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, (512ll << 20) / (12ll * f.B))));
-    if (!L.inv || L.inv_batch != NB || L.inv_B != f.B) {
-        if (L.inv)
-            (void)hipfftDestroy(L.inv);
-        L.inv = 0;
-        // The real inverse of length B as a complex inverse of length B / 2
-        // (k_product_z forms its input).
-        int nn = f.B / 2;
-        if (hipfftPlanMany(&L.inv, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_C2C, NB) != HIPFFT_SUCCESS)
-            return L.fail("hipFFT inverse plan failed", -2);
-        L.inv_batch = NB;
-        L.inv_B = f.B;
-    }
-    hipfftSetStream(L.inv, L.stream);
+    // The real inverse of length B as a complex inverse of length B / 2
+    // (k_product_z forms its input); NB depends on B only.
+    hipfftHandle inv = 0;
+    if (int prc = cached_plan(
+            L, L.inv, L.inv_B, f.B,
+            [&](hipfftHandle *h) {
+                int nn = f.B / 2;
+                return hipfftPlanMany(h, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_C2C, NB);
+            },
+            "hipFFT inverse plan failed", &inv))
+        return prc;
     LANE_HIP(L.Y.reserve(sizeof(float2) * static_cast<size_t>(NB) * f.bins), "hipMalloc FFT products");
     LANE_HIP(L.y.reserve(sizeof(float) * static_cast<size_t>(NB) * f.B), "hipMalloc FFT outputs");
     LANE_HIP(L.pairs.reserve(sizeof(FftPair) * pairs.size()), "hipMalloc FFT pairs");
@@ -742,7 +751,7 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::Ff
         LANE_HIP(launch_product_z(pp, cnt, f.spectra.as<float2>(), qs, L.slot.as<int32_t>(), f.bins,
                                   f.table.as<float2>(), L.Y.as<float2>(), L.stream),
                  "FFT product");
-        if (hipfftExecC2C(L.inv, reinterpret_cast<hipfftComplex *>(L.Y.as<float2>()),
+        if (hipfftExecC2C(inv, reinterpret_cast<hipfftComplex *>(L.Y.as<float2>()),
                           reinterpret_cast<hipfftComplex *>(L.y.as<float>()), HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
             return L.fail("hipFFT inverse failed", -2);
         LANE_HIP(launch_fft_filter(w, L.tmpl.as<TmplDev>(), pp, cnt, L.y.as<float>(), f.B, f.step, f.rb.as<float2>(),
