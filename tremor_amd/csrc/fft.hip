@@ -200,13 +200,33 @@ __global__ __launch_bounds__(kThreads) void k_fft_filter(
     float best_g = -INFINITY;
     uint32_t best_j = 0xffffffffu;
     float best_dt = 0.0f;
+    // Windows i of this block's share of the piece that lie in the waveform
+    // (j = base + i < W).  A counted loop: every read below is in bounds for
+    // i < step (m - 1 + i < B, i + m <= B), so the reads of several windows
+    // are issued together.
     const int i0 = blockIdx.x * (kThreads * 16);
-    for (int i = i0 + threadIdx.x; i < i0 + kThreads * 16 && i < step; i += kThreads) {
+    const int iend = i0 + kThreads * 16 < step ? i0 + kThreads * 16 : step;
+    const uint64_t iw = W > base ? W - base : 0;
+    const int ilim = static_cast<uint64_t>(iend) < iw ? iend : static_cast<int>(iw);
+    // One window ahead: the next window's reads are issued before this
+    // window's arithmetic.
+    int i = i0 + static_cast<int>(threadIdx.x);
+    float ny = 0.0f;
+    float2 nra = make_float2(0.0f, 0.0f), nrz = make_float2(0.0f, 0.0f);
+    if (i < ilim) {
+        ny = yb[m - 1 + i];
+        nra = rbp[i];
+        nrz = rbp[i + m];
+    }
+    for (; i < ilim; i += kThreads) {
         const uint64_t j = base + static_cast<uint64_t>(i);
-        if (j >= W)
-            break;
-        const float dt = yb[m - 1 + i] * invB;
-        const float2 ra = rbp[i], rz = rbp[i + m];
+        const float dt = ny * invB;
+        const float2 ra = nra, rz = nrz;
+        if (i + kThreads < ilim) {
+            ny = yb[m - 1 + i + kThreads];
+            nra = rbp[i + kThreads];
+            nrz = rbp[i + kThreads + m];
+        }
         const float S1 = rz.x - ra.x, S2 = rz.y - ra.y;
         const float eS1 = 2.0f * bi.y + 1.2e-7f * (fabsf(ra.x) + fabsf(rz.x) + fabsf(S1));
         const float eS2 = 2.0f * bi.z + 1.2e-7f * (fabsf(ra.y) + fabsf(rz.y) + fabsf(S2));
