@@ -125,7 +125,9 @@ int tmx_set_fft_mode(tmx_ctx *ctx, int32_t mode);
 int tmx_set_scan_mode(tmx_ctx *ctx, int32_t mode);
 
 /* Copies the waveform to the device and builds its FP64 prefix sums of x and
- * x^2 (missing samples as 0).  Replaces any waveform loaded before. */
+ * x^2 (missing samples as 0).  Replaces any waveform loaded before.  Every
+ * sample must be finite or the missing-sample sentinel: a NaN or infinite
+ * sample is refused with an error naming the first one (nothing is loaded). */
 int tmx_load_waveform(tmx_ctx *ctx, const float *x, uint64_t n);
 uint64_t tmx_waveform_length(const tmx_ctx *ctx);
 

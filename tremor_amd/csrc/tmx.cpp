@@ -1443,8 +1443,23 @@ int tmx_load_waveform(tmx_ctx *ctx, const float *x, uint64_t n)
     if ((e = launch_prefix(ctx->x.as<float>(), n, ctx->p.as<double2>(), ctx->prefix_work.as<double>(), wl_len,
                            L.stream)) != hipSuccess)
         return hip_err(e, "prefix sums");
+    double2 total;
+    if ((e = hipMemcpyAsync(&total, ctx->p.as<double2>() + n, sizeof(double2), hipMemcpyDeviceToHost, L.stream)) !=
+        hipSuccess)
+        return hip_err(e, "read prefix total");
     if ((e = hipStreamSynchronize(L.stream)) != hipSuccess)
         return hip_err(e, "load waveform");
+    // A non-finite sample makes the FP64 sum of squares non-finite from there
+    // on (finite float32 squares cannot overflow it), so the last prefix pair
+    // tells whether the waveform holds one.
+    if (!std::isfinite(total.x) || !std::isfinite(total.y)) {
+        uint64_t i = 0;
+        while (i < n && std::isfinite(x[i]))
+            i++;
+        return ctx->fail("waveform sample " + std::to_string(i) +
+                         " is not finite (NaN or infinity): only finite samples and the missing-sample sentinel "
+                         "-100000 are accepted");
+    }
     ctx->n = n;
     return 0;
 }
@@ -1619,8 +1634,11 @@ int tmx_window_stats(tmx_ctx *ctx, int32_t m, uint64_t first, uint64_t count, fl
         return -1;
     if (ctx->n == 0)
         return ctx->fail("no waveform loaded");
-    if (m <= 0 || !mean || !std_out || static_cast<uint64_t>(m) > ctx->n ||
-        first + count > ctx->n - static_cast<uint64_t>(m) + 1)
+    if (m <= 0 || !mean || !std_out || static_cast<uint64_t>(m) > ctx->n)
+        return ctx->fail("window range out of bounds");
+    // Written so that no sum can wrap.
+    const uint64_t windows = ctx->n - static_cast<uint64_t>(m) + 1;
+    if (first > windows || count > windows - first)
         return ctx->fail("window range out of bounds");
     if (count == 0)
         return 0;

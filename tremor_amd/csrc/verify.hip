@@ -393,7 +393,8 @@ __global__ __launch_bounds__(kThreads) void k_dist_batch(const float *__restrict
         return;
     const int m = tmpl[0].m;
     const uint64_t j = starts[i];
-    if (j + static_cast<uint64_t>(m) > n || x[j] == kMissing) {
+    // Written so that no sum can wrap: a start near 2^64 is out of range too.
+    if (static_cast<uint64_t>(m) > n || j > n - static_cast<uint64_t>(m) || x[j] == kMissing) {
         out[i] = FLT_MAX;
         return;
     }
