@@ -151,6 +151,22 @@ void tmx_hits_free(tmx_hits *hits);
 int tmx_distance_batch(tmx_ctx *ctx, const float *tmpl, int32_t m, const uint64_t *starts, int32_t count,
                        float *out_d);
 
+/* Allocation-free variant of tmx_distance_batch.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_tmpl (m raw samples), d_starts (count window starts) and d_out (count
+ * distances) are device pointers of the context's device; the work is
+ * enqueued on `stream` (a hipStream_t; NULL = the context's own stream) and
+ * the call returns without waiting: synchronise the stream before reading
+ * d_out.  Results are those of tmx_distance_batch.  The prepared template
+ * lives in a fixed per-device scratch array of the library (about 264 KB),
+ * so m is at most 65536 (longer: an error) and nothing is allocated or
+ * freed.  Every call on one device shares that scratch: calls whose work can
+ * overlap on one device (other streams, no ordering between them) race.  Do
+ * not load another waveform while such work is in flight. */
+int tmx_distance_batch_noalloc(tmx_ctx *ctx, const float *d_tmpl, int32_t m, const uint64_t *d_starts,
+                               int32_t count, float *d_out, void *stream);
+
 /* Exact reference window statistics (two-pass FP64, stored as float) of the
  * windows [first, first + count) for length m. */
 int tmx_window_stats(tmx_ctx *ctx, int32_t m, uint64_t first, uint64_t count, float *mean, float *std);

@@ -24,7 +24,8 @@ EXPORTS = [
     "tmx_cfg_default", "tmx_create", "tmx_destroy", "tmx_last_error", "tmx_set_precision", "tmx_set_fft_mode",
     "tmx_set_scan_mode",
     "tmx_load_waveform",
-    "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch", "tmx_window_stats",
+    "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch",
+    "tmx_distance_batch_noalloc", "tmx_window_stats",
     "tmx_prepare_template", "tmx_sync",
     "tmx_server_cfg_default", "tmx_server_start", "tmx_server_stop", "tmx_submit", "tmx_poll", "tmx_wait",
     "tmx_result_free",
@@ -107,6 +108,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.tmx_hits_free.argtypes = [ctypes.POINTER(Hits)]
     L.tmx_hits_free.restype = None
     L.tmx_distance_batch.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int32, _u64p, ctypes.c_int32, _f32p]
+    L.tmx_distance_batch_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                             ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
     L.tmx_window_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64, _f32p, _f32p]
     L.tmx_prepare_template.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int32, _f32p, _i32p]
     L.tmx_sync.argtypes = [ctypes.c_void_p]
@@ -280,6 +283,14 @@ class Context:
         self._check(self._L.tmx_distance_batch(self._h, _p(t, _f32p), t.size, _p(s, _u64p), s.size,
                                                _p(out, _f32p)))
         return out
+
+    def distance_batch_noalloc(self, d_template: int, m: int, d_starts: int, count: int, d_out: int,
+                               stream: int = 0):
+        """Device pointers in and out (integers); enqueued on `stream` (0: the
+        context's stream), not waited for."""
+        self._check(self._L.tmx_distance_batch_noalloc(self._h, ctypes.c_void_p(d_template), m,
+                                                       ctypes.c_void_p(d_starts), count, ctypes.c_void_p(d_out),
+                                                       ctypes.c_void_p(stream or None)))
 
     def window_stats(self, m: int, first: int, count: int):
         mean = np.zeros(count, dtype=np.float32)

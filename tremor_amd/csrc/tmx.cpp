@@ -76,6 +76,11 @@ hipError_t launch_fft_filter(const Waveform &w, const TmplDev *tmpl, const FftPa
                              hipStream_t stream);
 hipError_t launch_zero_f64(double *p, uint64_t count, hipStream_t stream);
 double fft_error_factor(int B);
+// noalloc.hip
+int noalloc_max_template();
+hipError_t noalloc_setup(int device);
+hipError_t noalloc_distance_batch(int device, const float *x, uint64_t n, const float *d_tmpl, int m,
+                                  const uint64_t *d_starts, int count, float *d_out, hipStream_t stream);
 // misc.hip
 hipError_t launch_zero_u64(unsigned long long *p, hipStream_t stream);
 hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const float *qz, const int32_t *order,
@@ -1358,6 +1363,12 @@ int tmx_create(tmx_ctx **out, const tmx_cfg *cfg)
         delete ctx;
         return -3;
     }
+    // The device address of the allocation-free entry point's scratch, once
+    // per device, with the device current (Lane::init made it so).
+    if (noalloc_setup(ctx->cfg.device) != hipSuccess) {
+        delete ctx;
+        return -3;
+    }
     *out = ctx;
     return 0;
 }
@@ -1625,6 +1636,31 @@ int tmx_distance_batch(tmx_ctx *ctx, const float *tmpl, int32_t m, const uint64_
         e = hipStreamSynchronize(L.stream);
     if (e != hipSuccess)
         return ctx->fail(std::string("distance batch: ") + hipGetErrorString(e), -2);
+    return 0;
+}
+
+int tmx_distance_batch_noalloc(tmx_ctx *ctx, const float *d_tmpl, int32_t m, const uint64_t *d_starts,
+                               int32_t count, float *d_out, void *stream)
+{
+    if (!ctx)
+        return -1;
+    if (ctx->n == 0)
+        return ctx->fail("no waveform loaded");
+    if (m <= 0 || m > noalloc_max_template())
+        return ctx->fail("template length must be in [1, " + std::to_string(noalloc_max_template()) +
+                         "] for the allocation-free distance batch");
+    if (!d_tmpl || !d_starts || !d_out || count < 0)
+        return ctx->fail("bad arguments");
+    if (count == 0)
+        return 0;
+    std::lock_guard<std::mutex> call(ctx->call_mu);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->direct.stream;
+    if (hipSetDevice(ctx->cfg.device) != hipSuccess)
+        return ctx->fail("hipSetDevice failed", -2);
+    const hipError_t e = noalloc_distance_batch(ctx->cfg.device, ctx->x.as<float>(), ctx->n, d_tmpl, m, d_starts,
+                                                count, d_out, s);
+    if (e != hipSuccess)
+        return ctx->fail(std::string("allocation-free distance batch: ") + hipGetErrorString(e), -2);
     return 0;
 }
 
