@@ -537,27 +537,3 @@ int64_t orc_scan_threshold(const float *raw, int64_t n, const float *templates, 
     }
     return rows;
 }
-
-/* Blocks visited by the early-abandoning kernel for one template over a
- * strided sample of windows at a fixed bound: the early-abandon profile that
- * the reference's cost model samples (Tremor.cpp:2065-2084). */
-double orc_sample_blocks(const float *qz, int m, const float *raw, int64_t n, const float *mean, const float *sd,
-                         float bound, int64_t samples)
-{
-    const int64_t windows = n - m + 1;
-    if (windows <= 0 || samples <= 0)
-        return 0.0;
-    int *order = (int *)malloc(sizeof(int) * (size_t)(m / ORC_BLOCK + 1));
-    orc_block_order(qz, m, order);
-    const int64_t stride = windows / samples > 0 ? windows / samples : 1;
-    int64_t total = 0, visited = 0;
-    for (int64_t id = 0; id < windows && visited < samples; id += stride, visited++) {
-        int blocks = 0;
-        if (raw[id] == ORC_MISSING || sd[id] <= 1e-12f)
-            continue;
-        orc_dist(qz, raw, id, m, mean[id], sd[id], bound, order, &blocks);
-        total += blocks;
-    }
-    free(order);
-    return visited ? (double)total / (double)visited : 0.0;
-}

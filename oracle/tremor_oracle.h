@@ -37,8 +37,6 @@ int orc_scan_knn(const float *raw, int64_t n, const float *templates, const int3
 int64_t orc_scan_threshold(const float *raw, int64_t n, const float *templates, const int32_t *lengths,
                            const int64_t *offsets, int count, float rho, int merge_offset, int threads,
                            int64_t cap, int32_t *out_t, uint64_t *out_pos, float *out_d);
-double orc_sample_blocks(const float *qz, int m, const float *raw, int64_t n, const float *mean, const float *sd,
-                         float bound, int64_t samples);
 
 #ifdef __cplusplus
 }
