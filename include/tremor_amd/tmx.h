@@ -82,6 +82,11 @@ typedef struct {
     int32_t autotune;        /* 1 = default: the templates per matrix-core workgroup are timed on a
                                 sampled pass the first time a batch shape is seen and the fastest
                                 kept for the context; 0 = fixed (8) */
+    int64_t max_candidates;  /* most candidate windows one candidate pass may hold; 0 = default
+                                (2^31 - 1, less if device memory is short).  A call whose templates
+                                together would exceed it is split into halves of its templates, run
+                                one after the other (same answers); one template alone exceeding it
+                                is an error */
 } tmx_cfg;
 
 typedef struct {
@@ -106,6 +111,8 @@ typedef struct {
     double seconds_tune;     /* time spent timing group sizes in this call (first call of a shape) */
     int32_t fft_lmax;        /* longest template of the FFT form's piece shape (pieces of fft_block
                                 samples every fft_block - fft_lmax + 1 windows; 0: none) */
+    int32_t splits;          /* times the call's templates were split in two because their
+                                candidates exceeded max_candidates (0: one pass) */
 } tmx_stats;
 
 typedef struct {
@@ -174,6 +181,24 @@ int tmx_window_stats(tmx_ctx *ctx, int32_t m, uint64_t first, uint64_t count, fl
 /* Z-normalised template and block order exactly as the reference prepares
  * them, computed on the device.  order needs m / 32 slots. */
 int tmx_prepare_template(tmx_ctx *ctx, const float *tmpl, int32_t m, float *out_z, int32_t *out_order);
+
+/* Diagnostics: runs one of the library's own FFT plans on host data, so the
+ * transform error the FFT forms' candidate filter allows for can be measured
+ * against an exact transform.  No counterpart in the reference (its FFTW calls,
+ * fft_fallback.cpp:149-150,170, are trusted with an assumed error constant,
+ * fft_fallback.hpp:84-85).  B: a power of two in [2^14, 2^18].
+ *   TMX_FFT_PROBE_R2C      the forward real-to-complex plan of the piece and
+ *                          template spectra: in = count x B floats, out =
+ *                          count x (B/2 + 1) complex (interleaved floats)
+ *   TMX_FFT_PROBE_INVERSE  the unnormalised complex inverse of length B/2 of
+ *                          the hipFFT form (B >= 2^16 in use): in and out =
+ *                          count x B/2 complex
+ * count: 1..64 transforms. */
+enum tmx_fft_probe_kind {
+    TMX_FFT_PROBE_R2C = 0,
+    TMX_FFT_PROBE_INVERSE = 1
+};
+int tmx_fft_probe(tmx_ctx *ctx, int32_t B, int32_t kind, const float *in, int32_t count, float *out);
 
 /* Waits for all device work of the context's synchronous calls. */
 int tmx_sync(tmx_ctx *ctx);

@@ -26,7 +26,7 @@ EXPORTS = [
     "tmx_load_waveform",
     "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch",
     "tmx_distance_batch_noalloc", "tmx_window_stats",
-    "tmx_prepare_template", "tmx_sync",
+    "tmx_prepare_template", "tmx_fft_probe", "tmx_sync",
     "tmx_server_cfg_default", "tmx_server_start", "tmx_server_stop", "tmx_submit", "tmx_poll", "tmx_wait",
     "tmx_result_free",
 ]
@@ -38,7 +38,8 @@ REQ_THRESHOLD = 1
 class Cfg(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32), ("cand_capacity", ctypes.c_int64),
                 ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32), ("fft_mode", ctypes.c_int32),
-                ("scan_mode", ctypes.c_int32), ("verify_seq", ctypes.c_int32), ("autotune", ctypes.c_int32)]
+                ("scan_mode", ctypes.c_int32), ("verify_seq", ctypes.c_int32), ("autotune", ctypes.c_int32),
+                ("max_candidates", ctypes.c_int64)]
 
 
 class Stats(ctypes.Structure):
@@ -49,7 +50,8 @@ class Stats(ctypes.Structure):
                 ("kernel_ms_main", ctypes.c_double), ("flops_main", ctypes.c_double),
                 ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64),
                 ("scan_templates", ctypes.c_int32), ("scan_blocks", ctypes.c_double),
-                ("tuned_group", ctypes.c_int32), ("seconds_tune", ctypes.c_double), ("fft_lmax", ctypes.c_int32)]
+                ("tuned_group", ctypes.c_int32), ("seconds_tune", ctypes.c_double), ("fft_lmax", ctypes.c_int32),
+                ("splits", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -112,6 +114,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                              ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
     L.tmx_window_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64, _f32p, _f32p]
     L.tmx_prepare_template.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int32, _f32p, _i32p]
+    L.tmx_fft_probe.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _f32p, ctypes.c_int32, _f32p]
     L.tmx_sync.argtypes = [ctypes.c_void_p]
     L.tmx_server_cfg_default.argtypes = [ctypes.POINTER(ServerCfg)]
     L.tmx_server_start.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerCfg)]
@@ -186,7 +189,7 @@ class Context:
 
     def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
                  seed_target: int = 0, verbose: bool = False, fft: str = "auto", scan: str = "auto",
-                 verify_seq: bool = False, autotune: bool = True):
+                 verify_seq: bool = False, autotune: bool = True, max_candidates: int = 0):
         L = lib()
         cfg = Cfg()
         L.tmx_cfg_default(ctypes.byref(cfg))
@@ -199,6 +202,7 @@ class Context:
         cfg.scan_mode = SCAN_MODES[scan]
         cfg.verify_seq = 1 if verify_seq else 0
         cfg.autotune = 1 if autotune else 0
+        cfg.max_candidates = int(max_candidates)
         h = ctypes.c_void_p()
         rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:
@@ -304,6 +308,22 @@ class Context:
         order = np.zeros(max(t.size // 32, 1), dtype=np.int32)
         self._check(self._L.tmx_prepare_template(self._h, _p(t, _f32p), t.size, _p(z, _f32p), _p(order, _i32p)))
         return z, order[: t.size // 32]
+
+    def fft_probe(self, B: int, kind: str, data):
+        """The library's own FFT plans on host data: kind "r2c" (real pieces of
+        B samples -> B/2 + 1 bins) or "inverse" (complex of length B/2,
+        unnormalised backward).  Returns complex64 rows."""
+        if kind == "r2c":
+            a = np.ascontiguousarray(data, dtype=np.float32).reshape(-1, B)
+            out = np.zeros((a.shape[0], B // 2 + 1), dtype=np.complex64)
+            k = 0
+        else:
+            a = np.ascontiguousarray(data, dtype=np.complex64).reshape(-1, B // 2)
+            out = np.zeros((a.shape[0], B // 2), dtype=np.complex64)
+            k = 1
+        self._check(self._L.tmx_fft_probe(self._h, B, k, a.ctypes.data_as(_f32p), a.shape[0],
+                                          out.ctypes.data_as(_f32p)))
+        return out
 
     def sync(self):
         self._check(self._L.tmx_sync(self._h))

@@ -237,7 +237,7 @@ struct tmx_ctx {
     std::mutex gate_mu;            // taken before res_mu: a waiting writer holds it, so readers cannot starve it
     tmx::Lane direct;              // lane of the synchronous calls
     // resident waveform
-    tmx::DevMem x, p, prefix_work;
+    tmx::DevMem x, p, pl, prefix_work;  // samples; double-double prefix sums (high, low parts)
     uint64_t n = 0;
     // block-rebased FP32 prefix pairs of the matrix-core pass
     tmx::DevMem rb, binfo;
@@ -282,6 +282,10 @@ struct WriteResident {
     std::unique_lock<std::shared_mutex> lock;
     explicit WriteResident(tmx_ctx *ctx) : gate(ctx->gate_mu), lock(ctx->res_mu) {}
 };
+
+// Return code of one batch pass whose candidates exceed what one pass may
+// hold; run_batch then splits the batch's templates.
+constexpr int kRcSplit = 3;
 
 // The whole search for one batch of templates on lane L: preparation, kNN
 // bound seeding, candidate pass, exact verification, selection.  out gets

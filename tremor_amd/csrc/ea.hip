@@ -85,14 +85,10 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
     const double md = static_cast<double>(m);
     const double invm = 1.0 / md;
     // Statistics from the prefix sums with their error bounds (filter.hpp's,
-    // with sqrt(a) + sqrt(b) <= sqrt(2 (a + b)) and products by the rounded
-    // 1 / m, whose extra roundings the 8 u64 terms cover).
-    const double2 pa = w.p[j], pb = w.p[j + m];
-    const double S1 = pb.x - pa.x, S2 = pb.y - pa.y;
-    const double eS1 = kPrefixGamma * sqrt(2.0 * (static_cast<double>(j + m) * fabs(pb.y) +
-                                                  static_cast<double>(j) * fabs(pa.y))) +
-                       4.0 * u64 * fabs(S1);
-    const double eS2 = kPrefixGamma * (fabs(pb.y) + fabs(pa.y)) + 4.0 * u64 * fabs(S2);
+    // with products by the rounded 1 / m, whose extra roundings the 8 u64
+    // terms cover).
+    const WindowSums ws = window_sums(w, j, j + static_cast<uint64_t>(m));
+    const double S1 = ws.S1, S2 = ws.S2, eS1 = ws.eS1, eS2 = ws.eS2;
     const double S1sq = S1 * S1 * invm;
     const double V = S2 - S1sq;
     const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) * invm * (1.0 + 4.0 * u64) + 8.0 * u64 * (fabs(S2) + S1sq);

@@ -228,8 +228,10 @@ __device__ __forceinline__ ChunkConst chunk_const(int m)
 // and the reference's FP32 value d_ref >= d_exact - E_ref(d_exact) >= dlo -
 // E_ref(dlo) (d - E_ref(d) increases wherever the result can exceed 0), with
 // E_ref as in filter.hpp and |mu_j| / sigma_j <= mu_abs / sigma_lo.
-// pa = P[a], pb = P[b] are the prefix pairs at the core's ends a = j0 + cnt - 1
-// and b = j0 + m (Lc = b - a, invL = the rounded 1 / Lc).  The differences and
+// pa, pb are the high parts of the prefix pairs at the core's ends a = j0 +
+// cnt - 1 and b = j0 + m (Lc = b - a, invL = the rounded 1 / Lc); where they
+// cannot bound the core (after a loud stretch) the chunk is queued and its
+// windows are judged with the full double-double sums.  The differences and
 // V_C are FP64; the square roots and 1 / sigma_lo are FP32 estimates made
 // one-sided (sigma_lo rounded down, the error terms up), so the result is
 // still a lower bound.  -inf when the core cannot bound sigma.
@@ -240,11 +242,8 @@ __device__ double chunk_dist_lo(double2 pa, double2 pb, uint64_t a, uint64_t b, 
     constexpr double u64 = 1.1102230246251565e-16;
     if (Lc < 2 || !(nmax <= 1e300))
         return -INFINITY;  // no core, or a non-finite dot product: examine every window
-    const double S1 = pb.x - pa.x, S2 = pb.y - pa.y;
-    const double eS1 = kPrefixGamma * (sqrt_up(static_cast<double>(b) * fabs(pb.y)) +
-                                       sqrt_up(static_cast<double>(a) * fabs(pa.y))) +
-                       4.0 * u64 * fabs(S1);
-    const double eS2 = kPrefixGamma * (fabs(pb.y) + fabs(pa.y)) + 4.0 * u64 * fabs(S2);
+    const WindowSums ws = window_sums_hi(pa, pb, a, b);
+    const double S1 = ws.S1, S2 = ws.S2, eS1 = ws.eS1, eS2 = ws.eS2;
     // S1^2 / Lc as S1^2 invL: one rounding more than a division (10 u64).
     const double S1L = S1 * S1 * invL;
     const double V = S2 - S1L;

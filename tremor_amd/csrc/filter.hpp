@@ -3,7 +3,7 @@
 // For window j of template t (length m) an inner-product pass delivers
 //   dt = sum_i q_i * (x0[j+i] - c)   (c: the pass's centring constant)
 // computed in reduced or FP32 arithmetic with |dt - dt_exact| <= eta *
-// sqrt(Q2) * sqrt(sum_i (x0[j+i] - c)^2) + eabs.  From the FP64 prefix sums this
+// sqrt(Q2) * sqrt(sum_i (x0[j+i] - c)^2) + eabs.  From the prefix sums this
 // function derives an interval [lo, hi] that contains the reference's FP32
 // distance d_ref (ZnormSearch.hpp:103-165 with the statistics of
 // SkipSequential.cpp:157-179):
@@ -17,9 +17,9 @@
 //       the 16 FMA lanes plus the horizontal sum and tail the gamma term)
 //
 // Windows whose statistics cannot be bounded away from the reference's
-// constant-window cut (sigma <= 1e-12) or whose FP64 prefix differences are
-// swamped by rounding (a quiet stretch after a loud one) get lo = -inf and
-// hi = +inf: they always become candidates and are decided exactly.
+// constant-window cut (sigma <= 1e-12) or whose prefix differences are
+// swamped by rounding get lo = -inf and hi = +inf: they always become
+// candidates and are decided exactly.
 #pragma once
 
 #include "tmx_internal.hpp"
@@ -37,16 +37,8 @@ __device__ inline DistInterval dist_interval(const Waveform &w, uint64_t j, int 
     constexpr double u = 5.9604644775390625e-08;   // 2^-24
     constexpr double u64 = 1.1102230246251565e-16;  // 2^-53
     const double md = static_cast<double>(m);
-    const double2 pa = w.p[j], pb = w.p[j + m];
-    const double p1a = pa.x, p1b = pb.x;
-    const double p2a = pa.y, p2b = pb.y;
-    const double S1 = p1b - p1a;
-    const double S2 = p2b - p2a;
-    // |P1[i] - exact| <= gamma * sum_{k<i} |x_k| <= gamma * sqrt(i * P2[i]).
-    const double eS1 = kPrefixGamma * (sqrt(static_cast<double>(j + m) * fabs(p2b)) +
-                                       sqrt(static_cast<double>(j) * fabs(p2a))) +
-                       4.0 * u64 * fabs(S1);
-    const double eS2 = kPrefixGamma * (fabs(p2b) + fabs(p2a)) + 4.0 * u64 * fabs(S2);
+    const WindowSums ws = window_sums(w, j, j + m);
+    const double S1 = ws.S1, S2 = ws.S2, eS1 = ws.eS1, eS2 = ws.eS2;
     const double V = S2 - S1 * S1 / md;  // m sigma^2
     const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) / md + 8.0 * u64 * (fabs(S2) + S1 * S1 / md);
     const double vlo = V - eV;

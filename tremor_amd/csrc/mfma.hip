@@ -43,6 +43,8 @@ namespace tmx {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
@@ -268,30 +270,31 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                 const float4 *src = reinterpret_cast<const float4 *>(w.x + J0 + k0 + e);
                 const float4 v0 = src[0], v1 = src[1];
                 const float xv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-                uint16_t hv[8], lv[8];
+                float xt[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const float v = xv[u] == kMissing ? 0.0f : xv[u];
-                    const float xt = v - c;
-                    if (PREC == TMX_PREC_FP16) {
-                        hv[u] = f16_bits(xt * inv_scale);
-                    } else {
-                        hv[u] = bf16_rne(xt);
-                        lv[u] = bf16_rne(xt - bf16_to_f(hv[u]));
-                    }
-                }
-                Frag fh;
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    fh.w[q] = static_cast<uint32_t>(hv[2 * q]) | (static_cast<uint32_t>(hv[2 * q + 1]) << 16);
-                *reinterpret_cast<Frag *>(&lhi[swz(e)]) = fh;
-                if (PREC != TMX_PREC_FP16) {
-                    Frag fl;
+                for (int u = 0; u < 8; u++)
+                    xt[u] = (xv[u] == kMissing ? 0.0f : xv[u]) - c;
+                Frag fh, fl;
+                if (PREC == TMX_PREC_FP16) {
 #pragma unroll
                     for (int q = 0; q < 4; q++)
-                        fl.w[q] = static_cast<uint32_t>(lv[2 * q]) | (static_cast<uint32_t>(lv[2 * q + 1]) << 16);
-                    *reinterpret_cast<Frag *>(&llo[swz(e)]) = fl;
+                        fh.w[q] = static_cast<uint32_t>(f16_bits(xt[2 * q] * inv_scale)) |
+                                  (static_cast<uint32_t>(f16_bits(xt[2 * q + 1] * inv_scale)) << 16);
+                } else {
+                    // Two samples per conversion (v_cvt_pk_bf16_f32, round to
+                    // nearest even); x~ - hi is exact in FP32.
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const f32x2 p = {xt[2 * q], xt[2 * q + 1]};
+                        const bf16x2 hp = __builtin_convertvector(p, bf16x2);
+                        const bf16x2 lp = __builtin_convertvector(p - __builtin_convertvector(hp, f32x2), bf16x2);
+                        fh.w[q] = __builtin_bit_cast(uint32_t, hp);
+                        fl.w[q] = __builtin_bit_cast(uint32_t, lp);
+                    }
                 }
+                *reinterpret_cast<Frag *>(&lhi[swz(e)]) = fh;
+                if (PREC != TMX_PREC_FP16)
+                    *reinterpret_cast<Frag *>(&llo[swz(e)]) = fl;
             }
             __syncthreads();
             // Template operands: loaded at each k-step (the small operand
@@ -601,8 +604,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                 const double ea = eta_mfma(PREC, tmpl[t].kpad) + abs_unit * sqrt(md / Q2) * 1.01;
                 DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(qdt[e]), ea);
                 if (PREC == TMX_PREC_FP16 && iv.lo > -INFINITY) {
-                    const double S1 = w.p[j + m].x - w.p[j].x;
-                    const double V = (w.p[j + m].y - w.p[j].y) - S1 * S1 / md;
+                    const WindowSums ws = window_sums(w, j, j + static_cast<uint64_t>(m));
+                    const double S1 = ws.S1;
+                    const double V = ws.S2 - S1 * S1 / md;
                     const double sig = sqrt(fmax(V, 1e-300) / md);
                     const double ex = 6.103515625e-05 * static_cast<double>(scale) * 128.0 * sqrt(md * Q2) * 1.01;
                     iv.lo -= 2.0 * ex / (sig * (1.0 - 1e-3)) * 1.01;
@@ -625,8 +629,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                     DistInterval iv =
                         dist_interval(w, bj0, m, Q2, S, c, static_cast<double>(qdt[i * 64 + lane]), ea);
                     if (PREC == TMX_PREC_FP16 && iv.hi < INFINITY) {
-                        const double S1 = w.p[bj0 + m].x - w.p[bj0].x;
-                        const double V = (w.p[bj0 + m].y - w.p[bj0].y) - S1 * S1 / md;
+                        const WindowSums ws = window_sums(w, bj0, bj0 + static_cast<uint64_t>(m));
+                        const double S1 = ws.S1;
+                        const double V = ws.S2 - S1 * S1 / md;
                         const double sig = sqrt(fmax(V, 1e-300) / md);
                         const double ex =
                             6.103515625e-05 * static_cast<double>(scale) * 128.0 * sqrt(md * Q2) * 1.01;

@@ -371,7 +371,7 @@ int build_block_prefix(tmx_ctx *ctx, Lane &L, int need)
     ctx->rb_n = 0;
     LANE_HIP(ctx->rb.reserve(sizeof(float2) * blocks * static_cast<uint64_t>(need)), "hipMalloc block prefix");
     LANE_HIP(ctx->binfo.reserve(sizeof(float4) * blocks), "hipMalloc block prefix");
-    LANE_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->n, nw, need, ctx->binfo.as<float4>(), ctx->rb.as<float2>(),
+    LANE_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n, nw, need, ctx->binfo.as<float4>(), ctx->rb.as<float2>(),
                                  L.stream),
              "block prefix");
     LANE_HIP(hipStreamSynchronize(L.stream), "block prefix");
@@ -385,6 +385,28 @@ int build_block_prefix(tmx_ctx *ctx, Lane &L, int need)
 // the shape fft_shape_for(Lt) (B, Lmax; step = B - Lmax + 1); per piece the
 // centre, the rebased prefix pairs of the level-1 filter (hipFFT lengths),
 // ||x~||_2 and ||X||_inf.
+// The transforms the FFT forms run, made in one place so that the
+// diagnostic probe (tmx_fft_probe) times and measures exactly these plans:
+// the r2c forward of kFftPieceBatch real pieces (or templates) of B samples,
+// out of place; the complex inverse of length B / 2 of the hipFFT form, NB
+// transforms per call (NB from B alone).
+hipfftResult make_r2c_plan(hipfftHandle *h, int B)
+{
+    int nn = B;
+    return hipfftPlanMany(h, 1, &nn, nullptr, 1, B, nullptr, 1, B / 2 + 1, HIPFFT_R2C, kFftPieceBatch);
+}
+
+int inverse_batch(int B)
+{
+    return static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, (512ll << 20) / (12ll * B))));
+}
+
+hipfftResult make_inverse_plan(hipfftHandle *h, int B)
+{
+    int nn = B / 2;
+    return hipfftPlanMany(h, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_C2C, inverse_batch(B));
+}
+
 int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
 {
     const FftShape sh = fft_shape_for(Lt);
@@ -422,12 +444,10 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
                  "copy FFT table");
         LANE_HIP(hipStreamSynchronize(L.stream), "copy FFT table");
     }
-    int nn = B;
-    if (hipfftPlanMany(&f.fwd, 1, &nn, nullptr, 1, B, nullptr, 1, f.bins, HIPFFT_R2C, kFftPieceBatch) !=
-        HIPFFT_SUCCESS)
+    if (make_r2c_plan(&f.fwd, B) != HIPFFT_SUCCESS)
         return L.fail("hipFFT forward plan failed", -2);
     hipfftSetStream(f.fwd, L.stream);
-    LANE_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->n, f.step, f.fused ? 0 : B + 1, f.pinfo.as<float4>(),
+    LANE_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n, f.step, f.fused ? 0 : B + 1, f.pinfo.as<float4>(),
                                  f.rb.as<float2>(), L.stream),
              "FFT prefix pairs");
     LANE_HIP(launch_zero_f64(f.norm2.as<double>(), f.P, L.stream), "clear norms");
@@ -508,11 +528,7 @@ int template_plan(Lane &L, const FftClass &f, hipfftHandle *out)
 {
     return cached_plan(
         L, L.tfwd, L.tfwd_B, f.B,
-        [&](hipfftHandle *h) {
-            int nn = f.B;
-            return hipfftPlanMany(h, 1, &nn, nullptr, 1, f.B, nullptr, 1, f.bins, HIPFFT_R2C, kFftPieceBatch);
-        },
-        "hipFFT template plan failed", out);
+        [&](hipfftHandle *h) { return make_r2c_plan(h, f.B); }, "hipFFT template plan failed", out);
 }
 
 // Spectra of the batch's FFT templates [Tg, T) (reversed, zero padded to the
@@ -705,7 +721,7 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::Ff
         any = static_cast<uint64_t>(plan.tmpl[t].m) <= ctx->n && !plan.tmpl[t].ea;
     if (!any)
         return 0;
-    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
+    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
     const double kfft = fft_error_factor(f.B);
     const float2 *qs = L.qspec.as<float2>() + g.qspec_off;
     float *qmax = L.qmax.as<float>() + (g.t0 - plan.Tg);
@@ -733,16 +749,12 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::Ff
         }
     if (pairs.empty())
         return 0;
-    const int NB = static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, (512ll << 20) / (12ll * f.B))));
+    const int NB = inverse_batch(f.B);
     // The real inverse of length B as a complex inverse of length B / 2
     // (k_product_z forms its input); NB depends on B only.
     hipfftHandle inv = 0;
     if (int prc = cached_plan(
-            L, L.inv, L.inv_B, f.B,
-            [&](hipfftHandle *h) {
-                int nn = f.B / 2;
-                return hipfftPlanMany(h, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_C2C, NB);
-            },
+            L, L.inv, L.inv_B, f.B, [&](hipfftHandle *h) { return make_inverse_plan(h, f.B); },
             "hipFFT inverse plan failed", &inv))
         return prc;
     LANE_HIP(L.Y.reserve(sizeof(float2) * static_cast<size_t>(NB) * f.bins), "hipMalloc FFT products");
@@ -828,7 +840,7 @@ int autotune_groups(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, t
     }
     if (G == 0) {
         const double t0 = now_s();
-        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
+        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
         const uint64_t blocks = waveform_blocks(ctx, cfg);
         const int S = static_cast<int>(std::max<uint64_t>(1, blocks / kTuneBlocks));
         float best = 0.0f;
@@ -886,7 +898,7 @@ int fft_seed_stride(const FftClass *f, uint64_t target, int max_want, bool defau
 int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const std::vector<const FftClass *> &fcls,
              int mode, int block_stride, uint64_t *count, const std::vector<int> *fft_strides = nullptr)
 {
-    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
+    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
     LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
     const int groups = static_cast<int>(plan.group_begin.size()) - 1;
     bool inner = false, fft_any = false;
@@ -942,10 +954,26 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     return 0;
 }
 
-// Main pass with buffer growth on overflow.
+// Most candidates one pass may hold: the configured limit, at most 2^31 - 1
+// (the grouping's 32-bit offsets).
+uint64_t cand_limit(const tmx_cfg &cfg)
+{
+    constexpr uint64_t hard = (1ull << 31) - 1;
+    return cfg.max_candidates > 0 ? std::min<uint64_t>(static_cast<uint64_t>(cfg.max_candidates), hard) : hard;
+}
+
+// Device bytes per candidate slot across the lane's candidate buffers
+// (two candidate sets, their estimates and bounds, the exact distances, the
+// sort keys and values), with room for the sort scratch.
+constexpr uint64_t kCandSlotBytes = 96;
+
+// Main pass with buffer growth on overflow.  More candidates than one pass
+// may hold (the limit, or what free device memory allows) returns
+// kRcSplit: the caller splits the batch's templates.
 int run_main(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan,
              const std::vector<const FftClass *> &fcls, uint64_t *count, int *retries)
 {
+    const uint64_t limit = cand_limit(cfg);
     for (;;) {
         int rc = run_pass(ctx, L, cfg, plan, fcls, kModeMain, 1, count);
         if (rc)
@@ -954,12 +982,17 @@ int run_main(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan,
             (*retries)++;
             continue;  // the next pass reserves the larger chunk queue
         }
+        if (*count > limit)
+            return L.fail("more than " + std::to_string(limit) + " candidate windows in one pass", kRcSplit);
         if (*count <= L.cand_cap)
             return 0;
-        if (*count > (1ull << 31) - 1)
-            return L.fail("more than 2^31 candidate windows; raise the bound or split the call", 1);
         (*retries)++;
-        const uint64_t cap = *count + (*count >> 3) + 1024;
+        const uint64_t cap = std::min(*count + (*count >> 3) + 1024, limit);
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+            (cap - L.cand_cap) * kCandSlotBytes > static_cast<uint64_t>(free_b) / 10 * 9)
+            return L.fail("candidate buffers for " + std::to_string(*count) + " windows exceed free device memory",
+                          kRcSplit);
         rc = ensure_cand(L, cap, plan.T);
         if (rc)
             return rc;
@@ -1041,7 +1074,7 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
         LANE_HIP(L.scan_visit.reserve(vb), "hipMalloc scan sample");
         LANE_HIP(launch_zero_f64(L.scan_visit.as<double>(), 2 * static_cast<uint64_t>(plan.T), L.stream),
                  "clear scan sample");
-        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->n};
+        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
         LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
                                 L.scan_ids.as<int32_t>(), static_cast<int>(ids.size()), 0, 1, kScanSample,
                                 cand_out(L), L.scan_visit.as<unsigned long long>(), L.stream),
@@ -1246,8 +1279,11 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
 
 namespace tmx {
 
-int run_batch(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count, std::vector<QueryOut> &out,
-              tmx_stats *stats)
+namespace {
+
+// One batch as one candidate pass per phase.
+int run_batch_once(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count,
+                   std::vector<QueryOut> &out, tmx_stats *stats)
 {
     if (count <= 0)
         return L.fail("template count must be positive");
@@ -1285,6 +1321,62 @@ int run_batch(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int cou
                      1e3 * st.seconds_verify, 1e3 * st.seconds_select);
     if (stats)
         *stats = st;
+    return 0;
+}
+
+void add_stats(tmx_stats &a, const tmx_stats &b)
+{
+    a.seconds_total += b.seconds_total;
+    a.seconds_seed += b.seconds_seed;
+    a.seconds_main += b.seconds_main;
+    a.seconds_verify += b.seconds_verify;
+    a.seconds_select += b.seconds_select;
+    a.comparisons += b.comparisons;
+    a.candidates += b.candidates;
+    a.retries += b.retries;
+    a.max_abs_rho_err = std::max(a.max_abs_rho_err, b.max_abs_rho_err);
+    a.kernel_ms_main += b.kernel_ms_main;
+    a.flops_main += b.flops_main;
+    a.fft_templates += b.fft_templates;
+    a.fft_block = std::max(a.fft_block, b.fft_block);
+    a.verified += b.verified;
+    a.scan_blocks = (a.scan_blocks * a.scan_templates + b.scan_blocks * b.scan_templates) /
+                    std::max(1, a.scan_templates + b.scan_templates);
+    a.scan_templates += b.scan_templates;
+    a.tuned_group = std::max(a.tuned_group, b.tuned_group);
+    a.seconds_tune += b.seconds_tune;
+    a.fft_lmax = std::max(a.fft_lmax, b.fft_lmax);
+    a.splits += b.splits;
+}
+
+}  // namespace
+
+// A batch whose candidates exceed what one pass may hold is split into two
+// halves of its templates (caller order), each run as a batch of its own,
+// recursively: every template's answer is the same in any batch.
+int run_batch(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count, std::vector<QueryOut> &out,
+              tmx_stats *stats)
+{
+    int rc = run_batch_once(ctx, L, cfg, q, count, out, stats);
+    if (rc != kRcSplit)
+        return rc;
+    if (count == 1)
+        return L.fail("one template has more candidate windows than one pass may hold (" +
+                          std::to_string(cand_limit(cfg)) + "; raise max_candidates or the threshold)",
+                      -4);
+    const int half = count / 2;
+    std::vector<QueryOut> a, b;
+    tmx_stats sa{}, sb{};
+    if ((rc = run_batch(ctx, L, cfg, q, half, a, &sa)))
+        return rc;
+    if ((rc = run_batch(ctx, L, cfg, q + half, count - half, b, &sb)))
+        return rc;
+    out = std::move(a);
+    out.insert(out.end(), std::make_move_iterator(b.begin()), std::make_move_iterator(b.end()));
+    add_stats(sa, sb);
+    sa.splits += 1;
+    if (stats)
+        *stats = sa;
     return 0;
 }
 
@@ -1335,6 +1427,7 @@ int tmx_cfg_default(tmx_cfg *cfg)
     cfg->scan_mode = TMX_SCAN_AUTO;
     cfg->verify_seq = 0;
     cfg->autotune = 1;
+    cfg->max_candidates = 0;
     return 0;
 }
 
@@ -1445,13 +1538,16 @@ int tmx_load_waveform(tmx_ctx *ctx, const float *x, uint64_t n)
         return hip_err(e, "hipMalloc waveform");
     if ((e = ctx->p.reserve(sizeof(double2) * (n + 1))) != hipSuccess)
         return hip_err(e, "hipMalloc prefix sums");
+    if ((e = ctx->pl.reserve(sizeof(double2) * (n + 1))) != hipSuccess)
+        return hip_err(e, "hipMalloc prefix sums");
     if ((e = ctx->prefix_work.reserve(sizeof(double) * wl_len)) != hipSuccess)
         return hip_err(e, "hipMalloc prefix scratch");
     if ((e = hipMemcpyAsync(ctx->x.p, x, sizeof(float) * n, hipMemcpyHostToDevice, L.stream)) != hipSuccess)
         return hip_err(e, "copy waveform");
     if ((e = hipMemsetAsync(ctx->x.as<float>() + n, 0, sizeof(float) * pad, L.stream)) != hipSuccess)
         return hip_err(e, "clear padding");
-    if ((e = launch_prefix(ctx->x.as<float>(), n, ctx->p.as<double2>(), ctx->prefix_work.as<double>(), wl_len,
+    if ((e = launch_prefix(ctx->x.as<float>(), n, ctx->p.as<double2>(), ctx->pl.as<double2>(),
+                           ctx->prefix_work.as<double>(), wl_len,
                            L.stream)) != hipSuccess)
         return hip_err(e, "prefix sums");
     double2 total;
@@ -1697,6 +1793,55 @@ int tmx_window_stats(tmx_ctx *ctx, int32_t m, uint64_t first, uint64_t count, fl
         e = hipStreamSynchronize(L.stream);
     if (e != hipSuccess)
         return ctx->fail(std::string("window stats: ") + hipGetErrorString(e), -2);
+    return 0;
+}
+
+int tmx_fft_probe(tmx_ctx *ctx, int32_t B, int32_t kind, const float *in, int32_t count, float *out)
+{
+    if (!ctx)
+        return -1;
+    if (B < (1 << 14) || B > (1 << 18) || (B & (B - 1)) != 0)
+        return ctx->fail("FFT probe: B must be a power of two in [2^14, 2^18]");
+    if (kind != TMX_FFT_PROBE_R2C && kind != TMX_FFT_PROBE_INVERSE)
+        return ctx->fail("FFT probe: unknown transform");
+    const int batch = kind == TMX_FFT_PROBE_R2C ? kFftPieceBatch : inverse_batch(B);
+    if (!in || !out || count <= 0 || count > std::min(batch, 64))
+        return ctx->fail("FFT probe: bad arguments");
+    std::lock_guard<std::mutex> call(ctx->call_mu);
+    Lane &L = ctx->direct;
+    if (hipSetDevice(ctx->cfg.device) != hipSuccess)
+        return ctx->fail("hipSetDevice failed", -2);
+    // Input and output element counts (floats) per transform.
+    const size_t in_len = static_cast<size_t>(B);  // B real samples, or B / 2 complex values
+    const size_t out_len = kind == TMX_FFT_PROBE_R2C ? static_cast<size_t>(B / 2 + 1) * 2 : static_cast<size_t>(B);
+    DevMem din, dout;
+    hipfftHandle h = 0;
+    hipError_t e = din.reserve(sizeof(float) * in_len * batch);
+    if (e == hipSuccess)
+        e = dout.reserve(sizeof(float) * out_len * batch);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(din.p, 0, sizeof(float) * in_len * batch, L.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(din.p, in, sizeof(float) * in_len * count, hipMemcpyHostToDevice, L.stream);
+    if (e != hipSuccess)
+        return ctx->fail(std::string("FFT probe: ") + hipGetErrorString(e), -2);
+    const hipfftResult pr = kind == TMX_FFT_PROBE_R2C ? make_r2c_plan(&h, B) : make_inverse_plan(&h, B);
+    if (pr != HIPFFT_SUCCESS)
+        return ctx->fail("FFT probe: plan failed", -2);
+    hipfftSetStream(h, L.stream);
+    const hipfftResult xr =
+        kind == TMX_FFT_PROBE_R2C
+            ? hipfftExecR2C(h, din.as<hipfftReal>(), reinterpret_cast<hipfftComplex *>(dout.p))
+            : hipfftExecC2C(h, reinterpret_cast<hipfftComplex *>(din.p), reinterpret_cast<hipfftComplex *>(dout.p),
+                            HIPFFT_BACKWARD);
+    e = xr == HIPFFT_SUCCESS ? hipMemcpyAsync(out, dout.p, sizeof(float) * out_len * count, hipMemcpyDeviceToHost,
+                                              L.stream)
+                             : hipErrorLaunchFailure;
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(L.stream);
+    (void)hipfftDestroy(h);
+    if (e != hipSuccess)
+        return ctx->fail(std::string("FFT probe: ") + hipGetErrorString(e), -2);
     return 0;
 }
 

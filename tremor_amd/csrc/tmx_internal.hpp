@@ -131,28 +131,113 @@ enum FilterMode : int {
     kModeSeed = 1,  // emit per-thread minima of the distance upper bound
 };
 
-// Relative error bound of the FP64 prefix sums: every prefix value is within
-// kPrefixGamma * (sum of |terms|) of the exact sum (a few thousand sequential
-// and tree additions at most, each 2^-53).
-constexpr double kPrefixGamma = 8192.0 * 1.1102230246251565e-16;
+// ---- double-double arithmetic (the resident prefix sums) ------------------
+// A value is hi + lo with |lo| <= ulp(hi) / 2.  Every function needs
+// unfused, unreordered FP64 (-ffp-contract=off, no fast-math).
+
+// s + e = a + b exactly (Knuth's two-sum).
+__host__ __device__ inline void two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// s + e = a + b exactly, for |a| >= |b| or a = 0.
+__host__ __device__ inline void quick_two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    e = b - (s - a);
+}
+
+// (ah + al) + (bh + bl) as a normalised pair, relative error <= 2^-104 of
+// |a| + |b| (the accurate double-double addition: both parts two-summed).
+__host__ __device__ inline double2 dd_add(double2 a, double2 b)
+{
+    double s1, s2, t1, t2;
+    two_sum(a.x, b.x, s1, s2);
+    two_sum(a.y, b.y, t1, t2);
+    s2 += t1;
+    quick_two_sum(s1, s2, s1, s2);
+    s2 += t2;
+    quick_two_sum(s1, s2, s1, s2);
+    return make_double2(s1, s2);
+}
+
+// Relative error bound of the double-double prefix sums: every prefix value
+// (hi + lo) is within kPrefixGamma * (sum of |terms|) of the exact sum.  Any
+// prefix value is built by at most ~600 double-double additions along its
+// sequential and tree paths, each within 2^-104 of its operands (whose
+// magnitudes are at most the sum of |terms|): 600 * 2^-104 < 2^-94.
+constexpr double kPrefixGamma = 0x1p-94;
 
 struct Waveform {
     const float *x;     // raw samples, n + tail padding (missing kept)
-    const double2 *p;   // prefix sums (x0, x0^2) interleaved, missing as 0, n + 1 pairs
+    const double2 *p;   // prefix sums (x0, x0^2), high parts, missing as 0, n + 1 pairs
+    const double2 *pl;  // their low parts: P[i] = p[i] + pl[i] (double-double), same layout
     uint64_t n;
 };
 
+// Sums of x0 and x0^2 over the samples [a, b) from the double-double prefix
+// sums, with bounds on their absolute errors: the prefix values' own error
+// (kPrefixGamma times the sum of |terms| before each end, sum_{k<i} |x_k| <=
+// sqrt(i P2[i])) plus the roundings of the three FP64 operations per sum.
+struct WindowSums {
+    double S1, S2;    // sum x0, sum x0^2
+    double eS1, eS2;  // |S - exact| bounds
+};
+
+__device__ inline WindowSums window_sums(double2 pa, double2 pla, double2 pb, double2 plb, uint64_t a, uint64_t b)
+{
+    constexpr double u64 = 1.1102230246251565e-16;
+    WindowSums r;
+    const double d1 = pb.x - pa.x, l1 = plb.x - pla.x;
+    const double d2 = pb.y - pa.y, l2 = plb.y - pla.y;
+    r.S1 = d1 + l1;
+    r.S2 = d2 + l2;
+    const double P2a = fabs(pa.y) * (1.0 + 4.0 * u64), P2b = fabs(pb.y) * (1.0 + 4.0 * u64);
+    r.eS1 = kPrefixGamma * (sqrt(static_cast<double>(b) * P2b) + sqrt(static_cast<double>(a) * P2a)) * 1.01 +
+            2.0 * u64 * (fabs(d1) + fabs(l1) + fabs(r.S1));
+    r.eS2 = kPrefixGamma * (P2b + P2a) + 2.0 * u64 * (fabs(d2) + fabs(l2) + fabs(r.S2));
+    return r;
+}
+
+__device__ inline WindowSums window_sums(const Waveform &w, uint64_t a, uint64_t b)
+{
+    return window_sums(w.p[a], w.pl[a], w.p[b], w.pl[b], a, b);
+}
+
+// The same from the high parts alone (one load per end, for register-tight
+// kernels): each high part is within u64 |P| + kPrefixGamma * (sum of |terms|)
+// of the exact prefix, |P| <= sum of |terms|.  Looser after loud stretches;
+// a caller whose bound fails there falls back to window_sums.
+__device__ inline WindowSums window_sums_hi(double2 pa, double2 pb, uint64_t a, uint64_t b)
+{
+    constexpr double u64 = 1.1102230246251565e-16;
+    constexpr double g = u64 + kPrefixGamma;
+    WindowSums r;
+    r.S1 = pb.x - pa.x;
+    r.S2 = pb.y - pa.y;
+    const double P2a = fabs(pa.y) * (1.0 + 4.0 * u64), P2b = fabs(pb.y) * (1.0 + 4.0 * u64);
+    r.eS1 = g * (sqrt(static_cast<double>(b) * P2b) + sqrt(static_cast<double>(a) * P2a)) * 1.01 +
+            2.0 * u64 * fabs(r.S1);
+    r.eS2 = g * (P2b + P2a) * 1.01 + 2.0 * u64 * fabs(r.S2);
+    return r;
+}
+
 // ---- launchers (all asynchronous on `stream`) ----------------------------
 
-// FP64 inclusive prefix sums (P1, P2) of x0 and x0^2, interleaved (P[0] = 0).
-hipError_t launch_prefix(const float *x, uint64_t n, double2 *p, double *work, size_t work_len, hipStream_t stream);
+// Double-double inclusive prefix sums (P1, P2) of x0 and x0^2 (P[0] = 0):
+// high parts into p, low parts into pl, n + 1 pairs each.
+hipError_t launch_prefix(const float *x, uint64_t n, double2 *p, double2 *pl, double *work, size_t work_len,
+                         hipStream_t stream);
 size_t prefix_work_len(uint64_t n);
 
 // Block-rebased FP32 prefix pairs (and per-block centre and error bounds) for
 // the level-1 filter of the matrix-core pass; `block` windows per block,
 // `len` entries per block.
-hipError_t launch_block_prefix(const double2 *p, uint64_t n, int block, int len, float4 *binfo, float2 *rb,
-                               hipStream_t stream);
+hipError_t launch_block_prefix(const double2 *p, const double2 *pl, uint64_t n, int block, int len, float4 *binfo,
+                               float2 *rb, hipStream_t stream);
 
 // Reference template preparation: z-norm (ZnormSearch.hpp:33-62), block
 // order (:72-90), plus Q2 and S.  One workgroup per template.
