@@ -57,13 +57,16 @@ def _args():
                     help="c3: lanes of the rate sweep (one lane keeps the tail shortest below saturation)")
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--long-subset", type=int, default=8, help="c4: templates of the GEMM and scan legs")
+    ap.add_argument("--group", type=int, default=0, choices=[0, 4, 8, 16],
+                    help="c1: templates per matrix-core workgroup, fixed (0: autotuned on the warmup call)")
     return ap.parse_args()
 
 
-def _measured_traffic(n, templates, k, prec):
-    """HBM bytes per main launch of the matrix-core kernel on this workload,
-    from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded under profiles/
-    (null for any other workload or precision)."""
+def _measured_traffic(n, templates, k, prec, group):
+    """HBM bytes per main launch of the matrix-core kernel on this workload at
+    the templates per workgroup the timed launches used, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded under profiles/ (null for any other
+    workload, precision or group size)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_scan_mfma_c1_traffic.json")
     try:
         with open(path) as f:
@@ -72,7 +75,23 @@ def _measured_traffic(n, templates, k, prec):
         return None
     if rec.get("workload") != {"n": n, "templates": templates, "k": k, "precision": prec}:
         return None
-    return {"bytes": rec["hbm_bytes_per_launch"], "unit": "B", "source": rec["source"]}
+    by_group = rec.get("by_group", {}).get(str(group))
+    if not by_group:
+        return None
+    return {"bytes": by_group["hbm_bytes_per_launch"], "unit": "B", "group": group, "source": rec["source"]}
+
+
+def host_cores():
+    """Threads for the CPU baseline: the CPU share the process may use (the
+    OpenMP thread count the environment sets, else the affinity mask), and
+    what nproc reports, for the record."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(omp) if omp.isdigit() and int(omp) > 0 else affinity
+    return threads, {"nproc": os.cpu_count(), "affinity": affinity, "OMP_NUM_THREADS": omp or None}
 
 
 def cpu_baseline(x, templates, k, budget_s):
@@ -80,7 +99,7 @@ def cpu_baseline(x, templates, k, budget_s):
     dynamic 2^16-window chunks, early abandoning) on a bounded sample:
     the first 2*10^7 samples, templates in order until the budget is spent."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_cores()
     n_s = min(x.size, 20_000_000)
     xs = np.ascontiguousarray(x[:n_s])
     comps = 0
@@ -95,8 +114,10 @@ def cpu_baseline(x, templates, k, budget_s):
         if time.time() - t0 > budget_s:
             break
     return {"value": comps / secs if secs > 0 else None, "unit": "comparisons/s", "cores": threads, "kind": "port",
+            "host": host,
             "sample": f"first {n_s} samples, {used} templates, kNN k={k}, query time only "
-                      f"(window statistics built beforehand, as the reference separates index and query time)"}
+                      f"(window statistics built beforehand, as the reference separates index and query time); "
+                      f"{threads} threads = the host CPU share (nproc {host['nproc']}, affinity {host['affinity']})"}
 
 
 def bench_threshold(a):
@@ -143,7 +164,7 @@ def bench_threshold(a):
     ns = min(w.x.size, 1 << 23)
     xs = np.ascontiguousarray(w.x[:ns])
     sample = [t for t in w.templates if len(t) <= ns][:: max(1, len(w.templates) // 12)][:12]
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_cores()
     ctx.load_waveform(xs)
     cpu_s = 0.0
     cpu_comps = 0
@@ -159,8 +180,9 @@ def bench_threshold(a):
         parity[str(rho)] = {"rows": int(r.pos.size), "identical_to_reference_scan": bool(same),
                             "max_abs_rho_err": float(r.stats["max_abs_rho_err"])}
         print(f"[bench] cpu sample rho {rho}: {r.pos.size} rows, identical {same}", file=sys.stderr, flush=True)
-    cpu = {"value": cpu_comps / cpu_s, "unit": "comparisons/s", "cores": threads, "kind": "port",
-           "sample": f"first {ns} samples, {len(sample)} templates, thresholds 0.9/0.7/0.5 (early abandoning)",
+    cpu = {"value": cpu_comps / cpu_s, "unit": "comparisons/s", "cores": threads, "kind": "port", "host": host,
+           "sample": f"first {ns} samples, {len(sample)} templates, thresholds 0.9/0.7/0.5 (early abandoning); "
+                     f"{threads} threads = the host CPU share (nproc {host['nproc']}, affinity {host['affinity']})",
            "rows_identical": parity}
     out = {"metric": METRIC, "value": 3 * comps / total_s, "unit": "comparisons/s", "n_gpus": 1,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -356,7 +378,7 @@ def main():
 
     # The headline measures the matrix-core form (every template on MFMA);
     # the cost model's own choice is measured after it.
-    ctx = Context(precision=a.prec, fft="off", scan="off")
+    ctx = Context(precision=a.prec, fft="off", scan="off", autotune=a.group if a.group else True)
     t0 = time.time()
     ctx.load_waveform(w.x)
     load_s = time.time() - t0
@@ -429,6 +451,7 @@ def main():
         print(f"[bench] cpu baseline {cpu['value']:.3e} comparisons/s", file=sys.stderr, flush=True)
 
     last = stats[-1]
+    traffic = _measured_traffic(w.x.size, len(w.templates), a.k, a.prec, last["tuned_group"])
     out = {
         "metric": METRIC,
         "value": value,
@@ -449,8 +472,9 @@ def main():
                    "n": int(w.x.size), "templates": len(w.templates), "k": a.k, "precision": a.prec},
         "roofline": {"bound": "mfma" if a.prec != "fp32" else "fp32-vector", "achieved": achieved / 1e12,
                      "peak": PEAK[a.prec] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[a.prec],
-                     "traffic": (_measured_traffic(w.x.size, len(w.templates), a.k, a.prec) or {}).get("bytes"),
-                     "traffic_source": (_measured_traffic(w.x.size, len(w.templates), a.k, a.prec) or {}).get("source"),
+                     "traffic": (traffic or {}).get("bytes"),
+                     "traffic_source": (traffic or {}).get("source"),
+                     "traffic_group": stats[-1]["tuned_group"],
                      "kernel": "k_scan_mfma" if a.prec != "fp32" else "k_scan_fp32",
                      "kernel_ms": kms, "algorithmic_flops_per_launch": flops,
                      "note": "achieved counts 2m flops per comparison once; split-BF16 issues 3 MFMA products, "

@@ -81,7 +81,7 @@ typedef struct {
                                 parallel shortcut, sequential only where it cannot decide) */
     int32_t autotune;        /* 1 = default: the templates per matrix-core workgroup are timed on a
                                 sampled pass the first time a batch shape is seen and the fastest
-                                kept for the context; 0 = fixed (8) */
+                                kept for the context; 0 = fixed (8); 4, 8 or 16 = that size, fixed */
     int64_t max_candidates;  /* most candidate windows one candidate pass may hold; 0 = default
                                 (2^31 - 1, less if device memory is short).  A call whose templates
                                 together would exceed it is split into halves of its templates, run

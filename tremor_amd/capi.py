@@ -189,7 +189,7 @@ class Context:
 
     def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
                  seed_target: int = 0, verbose: bool = False, fft: str = "auto", scan: str = "auto",
-                 verify_seq: bool = False, autotune: bool = True, max_candidates: int = 0):
+                 verify_seq: bool = False, autotune: bool | int = True, max_candidates: int = 0):
         L = lib()
         cfg = Cfg()
         L.tmx_cfg_default(ctypes.byref(cfg))
@@ -201,7 +201,8 @@ class Context:
         cfg.fft_mode = FFT_MODES[fft]
         cfg.scan_mode = SCAN_MODES[scan]
         cfg.verify_seq = 1 if verify_seq else 0
-        cfg.autotune = 1 if autotune else 0
+        # True: timed per batch shape; False: fixed 8; 4, 8 or 16: that size, fixed
+        cfg.autotune = (1 if autotune else 0) if isinstance(autotune, bool) else int(autotune)
         cfg.max_candidates = int(max_candidates)
         h = ctypes.c_void_p()
         rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))

@@ -93,8 +93,9 @@ struct Lane {
     uint64_t chunk_cap = 0;
     uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
     // template forward plans, one per FFT length (batches alternate between
-    // the classes, and a plan takes milliseconds to make)
-    static constexpr int kPlanSlots = 4;
+    // the classes, and a plan takes milliseconds to make): one slot for each
+    // length the FFT forms use, 2^14 .. 2^18, so none is ever evicted
+    static constexpr int kPlanSlots = 5;
     hipfftHandle tfwd[kPlanSlots] = {};
     int tfwd_B[kPlanSlots] = {};
     // inverse plans of the hipFFT form, one per FFT length (a call with

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
     // Statistics from the prefix sums with their error bounds (filter.hpp's,
     // with products by the rounded 1 / m, whose extra roundings the 8 u64
     // terms cover).
-    const WindowSums ws = window_sums(w, j, j + static_cast<uint64_t>(m));
+    const WindowSums ws = window_sums_fast(w, j, j + static_cast<uint64_t>(m));
     const double S1 = ws.S1, S2 = ws.S2, eS1 = ws.eS1, eS2 = ws.eS2;
     const double S1sq = S1 * S1 * invm;
     const double V = S2 - S1sq;

@@ -37,7 +37,7 @@ __device__ inline DistInterval dist_interval(const Waveform &w, uint64_t j, int 
     constexpr double u = 5.9604644775390625e-08;   // 2^-24
     constexpr double u64 = 1.1102230246251565e-16;  // 2^-53
     const double md = static_cast<double>(m);
-    const WindowSums ws = window_sums(w, j, j + m);
+    const WindowSums ws = window_sums_fast(w, j, j + m);
     const double S1 = ws.S1, S2 = ws.S2, eS1 = ws.eS1, eS2 = ws.eS2;
     const double V = S2 - S1 * S1 / md;  // m sigma^2
     const double eV = eS2 + (2.0 * fabs(S1) * eS1 + eS1 * eS1) / md + 8.0 * u64 * (fabs(S2) + S1 * S1 / md);

@@ -604,7 +604,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                 const double ea = eta_mfma(PREC, tmpl[t].kpad) + abs_unit * sqrt(md / Q2) * 1.01;
                 DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(qdt[e]), ea);
                 if (PREC == TMX_PREC_FP16 && iv.lo > -INFINITY) {
-                    const WindowSums ws = window_sums(w, j, j + static_cast<uint64_t>(m));
+                    const WindowSums ws = window_sums_fast(w, j, j + static_cast<uint64_t>(m));
                     const double S1 = ws.S1;
                     const double V = ws.S2 - S1 * S1 / md;
                     const double sig = sqrt(fmax(V, 1e-300) / md);
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                     DistInterval iv =
                         dist_interval(w, bj0, m, Q2, S, c, static_cast<double>(qdt[i * 64 + lane]), ea);
                     if (PREC == TMX_PREC_FP16 && iv.hi < INFINITY) {
-                        const WindowSums ws = window_sums(w, bj0, bj0 + static_cast<uint64_t>(m));
+                        const WindowSums ws = window_sums_fast(w, bj0, bj0 + static_cast<uint64_t>(m));
                         const double S1 = ws.S1;
                         const double V = ws.S2 - S1 * S1 / md;
                         const double sig = sqrt(fmax(V, 1e-300) / md);

@@ -503,6 +503,8 @@ int cached_plan(Lane &L, hipfftHandle *plans, int *keys, int B, Make make, const
             if (!plans[i])
                 slot = i;
         if (slot < 0) {
+            // Work enqueued with the oldest plan may still be running.
+            LANE_HIP(hipStreamSynchronize(L.stream), "FFT plan eviction");
             (void)hipfftDestroy(plans[0]);
             for (int i = 0; i + 1 < S; i++) {
                 plans[i] = plans[i + 1];
@@ -827,6 +829,13 @@ int autotune_groups(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, t
     int inner = 0;
     for (int d = 0; d < plan.Tg; d++)
         inner += plan.tmpl[d].ea ? 0 : 1;
+    // A fixed size asked for (measurements of one configuration).
+    for (const int g : kTuneGroups)
+        if (cfg.autotune == g) {
+            st.tuned_group = g;
+            set_groups(plan, g);
+            return upload_groups(L, plan);
+        }
     // Up to the smallest candidate every size gives one group: nothing to choose.
     if (!cfg.autotune || inner <= kTuneGroups[0])
         return 0;

@@ -225,6 +225,20 @@ __device__ inline WindowSums window_sums_hi(double2 pa, double2 pb, uint64_t a, 
     return r;
 }
 
+// The window's sums from the high parts where they bound its variance to
+// 1e-3 (almost every window: one load per end), else from the full
+// double-double sums (a quiet stretch after a loud one).
+__device__ inline WindowSums window_sums_fast(const Waveform &w, uint64_t a, uint64_t b)
+{
+    const WindowSums r = window_sums_hi(w.p[a], w.p[b], a, b);
+    const double L = static_cast<double>(b - a);
+    const double V = r.S2 - r.S1 * r.S1 / L;
+    const double eV = r.eS2 + (2.0 * fabs(r.S1) * r.eS1 + r.eS1 * r.eS1) / L;
+    if (eV < 1e-3 * V)
+        return r;
+    return window_sums(w, a, b);
+}
+
 // ---- launchers (all asynchronous on `stream`) ----------------------------
 
 // Double-double inclusive prefix sums (P1, P2) of x0 and x0^2 (P[0] = 0):
