@@ -10,8 +10,9 @@
   searched on the device, 8 sampled templates checked against the oracle's
   answers; and the same waveform with real-data dynamic range (a x1e6 stretch
   before a x1e-3 one, missing samples), kNN and threshold.
-- The FFT forms' error allowance: the library's own transform plans measured
-  against float64 transforms on adversarial pieces, at every FFT length used.
+- The FFT forms' error allowance: the forward transforms (hipFFT) and the
+  long-template form's hand-written inverse measured against float64
+  transforms on adversarial inputs, at every FFT length used.
 """
 import os
 import sys
@@ -66,6 +67,31 @@ def test_long_templates_each_form(gold, form, kw):
             if form == "mfma":
                 assert st["fft_templates"] == 0 and st["scan_templates"] == 0
             _same(pos, d, gold[f"long/knn{merge}/pos"], gold[f"long/knn{merge}/d"])
+
+
+def test_fft_length_2p16_form(oracle):
+    """Templates of 8193..16384 samples take FFT length 2^16 on the FFT form
+    (the hand-written inverse at its smallest size): kNN and threshold
+    answers against the oracle run here."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    rng = np.random.default_rng(41)
+    n = 1 << 19
+    x = synth.bandpassed_noise(n, rng)
+    onsets, _ = synth.inject_events(x, 24, rng)
+    tm = synth.cut_templates(x, onsets, [8200, 12000, 16384], rng)
+    with Context(precision="bf16x3", fft="always", scan="off") as ctx:
+        ctx.load_waveform(x)
+        for merge in (0, 16):
+            _log("fft 2^16", merge=merge)
+            pos, d, st = ctx.knn(tm, 10, merge)
+            assert st["fft_templates"] == 3 and st["fft_block"] == 1 << 16
+            po, do, _ = oracle.scan_knn(x, tm, 10, merge)
+            _same(pos, d, po, do)
+        r = ctx.threshold(tm, 0.5, 4)
+        tt, tp, td = oracle.scan_threshold(x, tm, 0.5, 4)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
 
 
 def test_c0_full_every_form(oracle):

@@ -85,7 +85,7 @@ struct Lane {
     DevMem out_pos, out_d, errbits;
     uint64_t cand_cap = 0;
     // FFT form: template spectra, products, inverse transforms
-    DevMem Y, y, pairs, qspec, qmax, qin, ids, slot;
+    DevMem Y, pairs, qspec, qmax, qin, ids, slot;
     // fused FFT form: queue of chunks for the per-window pass
     DevMem chunk_rec, chunk_dt, chunk_n;
     // early-abandoning scan: its templates and the sample counters
@@ -98,10 +98,6 @@ struct Lane {
     static constexpr int kPlanSlots = 5;
     hipfftHandle tfwd[kPlanSlots] = {};
     int tfwd_B[kPlanSlots] = {};
-    // inverse plans of the hipFFT form, one per FFT length (a call with
-    // templates in several length classes alternates between them)
-    hipfftHandle inv[kPlanSlots] = {};
-    int inv_B[kPlanSlots] = {};
     // pinned host staging of the threshold hits (grown on demand)
     void *host_keys = nullptr;
     size_t host_keys_bytes = 0;
@@ -139,12 +135,6 @@ struct Lane {
                 (void)hipfftDestroy(tfwd[i]);
             tfwd[i] = 0;
             tfwd_B[i] = 0;
-        }
-        for (int i = 0; i < kPlanSlots; i++) {
-            if (inv[i])
-                (void)hipfftDestroy(inv[i]);
-            inv[i] = 0;
-            inv_B[i] = 0;
         }
         if (ev0)
             (void)hipEventDestroy(ev0);

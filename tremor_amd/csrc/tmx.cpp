@@ -68,12 +68,15 @@ hipError_t launch_piece_norm(const double *norm2, float2 *pnorm, uint64_t P, hip
 hipError_t launch_spec_max(const float2 *spec, int rows, int bins, float *out, int stride_out, hipStream_t stream);
 hipError_t launch_tmpl_input(const TmplDev *tmpl, const int32_t *ids, int count, int B, const float *qz, float *out,
                              hipStream_t stream);
-hipError_t launch_product_z(const FftPair *pairs, int npairs, const float2 *spectra, const float2 *qspec,
-                            const int32_t *slot, int bins, const float2 *E, float2 *Z, hipStream_t stream);
-hipError_t launch_fft_filter(const Waveform &w, const TmplDev *tmpl, const FftPair *pairs, int npairs, const float *y,
-                             int B, int step, const float2 *rb, int rb_len, const float4 *pinfo, const float2 *pnorm,
-                             const float *qmax, const int32_t *slot, double kfft, int mode, const CandOut &co,
-                             hipStream_t stream);
+// fft_large.hip
+bool fft_large_supports(int B);
+hipError_t launch_fft_large(const Waveform &w, const TmplDev *tmpl, const FftPair *pairs, int npairs,
+                            const float2 *spectra, const float2 *qspec, const int32_t *slot, int B, const float2 *E,
+                            float2 *U, int step, const float2 *rb, int rb_len, const float4 *pinfo,
+                            const float2 *pnorm, const float *qmax, double kfft, int mode, const CandOut &co,
+                            hipStream_t stream);
+hipError_t launch_fft_large_probe(int B, const float2 *E, const float2 *zin, int count, float2 *U, float2 *zout,
+                                  hipStream_t stream);
 hipError_t launch_zero_f64(double *p, uint64_t count, hipStream_t stream);
 double fft_error_factor(int B);
 // noalloc.hip
@@ -385,26 +388,22 @@ int build_block_prefix(tmx_ctx *ctx, Lane &L, int need)
 // the shape fft_shape_for(Lt) (B, Lmax; step = B - Lmax + 1); per piece the
 // centre, the rebased prefix pairs of the level-1 filter (hipFFT lengths),
 // ||x~||_2 and ||X||_inf.
-// The transforms the FFT forms run, made in one place so that the
-// diagnostic probe (tmx_fft_probe) times and measures exactly these plans:
-// the r2c forward of kFftPieceBatch real pieces (or templates) of B samples,
-// out of place; the complex inverse of length B / 2 of the hipFFT form, NB
-// transforms per call (NB from B alone).
+// The forward transform the FFT forms run (hipFFT), made in one place so that
+// the diagnostic probe (tmx_fft_probe) measures exactly this plan: the r2c
+// forward of kFftPieceBatch real pieces (or templates) of B samples, out of
+// place.
 hipfftResult make_r2c_plan(hipfftHandle *h, int B)
 {
     int nn = B;
     return hipfftPlanMany(h, 1, &nn, nullptr, 1, B, nullptr, 1, B / 2 + 1, HIPFFT_R2C, kFftPieceBatch);
 }
 
+// (piece, template) pairs per launch pair of the long-template form: their
+// intermediates (B / 2 complex values each) within 128 MB, so that they stay
+// in the Infinity Cache between the two kernels.
 int inverse_batch(int B)
 {
-    return static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, (512ll << 20) / (12ll * B))));
-}
-
-hipfftResult make_inverse_plan(hipfftHandle *h, int B)
-{
-    int nn = B / 2;
-    return hipfftPlanMany(h, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_C2C, inverse_batch(B));
+    return static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, (128ll << 20) / (4ll * B))));
 }
 
 int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
@@ -751,32 +750,24 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::Ff
         }
     if (pairs.empty())
         return 0;
+    if (!fft_large_supports(f.B))
+        return L.fail("no FFT form for length " + std::to_string(f.B), -2);
+    // Pairs piece-major (the templates of one piece consecutive: its
+    // spectrum and prefix pairs are shared in L2); the hand-written inverse
+    // and filter (fft_large.hip) in launches of NB pairs, NB from B alone.
     const int NB = inverse_batch(f.B);
-    // The real inverse of length B as a complex inverse of length B / 2
-    // (k_product_z forms its input); NB depends on B only.
-    hipfftHandle inv = 0;
-    if (int prc = cached_plan(
-            L, L.inv, L.inv_B, f.B, [&](hipfftHandle *h) { return make_inverse_plan(h, f.B); },
-            "hipFFT inverse plan failed", &inv))
-        return prc;
-    LANE_HIP(L.Y.reserve(sizeof(float2) * static_cast<size_t>(NB) * f.bins), "hipMalloc FFT products");
-    LANE_HIP(L.y.reserve(sizeof(float) * static_cast<size_t>(NB) * f.B), "hipMalloc FFT outputs");
+    LANE_HIP(L.Y.reserve(sizeof(float2) * static_cast<size_t>(NB) * (f.B / 2)), "hipMalloc FFT intermediates");
     LANE_HIP(L.pairs.reserve(sizeof(FftPair) * pairs.size()), "hipMalloc FFT pairs");
     LANE_HIP(hipMemcpyAsync(L.pairs.p, pairs.data(), sizeof(FftPair) * pairs.size(), hipMemcpyHostToDevice, L.stream),
              "copy FFT pairs");
     for (size_t b0 = 0; b0 < pairs.size(); b0 += NB) {
         const int cnt = static_cast<int>(std::min<size_t>(NB, pairs.size() - b0));
         const FftPair *pp = L.pairs.as<FftPair>() + b0;
-        LANE_HIP(launch_product_z(pp, cnt, f.spectra.as<float2>(), qs, L.slot.as<int32_t>(), f.bins,
-                                  f.table.as<float2>(), L.Y.as<float2>(), L.stream),
-                 "FFT product");
-        if (hipfftExecC2C(inv, reinterpret_cast<hipfftComplex *>(L.Y.as<float2>()),
-                          reinterpret_cast<hipfftComplex *>(L.y.as<float>()), HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
-            return L.fail("hipFFT inverse failed", -2);
-        LANE_HIP(launch_fft_filter(w, L.tmpl.as<TmplDev>(), pp, cnt, L.y.as<float>(), f.B, f.step, f.rb.as<float2>(),
-                                   f.B + 1, f.pinfo.as<float4>(), f.pnorm.as<float2>(), qmax,
-                                   L.slot.as<int32_t>(), kfft, mode, cand_out(L), L.stream),
-                 "FFT filter");
+        LANE_HIP(launch_fft_large(w, L.tmpl.as<TmplDev>(), pp, cnt, f.spectra.as<float2>(), qs,
+                                  L.slot.as<int32_t>(), f.B, f.table.as<float2>(), L.Y.as<float2>(), f.step,
+                                  f.rb.as<float2>(), f.B + 1, f.pinfo.as<float4>(), f.pnorm.as<float2>(), qmax, kfft,
+                                  mode, cand_out(L), L.stream),
+                 "long-template FFT pass");
     }
     return 0;
 }
@@ -1813,8 +1804,9 @@ int tmx_fft_probe(tmx_ctx *ctx, int32_t B, int32_t kind, const float *in, int32_
         return ctx->fail("FFT probe: B must be a power of two in [2^14, 2^18]");
     if (kind != TMX_FFT_PROBE_R2C && kind != TMX_FFT_PROBE_INVERSE)
         return ctx->fail("FFT probe: unknown transform");
-    const int batch = kind == TMX_FFT_PROBE_R2C ? kFftPieceBatch : inverse_batch(B);
-    if (!in || !out || count <= 0 || count > std::min(batch, 64))
+    if (kind == TMX_FFT_PROBE_INVERSE && !fft_large_supports(B))
+        return ctx->fail("FFT probe: the long-template inverse has B = 2^16, 2^17 or 2^18");
+    if (!in || !out || count <= 0 || count > 64)
         return ctx->fail("FFT probe: bad arguments");
     std::lock_guard<std::mutex> call(ctx->call_mu);
     Lane &L = ctx->direct;
@@ -1823,8 +1815,8 @@ int tmx_fft_probe(tmx_ctx *ctx, int32_t B, int32_t kind, const float *in, int32_
     // Input and output element counts (floats) per transform.
     const size_t in_len = static_cast<size_t>(B);  // B real samples, or B / 2 complex values
     const size_t out_len = kind == TMX_FFT_PROBE_R2C ? static_cast<size_t>(B / 2 + 1) * 2 : static_cast<size_t>(B);
-    DevMem din, dout;
-    hipfftHandle h = 0;
+    const int batch = kind == TMX_FFT_PROBE_R2C ? kFftPieceBatch : count;
+    DevMem din, dout, dmid, dtab;
     hipError_t e = din.reserve(sizeof(float) * in_len * batch);
     if (e == hipSuccess)
         e = dout.reserve(sizeof(float) * out_len * batch);
@@ -1834,21 +1826,36 @@ int tmx_fft_probe(tmx_ctx *ctx, int32_t B, int32_t kind, const float *in, int32_
         e = hipMemcpyAsync(din.p, in, sizeof(float) * in_len * count, hipMemcpyHostToDevice, L.stream);
     if (e != hipSuccess)
         return ctx->fail(std::string("FFT probe: ") + hipGetErrorString(e), -2);
-    const hipfftResult pr = kind == TMX_FFT_PROBE_R2C ? make_r2c_plan(&h, B) : make_inverse_plan(&h, B);
-    if (pr != HIPFFT_SUCCESS)
-        return ctx->fail("FFT probe: plan failed", -2);
-    hipfftSetStream(h, L.stream);
-    const hipfftResult xr =
-        kind == TMX_FFT_PROBE_R2C
-            ? hipfftExecR2C(h, din.as<hipfftReal>(), reinterpret_cast<hipfftComplex *>(dout.p))
-            : hipfftExecC2C(h, reinterpret_cast<hipfftComplex *>(din.p), reinterpret_cast<hipfftComplex *>(dout.p),
-                            HIPFFT_BACKWARD);
-    e = xr == HIPFFT_SUCCESS ? hipMemcpyAsync(out, dout.p, sizeof(float) * out_len * count, hipMemcpyDeviceToHost,
-                                              L.stream)
-                             : hipErrorLaunchFailure;
+    if (kind == TMX_FFT_PROBE_R2C) {
+        hipfftHandle h = 0;
+        if (make_r2c_plan(&h, B) != HIPFFT_SUCCESS)
+            return ctx->fail("FFT probe: plan failed", -2);
+        hipfftSetStream(h, L.stream);
+        const hipfftResult xr = hipfftExecR2C(h, din.as<hipfftReal>(), reinterpret_cast<hipfftComplex *>(dout.p));
+        e = xr == HIPFFT_SUCCESS ? hipStreamSynchronize(L.stream) : hipErrorLaunchFailure;
+        (void)hipfftDestroy(h);
+    } else {
+        // The table the long-template form uses: E[t] = e^{i pi t / M}, t < B,
+        // rounded from FP64.
+        std::vector<float2> tab(B);
+        const double M = static_cast<double>(B / 2);
+        for (int t = 0; t < B; t++) {
+            const double a = M_PI * static_cast<double>(t) / M;
+            tab[t] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+        }
+        e = dtab.reserve(sizeof(float2) * B);
+        if (e == hipSuccess)
+            e = dmid.reserve(sizeof(float) * in_len * count);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dtab.p, tab.data(), sizeof(float2) * B, hipMemcpyHostToDevice, L.stream);
+        if (e == hipSuccess)
+            e = launch_fft_large_probe(B, dtab.as<float2>(), din.as<float2>(), count, dmid.as<float2>(),
+                                       dout.as<float2>(), L.stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(L.stream);
+    }
     if (e == hipSuccess)
-        e = hipStreamSynchronize(L.stream);
-    (void)hipfftDestroy(h);
+        e = hipMemcpy(out, dout.p, sizeof(float) * out_len * count, hipMemcpyDeviceToHost);
     if (e != hipSuccess)
         return ctx->fail(std::string("FFT probe: ") + hipGetErrorString(e), -2);
     return 0;
