@@ -19,8 +19,9 @@
 // The small transforms are Stockham-style in LDS: a length-N transform with
 // N = 16 Q is a radix-16 pass (16 elements per thread, in registers), a
 // twiddle w_N^{n1 k2} and a length-Q pass in registers; the register
-// transforms are radix-2 after a bit reversal, their twiddles from an LDS
-// table rounded from FP64.
+// transforms are radix-2 after a bit reversal, their twiddles from LDS copies
+// of tables rounded from FP64 (after E in the class's table: w_256^t,
+// w_M^t for t < 512, w_M^{512 t}, w_N2^t).
 //
 // Error of this inverse.  Counting, as in Higham's bound for radix-2
 // transforms, every radix-2 level and every twiddle multiplication as one
@@ -128,13 +129,14 @@ __global__ __launch_bounds__(kThreads) void k_large_a(const FftPair *__restrict_
         return;
     const int pair = item / kBlocks;
     const int k20 = (item % kBlocks) * kCols;
-    // E[t] = e^{i pi t / M}: e^{2 pi i s / M} = E[2 s].
+    // The compact twiddle tables after E (E has B = 2 M entries).
+    const float2 *lt = E + 2 * M;
     for (int t = tid; t < kN1; t += kThreads)
-        tw[t] = E[2 * t * (M / kN1)];
+        tw[t] = lt[t];
     for (int t = tid; t < 512; t += kThreads)
-        ta[t] = E[2 * t];
+        ta[t] = lt[kN1 + t];
     for (int t = tid; t < M / 512; t += kThreads)
-        tb[t] = E[2 * 512 * t];
+        tb[t] = lt[kN1 + 512 + t];
     const float2 *X = nullptr, *H = nullptr;
     if (SPEC) {
         const FftPair pr = pairs[pair];
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void k_large_b(Waveform w, const TmplDev 
     const int pair = item / kBlocks;
     const int r10 = (item % kBlocks) * RB;
     for (int t = tid; t < N2; t += kThreads)
-        tw[t] = E[2 * t * (M / N2)];
+        tw[t] = E[2 * M + kN1 + 512 + M / 512 + t];
     const float2 *src = U + static_cast<uint64_t>(pair) * M;
     for (int e = tid; e < RB * N2; e += kThreads) {
         const int row = e / N2, k2 = e % N2;

@@ -388,6 +388,37 @@ int build_block_prefix(tmx_ctx *ctx, Lane &L, int need)
 // the shape fft_shape_for(Lt) (B, Lmax; step = B - Lmax + 1); per piece the
 // centre, the rebased prefix pairs of the level-1 filter (hipFFT lengths),
 // ||x~||_2 and ||X||_inf.
+// e^{2 pi i num / den}, rounded from FP64.
+float2 unit_root(int64_t num, int64_t den)
+{
+    const double a = 2.0 * M_PI * static_cast<double>(num) / static_cast<double>(den);
+    return make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+}
+
+// The FFT forms' table for length B (M = B / 2): E[t] = e^{i pi t / M},
+// t < B; for B >= 2^16 followed by the long-template form's twiddles
+// (fft_large.hip): w_256^t (t < 256), w_M^t (t < 512), w_M^{512 t}
+// (t < M / 512), w_{N2}^t (t < N2 = M / 256).
+std::vector<float2> fft_table(int B)
+{
+    const int64_t M = B / 2;
+    std::vector<float2> tab;
+    tab.reserve(static_cast<size_t>(B) + 256 + 512 + M / 512 + M / 256);
+    for (int64_t t = 0; t < B; t++)
+        tab.push_back(unit_root(t, 2 * M));
+    if (B >= (1 << 16)) {
+        for (int64_t t = 0; t < 256; t++)
+            tab.push_back(unit_root(t, 256));
+        for (int64_t t = 0; t < 512; t++)
+            tab.push_back(unit_root(t, M));
+        for (int64_t t = 0; t < M / 512; t++)
+            tab.push_back(unit_root(512 * t, M));
+        for (int64_t t = 0; t < M / 256; t++)
+            tab.push_back(unit_root(t, M / 256));
+    }
+    return tab;
+}
+
 // The forward transform the FFT forms run (hipFFT), made in one place so that
 // the diagnostic probe (tmx_fft_probe) measures exactly this plan: the r2c
 // forward of kFftPieceBatch real pieces (or templates) of B samples, out of
@@ -430,14 +461,10 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
     LANE_HIP(f.work_real.reserve(sizeof(float) * static_cast<size_t>(kFftPieceBatch) * B), "hipMalloc FFT input");
     {
         // E[t] = e^{i pi t / M}, M = B / 2, rounded from FP64 (the fused pass's
-        // twiddles and both forms' real-inverse factors).
-        const int len = B;
-        std::vector<float2> tab(len);
-        const double M = static_cast<double>(B / 2);
-        for (int t = 0; t < len; t++) {
-            const double a = M_PI * static_cast<double>(t) / M;
-            tab[t] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
-        }
+        // twiddles and both forms' real-inverse factors), then the
+        // long-template form's compact twiddle tables.
+        std::vector<float2> tab = fft_table(B);
+        const int len = static_cast<int>(tab.size());
         LANE_HIP(f.table.reserve(sizeof(float2) * len), "hipMalloc FFT table");
         LANE_HIP(hipMemcpyAsync(f.table.p, tab.data(), sizeof(float2) * len, hipMemcpyHostToDevice, L.stream),
                  "copy FFT table");
@@ -1835,19 +1862,13 @@ int tmx_fft_probe(tmx_ctx *ctx, int32_t B, int32_t kind, const float *in, int32_
         e = xr == HIPFFT_SUCCESS ? hipStreamSynchronize(L.stream) : hipErrorLaunchFailure;
         (void)hipfftDestroy(h);
     } else {
-        // The table the long-template form uses: E[t] = e^{i pi t / M}, t < B,
-        // rounded from FP64.
-        std::vector<float2> tab(B);
-        const double M = static_cast<double>(B / 2);
-        for (int t = 0; t < B; t++) {
-            const double a = M_PI * static_cast<double>(t) / M;
-            tab[t] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
-        }
-        e = dtab.reserve(sizeof(float2) * B);
+        // The table the long-template form uses.
+        const std::vector<float2> tab = fft_table(B);
+        e = dtab.reserve(sizeof(float2) * tab.size());
         if (e == hipSuccess)
             e = dmid.reserve(sizeof(float) * in_len * count);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(dtab.p, tab.data(), sizeof(float2) * B, hipMemcpyHostToDevice, L.stream);
+            e = hipMemcpyAsync(dtab.p, tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice, L.stream);
         if (e == hipSuccess)
             e = launch_fft_large_probe(B, dtab.as<float2>(), din.as<float2>(), count, dmid.as<float2>(),
                                        dout.as<float2>(), L.stream);
