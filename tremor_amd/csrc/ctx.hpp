@@ -92,6 +92,11 @@ struct Lane {
     DevMem scan_ids, scan_visit;
     uint64_t chunk_cap = 0;
     uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
+    // matrix-core form: queue of level-1 survivors for the rigorous second level
+    DevMem surv, surv_n;
+    uint64_t surv_cap = 0;
+    uint64_t surv_need = 0;  // > surv_cap after a main pass that overflowed the queue
+    uint64_t last_survivors = 0;  // level-1 survivors of the last main pass
     // template forward plans, one per FFT length (batches alternate between
     // the classes, and a plan takes milliseconds to make): one slot for each
     // length the FFT forms use, 2^14 .. 2^18, so none is ever evicted
@@ -244,7 +249,7 @@ struct tmx_ctx {
     std::vector<std::unique_ptr<tmx::Lane>> spare_lanes;
     // largest candidate buffer and chunk queue (entries) any lane has needed:
     // lanes reserve at least these, so one overflow and rerun teaches all
-    std::atomic<uint64_t> cand_hint{0}, chunk_hint{0};
+    std::atomic<uint64_t> cand_hint{0}, chunk_hint{0}, surv_hint{0};
     // autotuner: templates per matrix-core workgroup chosen per batch shape
     // (precision, template count, mean padded length), guarded by tune_mu
     std::mutex tune_mu;

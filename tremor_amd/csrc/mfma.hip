@@ -29,15 +29,25 @@
 // matrix core's internal rounding mode).
 //
 // Epilogue.  A level-1 test in FP32 (per output: the window's m*sigma^2 from
-// the FP64 prefix sums, one rsqrt) rejects windows that cannot reach the
-// template's bound with generous slack; survivors get the rigorous FP64
-// interval of filter.hpp and are emitted as candidates.  In seed mode each
-// lane evaluates its most promising window rigorously and the wavefront
-// emits the minimum upper bound.
+// the block-rebased prefix sums, one rsqrt) rejects windows that cannot reach
+// the template's bound with generous slack.  It runs on the accumulators in
+// place (registers and global loads only), and its survivors go to a global
+// queue; k_level2 gives each the rigorous FP64 interval of filter.hpp and
+// emits the candidates.  In seed mode each lane evaluates its most promising
+// window rigorously and the wavefront emits the minimum upper bound.
 //
-// Workgroup: 4 wavefronts x 4 column tiles x 1024 windows = 16384 windows;
-// templates of a group (sorted by length) are processed two at a time.
+// Workgroup (one per CU): 8 wavefronts x 4 column tiles x 1024 windows =
+// 32768 windows; templates of a group (sorted by length) are processed two at
+// a time.  When the group's longest template fits one staged chunk (K <=
+// 2048) the block is staged once and the wavefronts then run through every
+// template pair with no further barrier: the epilogue leaves LDS untouched,
+// and the two wavefronts of each SIMD take the pairs in opposite orders, so
+// one's epilogue overlaps the other's matrix-core work.  Inside a K chunk the template operands are loaded one k-step ahead and the
+// waveform fragments of each column-tile pair are reloaded for the next
+// k-step right after their last product.
 #include "filter.hpp"
+
+#include <type_traits>
 
 namespace tmx {
 namespace {
@@ -48,7 +58,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kWaves = 4;
+constexpr int kWaves = 8;  // two per SIMD, one workgroup per CU
 constexpr int kThreads = 64 * kWaves;
 constexpr int kNM = 2;                        // templates per pass
 constexpr int kNN = 4;                        // 32-column tiles per wavefront
@@ -61,21 +71,15 @@ constexpr int kStage = kNW + kKC;             // staged samples per chunk
 // LDS layout of the staged tile (16-bit elements): rows of 32 elements (one
 // column of the Hankel view), 8-element chunks permuted within a row by
 // (row / 2) mod 4.  The 16-byte reads of 8 consecutive lanes (consecutive
-// rows) then land in 8 distinct 16-byte bank groups, without padding.
+// rows) then land in 8 distinct 16-byte bank groups, without padding.  The
+// permutation touches bits 3-4 only and reads bits 6-7, so it commutes with
+// adding multiples of 1024: the column tiles of a wavefront share one
+// address plus immediate offsets.
 __device__ inline int swz(int e) { return e ^ (((e >> 6) & 3) << 3); }
 
-// Epilogue scratch, aliased over the tile once a template pair's K loop is
-// done: per wavefront the window statistics of one 1024-window tile (padded
-// rows of 33) and a queue of windows that passed the level-1 test, decided
-// by the rigorous FP64 filter once the accumulators are no longer live.
-constexpr int kStatLen = kTileWin + kTileWin / 32;
-constexpr int kQ = 384;  // queue entries per wavefront (>= kNM * 64 for the seed mode)
-constexpr int kStatBatch = 2;  // tile-statistics loads in flight per lane
 constexpr int kTileBytesBf16 = 2 * kStage * 2;
 constexpr int kTileBytesF16 = kStage * 2;
-constexpr int kEpiBytes = kWaves * kStatLen * 4 + kWaves * kQ * 12 + kWaves * 16 * 64 * 4;
-static_assert(kQ >= kNM * 64, "seed entries need one queue slot per lane and template");
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kLoBytes = kStage * 2;  // byte offset of the lo half of the tile
 
 struct alignas(16) Frag {
     uint32_t w[4];
@@ -98,12 +102,16 @@ __device__ inline uint16_t f16_bits(float v)
 
 // ---- operand packing --------------------------------------------------------
 
-// Per template, the split template values in two 16-bit arrays (hi, then
-// lo) of P = kpad + 64 entries, entry 32 + i holding q[i] and every other
-// entry zero.  The lane (s, h) operand fragment of k-step kappa,
-// A[s][16 kappa + 8 h + e] = q[16 kappa + 8 h + e - s], e < 8, is then the 16
-// bytes at entry 32 + 16 kappa + 8 h - s: one (unaligned) vector load from an
-// array small enough to stay in the L1 cache.  One thread per entry.
+// Per template, the split template values (hi, then lo).  Let P be the
+// zero-padded template of L = kpad + 64 entries, P[32 + i] = q[i] and every
+// other entry zero.  The lane (s, h) operand fragment of k-step kappa,
+// A[s][16 kappa + 8 h + e] = q[16 kappa + 8 h + e - s], e < 8, is the 8
+// entries of P from E = 32 + 16 kappa + 8 h - s, which start at every
+// residue mod 8 across the lanes.  So each half holds P in 8 shifted copies,
+// interleaved by 8-entry chunk: entry (g, c, e) = P[8 g + c + e] (0 past L),
+// g < L / 8, c < 8.  A lane reads copy c = (-s) mod 8 at chunk g = (E - c) / 8:
+// one aligned 16-byte load, and a wavefront's 64 loads of one k-step fall in
+// 640 consecutive bytes.  One thread per entry.
 __global__ void k_pack(const TmplDev *__restrict__ tmpl, int count, const float *__restrict__ qz, int precision,
                        Frag *__restrict__ a)
 {
@@ -111,12 +119,19 @@ __global__ void k_pack(const TmplDev *__restrict__ tmpl, int count, const float 
     if (t >= count)
         return;
     const int m = tmpl[t].m;
-    const int P = tmpl[t].kpad + 64;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= P)
+    const int L = tmpl[t].kpad + 64;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    uint16_t *const base = reinterpret_cast<uint16_t *>(a + tmpl[t].a_off);
+    // The 64 fragments after the last template are zero too: no read of the
+    // operand arrays meets uninitialised memory.
+    if (t == count - 1 && idx < 512)
+        base[16 * L + idx] = 0;
+    if (idx >= 8 * L)
         return;
-    const int i = e - 32;
-    const float v = (i >= 0 && i < m) ? qz[tmpl[t].q_off + i] : 0.0f;
+    const int g = idx >> 6, c = (idx >> 3) & 7, e = idx & 7;
+    const int src = 8 * g + c + e;  // entry of P
+    const int i = src - 32;
+    const float v = (src < L && i >= 0 && i < m) ? qz[tmpl[t].q_off + i] : 0.0f;
     uint16_t hi, lo;
     if (precision == TMX_PREC_FP16) {
         hi = f16_bits(v * 128.0f);  // |q| * 2^7 <= 2^15: no overflow, tiny values stay normal
@@ -125,9 +140,8 @@ __global__ void k_pack(const TmplDev *__restrict__ tmpl, int count, const float 
         hi = bf16_rne(v);
         lo = bf16_rne(v - bf16_to_f(hi));
     }
-    uint16_t *base = reinterpret_cast<uint16_t *>(a + tmpl[t].a_off);
-    base[e] = hi;
-    base[P + e] = lo;
+    base[idx] = hi;
+    base[8 * L + idx] = lo;
 }
 
 // ---- level-1 filter -------------------------------------------------------------
@@ -152,38 +166,80 @@ __device__ inline double eta_mfma(int precision, int kpad)
     return (rep + acc) * 1.001;
 }
 
+
+// Level-1 constants of every matrix-core template from its final bound (one
+// thread per template; FP64 once here instead of in every workgroup).
+__global__ void k_mfma_consts(TmplDev *__restrict__ tmpl, int count, int precision)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count)
+        return;
+    const int m = tmpl[t].m;
+    const double Q2 = tmpl[t].Q2;
+    const double md = static_cast<double>(m);
+    const double eta = eta_mfma(precision, tmpl[t].kpad);
+    // FP16 floor terms: staged |x~| * 2^-ex below 2^-14 and |q| * 2^7 below
+    // 2^-14 may be flushed.  x side: sum |q| 2^-14 2^ex <= 2^-14 2^ex
+    // sqrt(m Q2) (absolute; the block's scale is 2^ex 2^-7); q side: sum
+    // 2^-21 |x~| <= 2^-21 sqrt(m) sqrt(sum x~^2) (relative to the window).
+    const double abs_unit = precision == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;  // 2^-21
+    L1Consts k;
+    k.gmin = static_cast<float>((Q2 + md - static_cast<double>(tmpl[t].bound) - eref_cap(m)) * 0.5 * (1.0 - 1e-3));
+    k.eta_sq = static_cast<float>(eta * sqrt(Q2) * (1.0 + 1e-3));
+    k.sqrt_m = static_cast<float>(sqrt(md));
+    k.inv_mf = 1.0f / static_cast<float>(m);
+    k.rsqrt_m = static_cast<float>(1.0 / sqrt(md)) * (1.0f + 1e-6f);
+    k.absS = static_cast<float>(fabs(tmpl[t].S) * (1.0 + 1e-3));
+    k.eabs_q = static_cast<float>(abs_unit * sqrt(md) * 1.01);
+    k.exq = precision == TMX_PREC_FP16 ? static_cast<float>(6.103515625e-05 * 128.0 * sqrt(md * Q2) * 1.011) : 0.0f;
+    k.Kc = k.gmin * k.rsqrt_m * 0.999f - (k.eta_sq + k.eabs_q) * 1.001f;
+    k.pad = 0.0f;
+    tmpl[t].l1 = k;
+}
+
 // ---- main kernel ------------------------------------------------------------
 
-// Orders LDS traffic among the lanes of one wavefront (its private scratch).
-__device__ inline void wave_sync()
+// The rigorous interval of one level-1 survivor (main mode) or seed entry.
+template <int PREC>
+__device__ inline DistInterval survivor_interval(const Waveform &w, uint64_t j, const TmplDev &tp, float c, float dt,
+                                                 float scale)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int m = tp.m;
+    const double md = static_cast<double>(m);
+    const double abs_unit = PREC == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;
+    const double ea = eta_mfma(PREC, tp.kpad) + abs_unit * sqrt(md / tp.Q2) * 1.01;
+    DistInterval iv = dist_interval(w, j, m, tp.Q2, tp.S, c, static_cast<double>(dt), ea);
+    if (PREC == TMX_PREC_FP16 && iv.lo > -INFINITY) {
+        // Staged samples below 2^-14 in scaled units may be flushed: at most
+        // 2^-14 2^ex sqrt(m Q2) on the dot product, divided by sigma.
+        const WindowSums ws = window_sums_fast(w, j, j + static_cast<uint64_t>(m));
+        const double V = ws.S2 - ws.S1 * ws.S1 / md;
+        const double sig = sqrt(fmax(V, 1e-300) / md);
+        const double ex = 6.103515625e-05 * static_cast<double>(scale) * 128.0 * sqrt(md * tp.Q2) * 1.01;
+        const double d = 2.0 * ex / (sig * (1.0 - 1e-3)) * 1.01;
+        iv.lo -= d;
+        iv.hi += d;
+    }
+    return iv;
 }
 
 template <int PREC>
-__global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const TmplDev *__restrict__ tmpl,
+__global__ __launch_bounds__(kThreads, 1) void k_scan_mfma(Waveform w, const TmplDev *__restrict__ tmpl,
                                                            const int32_t *__restrict__ group_begin,
                                                            const Frag *__restrict__ afrag, int mode,
                                                            int block_stride, int groups, uint32_t nblocks,
                                                            const float2 *__restrict__ rb,
                                                            const float4 *__restrict__ binfo, int rb_len,
-                                                           CandOut co)
+                                                           SurvOut so, CandOut co)
 {
-    constexpr int kLds = cmax(PREC == TMX_PREC_FP16 ? kTileBytesF16 : kTileBytesBf16, kEpiBytes);
+    constexpr bool kSplit = PREC != TMX_PREC_FP16;
+    constexpr int kLds = kSplit ? kTileBytesBf16 : kTileBytesF16;
     __shared__ __attribute__((aligned(16))) unsigned char lds[kLds];
-    __shared__ float s_red[kWaves];
     uint16_t *const lhi = reinterpret_cast<uint16_t *>(lds);
     uint16_t *const llo = lhi + kStage;
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int col = lane & 31, h = lane >> 5;
-    float *const stats = reinterpret_cast<float *>(lds) + wave * kStatLen;
-    uint32_t *const qj = reinterpret_cast<uint32_t *>(lds + kWaves * kStatLen * 4) + wave * kQ;
-    float *const qdt = reinterpret_cast<float *>(lds + kWaves * kStatLen * 4 + kWaves * kQ * 4) + wave * kQ;
-    int32_t *const qi = reinterpret_cast<int32_t *>(lds + kWaves * kStatLen * 4 + kWaves * kQ * 8) + wave * kQ;
-    float *const sdt = reinterpret_cast<float *>(lds + kWaves * kStatLen * 4 + kWaves * kQ * 12) + wave * 16 * 64;
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
     // on each XCD, consecutive workgroups take every template group of one
     // waveform block, so the block's samples and prefix sums are shared in
@@ -201,30 +257,25 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
     const float4 bi = binfo[blk];  // block centre and error bounds of the rebased prefix pairs
     const float c = bi.x;
     const float2 *const rbb = rb + blk * static_cast<uint64_t>(rb_len);
+    // The block's rebased prefix pairs through a buffer resource: the lane's
+    // part of every address is one register, the rest scalar or immediate.
+    const __amdgpu_buffer_rsrc_t rbr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2 *>(rbb), static_cast<short>(0), rb_len * 8, 0x00020000);
+    auto rbl = [&](uint32_t voff, uint32_t soff) {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rbr, voff, soff, 0));
+    };
 
     // Largest K of the group (templates sorted by length: the last one).
     const int kmax = tmpl[te - 1].kpad;
 
-    // FP16: power-of-two scale so the staged block fits FP16.  The waveform
-    // buffer carries zero padding past n, so tile reads need no guard.
-    // Largest |x~| over the block's samples: bounds every window's |mu - c|
-    // for the tile test (and sets the FP16 scale).
-    float mx = 0.0f;
-    for (int e = tid; e < kNW + kmax; e += kThreads) {
-        float v = w.x[J0 + e];
-        v = v == kMissing ? 0.0f : v;
-        mx = fmaxf(mx, fabsf(v - c));
-    }
-    for (int off = 32; off > 0; off >>= 1)
-        mx = fmaxf(mx, __shfl_xor(mx, off));
-    if (lane == 0)
-        s_red[wave] = mx;
-    __syncthreads();
-    mx = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+    // Largest |x~| over the block's samples (resident, over at least the
+    // samples staged here): bounds every window's |mu - c| for the level-1
+    // test (and sets the FP16 scale).
+    const float mx = bi.w;
     const float aD_max = mx * 1.0001f;  // >= |x0 - c| over the block, hence >= |mu_j - c|
     float scale = 1.0f, inv_scale = 1.0f;
     if (PREC == TMX_PREC_FP16) {
-        // max |x~| * 2^-e <= 2^14
+        // FP16: power-of-two scale so the staged block fits (max |x~| * 2^-e <= 2^14).
         int ex = 0;
         if (mx > 0.0f) {
             (void)frexpf(mx, &ex);  // mx < 2^ex
@@ -234,46 +285,34 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
         scale = ldexpf(1.0f, ex - 7);   // real dot = accumulated * 2^ex * 2^-7
     }
 
-    for (int tp = tb; tp < te; tp += kNM) {
-        const int nt = te - tp < kNM ? te - tp : kNM;
-        // Operand arrays through buffer resources: the lane's entry offset
-        // (shared by both templates) in one register, the k-step and the lo
-        // part's offset in scalar registers.
-        int ks[kNM];
-        uint32_t lo_off[kNM];
-        __amdgpu_buffer_rsrc_t ar[kNM];
+    // Stages x~ = (x0 - c) [* 2^-e] for [J0 + k0, J0 + k0 + kNW + kc), 8
+    // samples per item, NB items per thread in flight.
+    auto stage = [&](int k0, int kc, auto nb) {
+        constexpr int NB = decltype(nb)::value;
+        const int items = (kNW + kc) / 8;
+        for (int e0 = tid; e0 < items; e0 += NB * kThreads) {
+            float4 v[NB][2];
 #pragma unroll
-        for (int i = 0; i < kNM; i++) {
-            const int ti = tp + (i < nt ? i : 0);
-            ks[i] = i < nt && !tmpl[ti].ea ? tmpl[ti].kpad / 16 : 0;
-            lo_off[i] = static_cast<uint32_t>(tmpl[ti].kpad + 64) * 2u;
-            ar[i] = __builtin_amdgcn_make_buffer_rsrc(const_cast<Frag *>(afrag + tmpl[ti].a_off), static_cast<short>(0),
-                                                      static_cast<int>(2u * lo_off[i]), 0x00020000);
-        }
-        const uint32_t avoff = static_cast<uint32_t>(32 + 8 * h - col) * 2u;
-        const int kt = (ks[0] > ks[kNM - 1] ? ks[0] : ks[kNM - 1]) * 16;
-        f32x16 acc[kNM][kNN];
+            for (int u = 0; u < NB; u++) {
+                const int e8 = e0 + u * kThreads;
+                if (e8 < items) {
+                    const float4 *src = reinterpret_cast<const float4 *>(w.x + J0 + k0 + 8 * e8);
+                    v[u][0] = src[0];
+                    v[u][1] = src[1];
+                }
+            }
 #pragma unroll
-        for (int i = 0; i < kNM; i++)
-#pragma unroll
-            for (int jn = 0; jn < kNN; jn++)
-#pragma unroll
-                for (int r = 0; r < 16; r++)
-                    acc[i][jn][r] = 0.0f;
-
-        for (int k0 = 0; k0 < kt; k0 += kKC) {
-            const int kc = kt - k0 < kKC ? kt - k0 : kKC;
-            __syncthreads();
-            // Stage x~ = (x0 - c) [* 2^-e] for [J0 + k0, J0 + k0 + kNW + kc), 8 samples per item.
-            for (int e8 = tid; e8 < (kNW + kc) / 8; e8 += kThreads) {
+            for (int u = 0; u < NB; u++) {
+                const int e8 = e0 + u * kThreads;
+                if (e8 >= items)
+                    continue;
                 const int e = e8 * 8;
-                const float4 *src = reinterpret_cast<const float4 *>(w.x + J0 + k0 + e);
-                const float4 v0 = src[0], v1 = src[1];
-                const float xv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                const float xv[8] = {v[u][0].x, v[u][0].y, v[u][0].z, v[u][0].w,
+                                     v[u][1].x, v[u][1].y, v[u][1].z, v[u][1].w};
                 float xt[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++)
-                    xt[u] = (xv[u] == kMissing ? 0.0f : xv[u]) - c;
+                for (int q = 0; q < 8; q++)
+                    xt[q] = (xv[q] == kMissing ? 0.0f : xv[q]) - c;
                 Frag fh, fl;
                 if (PREC == TMX_PREC_FP16) {
 #pragma unroll
@@ -293,173 +332,260 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                     }
                 }
                 *reinterpret_cast<Frag *>(&lhi[swz(e)]) = fh;
-                if (PREC != TMX_PREC_FP16)
+                if (kSplit)
                     *reinterpret_cast<Frag *>(&llo[swz(e)]) = fl;
             }
-            __syncthreads();
-            // Template operands: loaded at each k-step (the small operand
-            // arrays stay in the L1 cache, and the other wavefront of the
-            // SIMD covers the latency).
-            Frag nh[kNM], nl[kNM];
-            auto load_a = [&](int kappa) {
-#pragma unroll
-                for (int i = 0; i < kNM; i++) {
-                    if (kappa < ks[i]) {
-                        const uint32_t so = 32u * static_cast<uint32_t>(kappa);
-                        nh[i] = __builtin_bit_cast(Frag, __builtin_amdgcn_raw_buffer_load_b128(ar[i], avoff, so, 0));
-                        if (PREC != TMX_PREC_FP16)
-                            nl[i] = __builtin_bit_cast(
-                                Frag, __builtin_amdgcn_raw_buffer_load_b128(ar[i], avoff, so + lo_off[i], 0));
-                    } else {
-                        nh[i] = Frag{{0u, 0u, 0u, 0u}};
-                        nl[i] = Frag{{0u, 0u, 0u, 0u}};
-                    }
-                }
-            };
-            for (int kk = 0; kk < kc; kk += 16) {
-                const int kappa = (k0 + kk) >> 4;
-                load_a(kappa);
-                Frag ah[kNM], al[kNM];
-#pragma unroll
-                for (int i = 0; i < kNM; i++) {
-                    ah[i] = nh[i];
-                    al[i] = nl[i];
-                }
-#pragma unroll
-                for (int jn = 0; jn < (PREC == TMX_PREC_FP16 ? kNN : 0); jn++) {
-                    const int cc = (wave * kNN + jn) * 32 + col;  // column within the block
-                    const int e = swz(32 * cc + kk + 8 * h);
-                    const Frag bh = *reinterpret_cast<const Frag *>(&lhi[e]);
-                    if (PREC == TMX_PREC_FP16) {
-                        f16x8 bv;
-                        __builtin_memcpy(&bv, &bh, 16);
-#pragma unroll
-                        for (int i = 0; i < kNM; i++) {
-                            f16x8 av;
-                            __builtin_memcpy(&av, &ah[i], 16);
-                            acc[i][jn] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc[i][jn], 0, 0, 0);
-                        }
-                    }
-                }
-                if (PREC != TMX_PREC_FP16) {
-                    // Two column tiles at a time: the three products of each
-                    // accumulator (same order as always) interleaved with the
-                    // other three accumulators' products.
-#pragma unroll
-                    for (int jp = 0; jp < kNN; jp += 2) {
-                        bf16x8 bvh[2], bvl[2];
-#pragma unroll
-                        for (int q = 0; q < 2; q++) {
-                            const int cc = (wave * kNN + jp + q) * 32 + col;
-                            const int e = swz(32 * cc + kk + 8 * h);
-                            const Frag bh = *reinterpret_cast<const Frag *>(&lhi[e]);
-                            const Frag bl = *reinterpret_cast<const Frag *>(&llo[e]);
-                            __builtin_memcpy(&bvh[q], &bh, 16);
-                            __builtin_memcpy(&bvl[q], &bl, 16);
-                        }
-                        bf16x8 avh[kNM], avl[kNM];
-#pragma unroll
-                        for (int i = 0; i < kNM; i++) {
-                            __builtin_memcpy(&avh[i], &ah[i], 16);
-                            __builtin_memcpy(&avl[i], &al[i], 16);
-                        }
-#pragma unroll
-                        for (int q = 0; q < 2; q++)
-#pragma unroll
-                            for (int i = 0; i < kNM; i++)
-                                acc[i][jp + q] =
-                                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(avl[i], bvh[q], acc[i][jp + q], 0, 0, 0);
-#pragma unroll
-                        for (int q = 0; q < 2; q++)
-#pragma unroll
-                            for (int i = 0; i < kNM; i++)
-                                acc[i][jp + q] =
-                                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(avh[i], bvl[q], acc[i][jp + q], 0, 0, 0);
-#pragma unroll
-                        for (int q = 0; q < 2; q++)
-#pragma unroll
-                            for (int i = 0; i < kNM; i++)
-                                acc[i][jp + q] =
-                                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(avh[i], bvh[q], acc[i][jp + q], 0, 0, 0);
-                    }
-                }
-            }
         }
+    };
+    // The group's longest template fits one chunk: stage the block once for
+    // every template pair.
+    const bool once = kmax <= kKC;
+    if (once) {
+        stage(0, kmax, std::integral_constant<int, (kStage / 8 + kThreads - 1) / kThreads>{});
+        __syncthreads();
+    }
+    // Byte offset of the wavefront's first column tile in the staged tile.
+    const uint32_t wbase = 2048u * static_cast<uint32_t>(wave * kNN);
 
-        // ---- epilogue (the tile area of LDS is free from here on) -------------
-        // Level 1, while the accumulators are live: FP32 only.  Windows that
-        // pass go to the wavefront's queue (or, if it is full, straight to
-        // the candidate list: extra candidates cost verification time only).
-        int qn = 0;         // wavefront-uniform queue length
-        uint32_t ovf = 0;   // tiles (i * kNN + jn) whose survivors overflowed the queue
-        __syncthreads();  // the epilogue scratch overlays the tile other wavefronts were reading
+    // The two wavefronts of a SIMD (wave and wave + 4) take the template
+    // pairs in opposite orders when nothing makes them wait for each other
+    // (one staged chunk): pairs differ in length, so one wavefront's epilogue
+    // falls in the other's matrix-core work instead of both idling the
+    // matrix core at once.
+    const int npairs = (te - tb + kNM - 1) / kNM;
+    const bool rev = once && wave >= kWaves / 2;
+    for (int pi = 0; pi < npairs; pi++) {
+        const int tp = tb + kNM * (rev ? npairs - 1 - pi : pi);
+        const int nt = te - tp < kNM ? te - tp : kNM;
+        // Template operands through buffer resources (hi and lo halves of the
+        // shifted copies).  Past a template's own K the k-step is clamped to
+        // its K, whose fragments read only zero padding, so the shorter
+        // template of a pair needs no branch; a template the early-abandoning
+        // scan takes, or the missing second template of an odd group, has no
+        // records (every read is zero).
+        __amdgpu_buffer_rsrc_t rh[kNM], rl[kNM];
+        int ks[kNM];
+        int kt = 0;
 #pragma unroll
         for (int i = 0; i < kNM; i++) {
+            const int ti = tp + (i < nt ? i : 0);
+            const bool live = i < nt && !tmpl[ti].ea;
+            const int L = tmpl[ti].kpad + 64;
+            const int bytes = live ? 16 * L : 0;
+            uint16_t *base = const_cast<uint16_t *>(reinterpret_cast<const uint16_t *>(afrag + tmpl[ti].a_off));
+            rh[i] = __builtin_amdgcn_make_buffer_rsrc(base, static_cast<short>(0), bytes, 0x00020000);
+            rl[i] = __builtin_amdgcn_make_buffer_rsrc(base + 8 * L, static_cast<short>(0), bytes, 0x00020000);
+            ks[i] = tmpl[ti].kpad / 16;
+            if (live)
+                kt = tmpl[ti].kpad > kt ? tmpl[ti].kpad : kt;
+        }
+        // The lane's copy c = (-col) mod 8 and chunk at k-step 0.
+        const int cs = (8 - (col & 7)) & 7;
+        const uint32_t avoff = static_cast<uint32_t>(((32 + 8 * h - col - cs) / 8) * 8 + cs) * 16u;
+        f32x16 acc[kNM][kNN];
+#pragma unroll
+        for (int i = 0; i < kNM; i++)
+#pragma unroll
+            for (int jn = 0; jn < kNN; jn++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    acc[i][jn][r] = 0.0f;
+
+        for (int k0 = 0; k0 < kt; k0 += kKC) {
+            const int kc = kt - k0 < kKC ? kt - k0 : kKC;
+            if (!once) {
+                __syncthreads();
+                stage(k0, kc, std::integral_constant<int, 3>{});
+                __syncthreads();
+            }
+            // A fragments of k-step kappa (absolute): 8 consecutive template
+            // values at entry 32 + 16 kappa + 8 h - col of each half.
+            auto load_a = [&](int kappa, bf16x8 (&oh)[kNM], bf16x8 (&ol)[kNM]) {
+#pragma unroll
+                for (int i = 0; i < kNM; i++) {
+                    const uint32_t so = 256u * static_cast<uint32_t>(kappa < ks[i] ? kappa : ks[i]);
+                    oh[i] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rh[i], avoff, so, 0));
+                    if (kSplit)
+                        ol[i] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rl[i], avoff, so, 0));
+                }
+            };
+            // B fragments of column tile jn at chunk-relative k-step offset kk.
+            auto lds_addr = [&](int kk) { return wbase + 2u * static_cast<uint32_t>(swz(32 * col + 8 * h + kk)); };
+            auto load_b = [&](uint32_t ad, int jn, bf16x8 &bh, bf16x8 &bl) {
+                bh = *reinterpret_cast<const bf16x8 *>(lds + ad + 2048u * jn);
+                if (kSplit)
+                    bl = *reinterpret_cast<const bf16x8 *>(lds + ad + 2048u * jn + kLoBytes);
+            };
+            // Two sets of template operands, alternating by k-step, so the
+            // next k-step's loads need no register copies.
+            bf16x8 a0h[kNM], a0l[kNM], a1h[kNM], a1l[kNM], bh[kNN], bl[kNN];
+            {
+                const uint32_t ad = lds_addr(0);
+#pragma unroll
+                for (int jn = 0; jn < kNN; jn++)
+                    load_b(ad, jn, bh[jn], bl[jn]);
+                load_a(k0 >> 4, a0h, a0l);
+            }
+            // One k-step on operands (ah, al); with PF the operands of the
+            // next are loaded meanwhile (template operands into (nh, nl)
+            // first, each column-tile pair's waveform fragments after its
+            // last product).
+            auto kstep = [&](int kk, bf16x8 (&ah)[kNM], bf16x8 (&al)[kNM], bf16x8 (&nh)[kNM], bf16x8 (&nl)[kNM],
+                             auto pf) __attribute__((always_inline)) {
+                constexpr bool PF = decltype(pf)::value;
+                uint32_t adn = 0;
+                if constexpr (PF) {
+                    load_a(((k0 + kk) >> 4) + 1, nh, nl);
+                    adn = lds_addr(kk + 16);
+                }
+#pragma unroll
+                for (int jp = 0; jp < kNN; jp += 2) {
+                    if constexpr (kSplit) {
+                        // The three products of each accumulator in the same
+                        // order always (lo*hi, hi*lo, hi*hi), interleaved
+                        // over four accumulators.
+#pragma unroll
+                        for (int q = 0; q < 2; q++)
+#pragma unroll
+                            for (int i = 0; i < kNM; i++)
+                                acc[i][jp + q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[jp + q],
+                                                                                         acc[i][jp + q], 0, 0, 0);
+#pragma unroll
+                        for (int q = 0; q < 2; q++)
+#pragma unroll
+                            for (int i = 0; i < kNM; i++)
+                                acc[i][jp + q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[jp + q],
+                                                                                         acc[i][jp + q], 0, 0, 0);
+#pragma unroll
+                        for (int q = 0; q < 2; q++)
+#pragma unroll
+                            for (int i = 0; i < kNM; i++)
+                                acc[i][jp + q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[jp + q],
+                                                                                         acc[i][jp + q], 0, 0, 0);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 2; q++)
+#pragma unroll
+                            for (int i = 0; i < kNM; i++) {
+                                f16x8 av, bv;
+                                __builtin_memcpy(&av, &ah[i], 16);
+                                __builtin_memcpy(&bv, &bh[jp + q], 16);
+                                acc[i][jp + q] =
+                                    __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc[i][jp + q], 0, 0, 0);
+                            }
+                    }
+                    if constexpr (PF) {
+#pragma unroll
+                        for (int q = 0; q < 2; q++)
+                            load_b(adn, jp + q, bh[jp + q], bl[jp + q]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            };
+            // kc is a multiple of 32 (padded K and chunk length): k-steps in pairs.
+            using Pf = std::integral_constant<bool, true>;
+            using NoPf = std::integral_constant<bool, false>;
+            int kk = 0;
+            for (; kk + 32 < kc; kk += 32) {
+                kstep(kk, a0h, a0l, a1h, a1l, Pf{});
+                kstep(kk + 16, a1h, a1l, a0h, a0l, Pf{});
+            }
+            kstep(kk, a0h, a0l, a1h, a1l, Pf{});
+            kstep(kk + 16, a1h, a1l, a0h, a0l, NoPf{});
+        }
+
+        // ---- epilogue: registers and global memory only ----------------------
+        // Accumulator layout: lane (col, h), entry r holds window
+        // tile0 + jl + 8 (r / 4) + r % 4, jl = 32 col + 4 h.  (jl is rebuilt
+        // opaquely per pair so that its 16 offsets are not hoisted out of the
+        // template loop into registers the K loop needs.)
+        int jl = 32 * col + 4 * h;
+        asm volatile("" : "+v"(jl));
+        const uint32_t vl = static_cast<uint32_t>(jl) * 8u;  // lane's byte offset in the prefix pairs
+        float best_g[kNM], best_dt[kNM];
+        uint32_t best_j[kNM];
+        // One template's four tiles.
+        auto epilogue = [&](int i, const f32x16 &a0, const f32x16 &a1, const f32x16 &a2,
+                            const f32x16 &a3) __attribute__((always_inline)) {
+            best_g[i] = -INFINITY;
+            best_dt[i] = 0.0f;
+            best_j[i] = 0xffffffffu;
             if (i >= nt)
-                continue;
+                return;
             const int t = tp + i;
             const int m = tmpl[t].m;
             if (static_cast<uint64_t>(m) > w.n || tmpl[t].ea)
-                continue;
+                return;
             const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
-            const double Q2 = tmpl[t].Q2;
-            const double md = static_cast<double>(m);
-            const double eta = eta_mfma(PREC, tmpl[t].kpad);
-            // FP16 floor terms: staged |x~| * 2^-ex below 2^-14 and |q| * 2^7
-            // below 2^-14 may be flushed.  x side: sum |q| 2^-14 2^ex <=
-            // 2^-14 2^ex sqrt(m Q2) (absolute); q side: sum 2^-21 |x~| <=
-            // 2^-21 sqrt(m) sqrt(sum x~^2) (relative to the window).
-            const double abs_unit = PREC == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;  // 2^-21
-            const float gmin =
-                static_cast<float>((Q2 + md - static_cast<double>(tmpl[t].bound) - eref_cap(m)) * 0.5 * (1.0 - 1e-3));
-            const float eta_sq = static_cast<float>(eta * sqrt(Q2) * (1.0 + 1e-3));
-            const float sqrt_m = static_cast<float>(sqrt(md));
+            const L1Consts k1 = tmpl[t].l1;
+            const float gmin = k1.gmin, eta_sq = k1.eta_sq, sqrt_m = k1.sqrt_m, inv_mf = k1.inv_mf;
+            const float rsqrt_m = k1.rsqrt_m, absS = k1.absS, eabs_q = k1.eabs_q, Kc = k1.Kc;
             const float mf = static_cast<float>(m);
-            const float inv_mf = 1.0f / mf;
-            const float rsqrt_m = static_cast<float>(1.0 / sqrt(md)) * (1.0f + 1e-6f);
-            const float absS = static_cast<float>(fabs(tmpl[t].S) * (1.0 + 1e-3));
-            const float eabs_q = static_cast<float>(abs_unit * sqrt(md) * 1.01);  // * sqrt(sum x~^2)
-            const float eabs_x = PREC == TMX_PREC_FP16
-                                     ? static_cast<float>(6.103515625e-05 * static_cast<double>(scale) * 128.0 *
-                                                          sqrt(md * Q2) * 1.01)
-                                     : 0.0f;  // absolute
-            float best_g = -INFINITY;
-            uint32_t best_j = 0xffffffffu;
-            float best_dt = 0.0f;
-            // Tile test constants: survive iff dt >= K sqrt(V) - rest, V the
-            // window's m sigma^2 (the per-window threshold below, rewritten).
-            const float Kc = gmin * rsqrt_m * 0.999f - (eta_sq + eabs_q) * 1.001f;
+            const float eabs_x = k1.exq * scale;  // FP16 only (absolute)
+            // Per window the level-1 test is dt >= Kc sqrt(V) - rest(aD),
+            // V the window's m sigma^2 and aD >= |mu - c|; rest uses the
+            // block's largest |x~| for aD.
             const float rest = ((eta_sq + eabs_q) * sqrt_m * aD_max + aD_max * absS + eabs_x) * 1.001f + 1e-30f;
             const float amu_max = fabsf(c) + aD_max;
+            // Window sums of x0 - c and (x0 - c)^2 over a window from its
+            // block-rebased prefix pairs, with their error bounds: the
+            // window's m sigma^2, a bound on |mu - c|, and whether the FP32
+            // statistics can judge it (else rounding swamps them, or
+            // |mu| / sigma > 1e3).
+            struct Win {
+                float V, aD;
+                bool judge;
+            };
+            auto window = [&](float2 ra, float2 rz) __attribute__((always_inline)) {
+                const float S1 = rz.x - ra.x, S2 = rz.y - ra.y;
+                const float eS1 = 2.0f * bi.y + 1.2e-7f * (fabsf(ra.x) + fabsf(rz.x) + fabsf(S1));
+                const float eS2 = 2.0f * bi.z + 1.2e-7f * (fabsf(ra.y) + fabsf(rz.y) + fabsf(S2));
+                Win r;
+                r.V = S2 - S1 * S1 * inv_mf;
+                const float eV = eS2 + (2.0f * fabsf(S1) * eS1 + eS1 * eS1) * inv_mf +
+                                 2.4e-7f * (fabsf(S2) + S1 * S1 * inv_mf);
+                r.aD = fabsf(S1 * inv_mf) + eS1 * inv_mf;
+                const float amu = fabsf(c) + r.aD;
+                r.judge = r.V > 1e3f * eV && r.V > mf * 1e-20f && !(amu * amu * mf > 1e6f * r.V);
+                return r;
+            };
+            // Windows of tile jn that exist, [0, Wt) relative to its first, and
+            // the column's core prefix pairs (loaded for all four tiles at once).
+            auto tile_wt = [&](int jn) __attribute__((always_inline)) {
+                const uint64_t tile0 = J0 + static_cast<uint64_t>(1024 * (wave * kNN + jn));
+                return W > tile0 ? (W - tile0 < 1024 ? static_cast<int>(W - tile0) : 1024) : 0;
+            };
+            const int c0 = jl - 4 * h;  // the lane's column's first window, relative to its tile
+            float2 cra[kNN], crz[kNN];
+            if (mode == kModeMain) {
 #pragma unroll
-            for (int jn = 0; jn < kNN; jn++) {
-                const uint64_t tile0 = J0 + 32ull * static_cast<uint64_t>((wave * kNN + jn) * 32);
+                for (int jn = 0; jn < kNN; jn++) {
+                    const int Wt = tile_wt(jn);
+                    const int cnt = Wt - c0 < 32 ? (Wt - c0 > 1 ? Wt - c0 : 1) : 32;
+                    const uint32_t vc = static_cast<uint32_t>(1024 * (wave * kNN + jn) + c0) * 8u;
+                    cra[jn] = rbl(vc + 8u * static_cast<uint32_t>(cnt - 1), 0u);
+                    crz[jn] = rbl(vc, 8u * static_cast<uint32_t>(m));
+                }
+            }
+            auto tile = [&](const f32x16 &av, int jn) __attribute__((always_inline)) {
+                const int toff = 1024 * (wave * kNN + jn);  // tile's first window, relative to J0
+                const uint64_t tile0 = J0 + static_cast<uint64_t>(toff);
+                const int Wt = tile_wt(jn);
                 if (mode == kModeMain) {
-                    // Tile test.  Column col holds windows j0 + s, s < 32 (this
-                    // lane and lane ^ 32 hold 16 each); their common core
-                    // [j0 + cnt - 1, j0 + m) bounds every window's m sigma^2
-                    // from below, |x~| bounds |mu - c|, and the column's
-                    // largest dot product bounds every dt.  A tile none of
-                    // whose columns can reach the threshold is skipped.
-                    float cm = -INFINITY;
-                    bool nonfinite = false;
-#pragma unroll
-                    for (int r = 0; r < 16; r++) {
-                        const float v = acc[i][jn][r];
-                        nonfinite |= !(v - v == 0.0f);
-                        cm = fmaxf(cm, v);
-                    }
-                    cm = fmaxf(cm, __shfl_xor(cm, 32)) * scale;
-                    nonfinite = __shfl_xor(static_cast<int>(nonfinite), 32) != 0 || nonfinite;
-                    const uint64_t j0 = tile0 + 32ull * static_cast<uint64_t>(col);
-                    bool surv = false;
-                    if (j0 < W) {
-                        surv = true;
-                        const int cnt = W - j0 < 32 ? static_cast<int>(W - j0) : 32;
+                    // Column threshold.  Column col holds windows j0 + s, s <
+                    // 32 (this lane and lane ^ 32 hold 16 each); their common
+                    // core [j0 + cnt - 1, j0 + m) bounds every window's
+                    // m sigma^2 from below and |x~| bounds |mu - c|, so tc is
+                    // at most the level-1 threshold of each of the column's
+                    // windows: a window whose dot product is below tc cannot
+                    // survive.
+                    float tc = INFINITY;
+                    if (c0 < Wt) {
+                        tc = -INFINITY;
+                        const int cnt = Wt - c0 < 32 ? Wt - c0 : 32;
                         const int Lc = m - (cnt - 1);
-                        if (!nonfinite && Lc >= 2 && gmin > 0.0f && Kc > 0.0f) {
-                            const float2 ra = rbb[static_cast<int>(j0 - J0) + cnt - 1];
-                            const float2 rz = rbb[static_cast<int>(j0 - J0) + m];
+                        if (Lc >= 2 && gmin > 0.0f && Kc > 0.0f) {
+                            const float2 ra = cra[jn], rz = crz[jn];
                             const float S1 = rz.x - ra.x, S2 = rz.y - ra.y;
                             const float eS1 = 2.0f * bi.y + 1.2e-7f * (fabsf(ra.x) + fabsf(rz.x) + fabsf(S1));
                             const float eS2 = 2.0f * bi.z + 1.2e-7f * (fabsf(ra.y) + fabsf(rz.y) + fabsf(S2));
@@ -468,175 +594,113 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
                             const float eV = eS2 + (2.0f * fabsf(S1) * eS1 + eS1 * eS1) * invL +
                                              2.4e-7f * (fabsf(S2) + S1 * S1 * invL);
                             const float Vlo = V - eV;
-                            if (Vlo > 1e3f * eV && Vlo > mf * 1e-20f && !(amu_max * amu_max * mf > 1e6f * Vlo)) {
-                                const float thr = Kc * (Vlo * __frsqrt_rn(Vlo)) * 0.999f - rest;
-                                surv = !(cm < thr);
-                            }
+                            if (Vlo > 1e3f * eV && Vlo > mf * 1e-20f && !(amu_max * amu_max * mf > 1e6f * Vlo))
+                                tc = Kc * (Vlo * __frsqrt_rn(Vlo)) * 0.999f - rest;
                         }
                     }
-                    if (__ballot(surv) == 0ull)
-                        continue;
-                }
-                // Per window of the tile, coalesced, from the block-rebased
-                // prefix pairs: main mode, the threshold the dot product must
-                // reach to survive level 1 (-inf: cannot be judged - stats
-                // swamped by rounding or |mu| / sigma > 1e3 - so it survives;
-                // +inf: past the last window); seed mode, 1 / sqrt(m sigma^2)
-                // (or -1).
-                wave_sync();
+                    uint32_t pre = 0u;  // entries that pass the column threshold
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const int wl = jl + 8 * (r >> 2) + (r & 3);
+                        if (wl < Wt && av[r] * scale >= tc)
+                            pre |= 1u << r;
+                    }
+                    if (__ballot(pre != 0u) == 0ull)
+                        return;
+                    // The window's own threshold, only for entries some lane
+                    // still holds (-inf: cannot be judged, so it survives).
+                    uint32_t sm = 0u;
 #pragma unroll 1
-                for (int it0 = 0; it0 < 16; it0 += kStatBatch) {
-                    float2 ra[kStatBatch], rz[kStatBatch];
-#pragma unroll
-                    for (int u = 0; u < kStatBatch; u++) {
-                        const int i0 = static_cast<int>(tile0 - J0) + (it0 + u) * 64 + lane;
-                        ra[u] = rbb[i0];
-                        rz[u] = rbb[i0 + m];
-                    }
-#pragma unroll
-                    for (int u = 0; u < kStatBatch; u++) {
-                        const int wl = (it0 + u) * 64 + lane;
-                        const uint64_t j = tile0 + static_cast<uint64_t>(wl);
-                        float thr = INFINITY;
-                        if (j < W) {
-                            // Window sums of x0 - c and (x0 - c)^2 with their error bounds.
-                            const float S1 = rz[u].x - ra[u].x, S2 = rz[u].y - ra[u].y;
-                            const float eS1 = 2.0f * bi.y + 1.2e-7f * (fabsf(ra[u].x) + fabsf(rz[u].x) + fabsf(S1));
-                            const float eS2 = 2.0f * bi.z + 1.2e-7f * (fabsf(ra[u].y) + fabsf(rz[u].y) + fabsf(S2));
-                            const float V = S2 - S1 * S1 * inv_mf;
-                            const float eV = eS2 + (2.0f * fabsf(S1) * eS1 + eS1 * eS1) * inv_mf +
-                                             2.4e-7f * (fabsf(S2) + S1 * S1 * inv_mf);
-                            const float aD = fabsf(S1 * inv_mf) + eS1 * inv_mf;  // >= |mu - c|
-                            const float amu = fabsf(c) + aD;
-                            const bool judge = V > 1e3f * eV && V > mf * 1e-20f && !(amu * amu * mf > 1e6f * V);
-                            if (mode == kModeMain) {
-                                if (!judge || gmin <= 0.0f) {
-                                    thr = -INFINITY;
-                                } else {
-                                    const float sV = V * __frsqrt_rn(V);
-                                    const float b = eta_sq * (sV + sqrt_m * aD) + aD * absS +
-                                                    eabs_q * (sV + sqrt_m * aD) + eabs_x;
-                                    thr = gmin * sV * rsqrt_m * 0.999f - b * 1.001f - 1e-30f;
-                                }
-                            } else {
-                                thr = judge ? __frsqrt_rn(V) : -1.0f;
+                    for (int r = 0; r < 16; r++) {
+                        if (__ballot((pre >> r) & 1u) == 0ull)
+                            continue;
+                        const float dt = av[r] * scale;
+                        if ((pre >> r) & 1u) {
+                            const uint32_t so0 = static_cast<uint32_t>(toff + 8 * (r >> 2) + (r & 3)) * 8u;
+                            const Win wv = window(rbl(vl, so0), rbl(vl, so0 + 8u * static_cast<uint32_t>(m)));
+                            float thr = -INFINITY;
+                            if (wv.judge) {
+                                const float sV = wv.V * __frsqrt_rn(wv.V);
+                                const float b = eta_sq * (sV + sqrt_m * wv.aD) + wv.aD * absS +
+                                                eabs_q * (sV + sqrt_m * wv.aD) + eabs_x;
+                                thr = gmin * sV * rsqrt_m * 0.999f - b * 1.001f - 1e-30f;
                             }
+                            if (dt >= thr)
+                                sm |= 1u << r;
                         }
-                        stats[wl + (wl >> 5)] = thr;
                     }
-                }
-                // The tile's dot products go to LDS (this tile's accumulators
-                // die here) and the level-1 test runs as a loop.
+                    if (__ballot(sm != 0u) == 0ull)
+                        return;
+                    // Survivors to the queue, one atomic per wavefront and
+                    // tile: the lane's exclusive prefix of the survivor
+                    // counts (at most 16, five bits) from one ballot per bit.
+                    const int cnt = __popc(sm);
+                    uint32_t excl = 0u, total = 0u;
 #pragma unroll
-                for (int r = 0; r < 16; r++)
-                    sdt[r * 64 + lane] = acc[i][jn][r] * scale;
-                wave_sync();
-#pragma unroll 2
-                for (int r = 0; r < 16; r++) {
-                    const int wl = 32 * col + 8 * (r >> 2) + 4 * h + (r & 3);
-                    const float thr = stats[wl + (wl >> 5)];
-                    const float dt = sdt[r * 64 + lane];
-                    if (mode == kModeMain) {
-                        const bool survive = dt >= thr;
-                        const uint64_t mask = __ballot(survive);
-                        if (mask != 0ull) {
-                            // Survivors go to the queue; a tile whose survivors do
-                            // not fit is marked and, once the accumulators are
-                            // dead, all its windows become candidates (extra
-                            // candidates cost verification time only).
-                            const int idx = qn + __popcll(mask & ((1ull << lane) - 1ull));
-                            if (survive && idx < kQ) {
-                                qj[idx] = static_cast<uint32_t>(tile0 + static_cast<uint64_t>(wl));
-                                qdt[idx] = dt;
-                                qi[idx] = i * kNN + jn;
-                            }
-                            qn += __popcll(mask);
-                            if (qn > kQ)
-                                ovf |= 1u << (i * kNN + jn);
+                    for (int b = 0; b < 5; b++) {
+                        const unsigned long long bm = __ballot((cnt >> b) & 1);
+                        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(bm >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bm), 0u));
+                        excl += below << b;
+                        total += static_cast<uint32_t>(__popcll(bm)) << b;
+                    }
+                    unsigned long long base = 0ull;
+                    if (lane == 0)
+                        base = atomicAdd(so.count, static_cast<unsigned long long>(total));
+                    base = __shfl(base, 0);
+                    unsigned long long slot = base + excl;
+                    const uint32_t jb = static_cast<uint32_t>(tile0) + static_cast<uint32_t>(jl);
+#pragma unroll 1
+                    for (int r = 0; r < 16; r++) {
+                        const float dt = av[r] * scale;
+                        if ((sm >> r) & 1u) {
+                            if (slot < so.cap)
+                                so.rec[slot] = SurvRec{static_cast<uint32_t>(t),
+                                                       jb + static_cast<uint32_t>(8 * (r >> 2) + (r & 3)), dt, scale};
+                            slot++;
                         }
-                    } else if (thr > 0.0f && thr < INFINITY) {
-                        const float gapprox = dt * thr;
-                        if (gapprox > best_g) {
-                            best_g = gapprox;
-                            best_j = static_cast<uint32_t>(tile0 + static_cast<uint64_t>(wl));
-                            best_dt = dt;
+                    }
+                } else {
+                    // Seed mode: the lane's most promising window by
+                    // dt / sqrt(m sigma^2), over the windows it can judge.
+#pragma unroll 1
+                    for (int r = 0; r < 16; r++) {
+                        const int wl = jl + 8 * (r >> 2) + (r & 3);
+                        const uint32_t so0 = static_cast<uint32_t>(toff + 8 * (r >> 2) + (r & 3)) * 8u;
+                        const Win wv = window(rbl(vl, so0), rbl(vl, so0 + 8u * static_cast<uint32_t>(m)));
+                        const float rs = __frsqrt_rn(wv.V);
+                        const float dt = av[r] * scale;
+                        const float gapprox = dt * rs;
+                        if (wl < Wt && wv.judge && rs > 0.0f && rs < INFINITY && gapprox > best_g[i]) {
+                            best_g[i] = gapprox;
+                            best_j[i] = static_cast<uint32_t>(tile0) + static_cast<uint32_t>(wl);
+                            best_dt[i] = dt;
                         }
                     }
                 }
-            }
-            if (mode == kModeSeed) {
-                qj[i * 64 + lane] = best_j;
-                qdt[i * 64 + lane] = best_dt;
-            }
-        }
-        __syncthreads();
+            };
+            tile(a0, 0);
+            tile(a1, 1);
+            tile(a2, 2);
+            tile(a3, 3);
+        };
+        static_assert(kNM == 2 && kNN == 4, "the epilogue calls name every accumulator");
+        epilogue(0, acc[0][0], acc[0][1], acc[0][2], acc[0][3]);
+        epilogue(1, acc[1][0], acc[1][1], acc[1][2], acc[1][3]);
 
-        // Level 2: the rigorous FP64 interval of filter.hpp, accumulators dead.
-        if (mode == kModeMain) {
-            // Overflowed tiles: every valid window is a candidate.
-            for (int b = 0; b < kNM * kNN; b++) {
-                if (!((ovf >> b) & 1u))
+        // Seed mode: each lane's most promising window of each template,
+        // evaluated rigorously (accumulators dead); the wavefront emits the
+        // smallest upper bound.
+        if (mode == kModeSeed) {
+#pragma unroll
+            for (int i = 0; i < kNM; i++) {
+                if (i >= nt)
                     continue;
-                const int t = tp + b / kNN;
-                const int m = tmpl[t].m;
-                const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
-                const uint64_t tile0 = J0 + 32ull * static_cast<uint64_t>((wave * kNN + b % kNN) * 32);
-                for (int wl = lane; wl < kTileWin; wl += 64) {
-                    const uint64_t j = tile0 + static_cast<uint64_t>(wl);
-                    if (j >= W || w.x[j] == kMissing)
-                        continue;
-                    cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN,
-                             -INFINITY, INFINITY);
-                }
-            }
-            const int qe = qn < kQ ? qn : kQ;
-            for (int e = lane; e < qe; e += 64) {
-                if ((ovf >> qi[e]) & 1u)
-                    continue;
-                const int t = tp + qi[e] / kNN;
-                const uint64_t j = qj[e];
-                if (w.x[j] == kMissing)
-                    continue;
-                const int m = tmpl[t].m;
-                const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
-                const double md = static_cast<double>(m);
-                const double abs_unit = PREC == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;
-                const double ea = eta_mfma(PREC, tmpl[t].kpad) + abs_unit * sqrt(md / Q2) * 1.01;
-                DistInterval iv = dist_interval(w, j, m, Q2, S, c, static_cast<double>(qdt[e]), ea);
-                if (PREC == TMX_PREC_FP16 && iv.lo > -INFINITY) {
-                    const WindowSums ws = window_sums_fast(w, j, j + static_cast<uint64_t>(m));
-                    const double S1 = ws.S1;
-                    const double V = ws.S2 - S1 * S1 / md;
-                    const double sig = sqrt(fmax(V, 1e-300) / md);
-                    const double ex = 6.103515625e-05 * static_cast<double>(scale) * 128.0 * sqrt(md * Q2) * 1.01;
-                    iv.lo -= 2.0 * ex / (sig * (1.0 - 1e-3)) * 1.01;
-                }
-                if (iv.lo <= static_cast<double>(tmpl[t].bound))
-                    cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j),
-                             static_cast<float>(iv.est), iv.lo, iv.hi);
-            }
-        } else {
-            for (int i = 0; i < nt; i++) {
                 const int t = tp + i;
-                const int m = tmpl[t].m;
                 float ub = INFINITY;
-                const uint32_t bj0 = qj[i * 64 + lane];
-                if (static_cast<uint64_t>(m) <= w.n && !tmpl[t].ea && bj0 != 0xffffffffu && w.x[bj0] != kMissing) {
-                    const double Q2 = tmpl[t].Q2, S = tmpl[t].S;
-                    const double md = static_cast<double>(m);
-                    const double abs_unit = PREC == TMX_PREC_FP16 ? 4.76837158203125e-07 : 0.0;
-                    const double ea = eta_mfma(PREC, tmpl[t].kpad) + abs_unit * sqrt(md / Q2) * 1.01;
-                    DistInterval iv =
-                        dist_interval(w, bj0, m, Q2, S, c, static_cast<double>(qdt[i * 64 + lane]), ea);
-                    if (PREC == TMX_PREC_FP16 && iv.hi < INFINITY) {
-                        const WindowSums ws = window_sums_fast(w, bj0, bj0 + static_cast<uint64_t>(m));
-                        const double S1 = ws.S1;
-                        const double V = ws.S2 - S1 * S1 / md;
-                        const double sig = sqrt(fmax(V, 1e-300) / md);
-                        const double ex =
-                            6.103515625e-05 * static_cast<double>(scale) * 128.0 * sqrt(md * Q2) * 1.01;
-                        iv.hi += 2.0 * ex / (sig * (1.0 - 1e-3)) * 1.01;
-                    }
+                const uint32_t bj0 = best_j[i];
+                if (bj0 != 0xffffffffu && w.x[bj0] != kMissing) {
+                    const DistInterval iv = survivor_interval<PREC>(w, bj0, tmpl[t], c, best_dt[i], scale);
                     if (iv.hi < 3.0e38)
                         ub = static_cast<float>(iv.hi) * (1.0f + 1e-6f);
                 }
@@ -656,21 +720,48 @@ __global__ __launch_bounds__(kThreads, 2) void k_scan_mfma(Waveform w, const Tmp
     }
 }
 
+// Level 2: the rigorous FP64 interval of every level-1 survivor; windows
+// whose lower bound reaches the template's bound become candidates.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_level2(Waveform w, const TmplDev *__restrict__ tmpl,
+                                                const float4 *__restrict__ binfo, SurvOut so, CandOut co)
+{
+    const unsigned long long cnt = *so.count;
+    const uint64_t n = cnt < so.cap ? cnt : so.cap;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const SurvRec s = so.rec[e];
+        const uint64_t j = s.j;
+        if (w.x[j] == kMissing)
+            continue;
+        const TmplDev &tp = tmpl[s.t];
+        // The block that produced it: its centre is the one the dot product used.
+        const float c = binfo[j / static_cast<uint64_t>(kNW)].x;
+        const DistInterval iv = survivor_interval<PREC>(w, j, tp, c, s.dt, s.scale);
+        if (iv.lo <= static_cast<double>(tp.bound))
+            cand_put(co, wave_claim(co.count), s.t, s.j, static_cast<float>(iv.est), iv.lo, iv.hi);
+    }
+}
+
 }  // namespace
 
 int mfma_group_size() { return kGroup; }
 uint64_t mfma_block_windows() { return kNW; }
-int mfma_kpad(int m) { return ((m + 31 + 15) / 16) * 16; }
-// Two 16-bit arrays (hi, lo) of kpad + 64 entries, in 16-byte units.
-int64_t mfma_frag_count(int kpad) { return static_cast<int64_t>(kpad + 64) / 4; }
+// K of a template: its m + 31 Toeplitz columns, padded to whole pairs of
+// k-steps (the K loop runs k-steps in pairs).
+int mfma_kpad(int m) { return ((m + 31 + 31) / 32) * 32; }
+static_assert(kKC % 32 == 0, "chunks hold whole pairs of k-steps");
+// Two halves (hi, lo) of 8 shifted copies of kpad + 64 16-bit entries, in
+// 16-byte units.
+int64_t mfma_frag_count(int kpad) { return 2 * static_cast<int64_t>(kpad + 64); }
 
-hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, const float *qz, int precision, void *a_frag,
-                            hipStream_t stream)
+hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, int max_kpad, const float *qz, int precision,
+                            void *a_frag, hipStream_t stream)
 {
     if (count <= 0)
         return hipSuccess;
-    // Longest template: kpad = 65536 + 32 (+ 64 padding entries).
-    const int per_t = 65536 + 32 + 64;
+    // Entries per half of the longest template: 8 copies of kpad + 64.
+    const int per_t = 8 * (max_kpad + 64);
     const dim3 grid((per_t + 255) / 256, count);
     hipLaunchKernelGGL(k_pack, grid, dim3(256), 0, stream, tmpl, count, qz, precision, static_cast<Frag *>(a_frag));
     return hipGetLastError();
@@ -678,7 +769,7 @@ hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, const float *qz, int
 
 hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, const int32_t *group_begin, int groups,
                             int precision, const void *a_frag, const float2 *rb, const float4 *binfo, int rb_len,
-                            int mode, int block_stride, const CandOut &co, hipStream_t stream)
+                            int mode, int block_stride, const SurvOut &so, const CandOut &co, hipStream_t stream)
 {
     (void)count;
     if (groups <= 0 || w.n == 0)
@@ -692,10 +783,29 @@ hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, c
     const Frag *a = static_cast<const Frag *>(a_frag);
     if (precision == TMX_PREC_FP16)
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_FP16>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a, mode,
-                           block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, co);
+                           block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, so, co);
     else
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_BF16X3>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a,
-                           mode, block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, co);
+                           mode, block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, so, co);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma_consts(TmplDev *tmpl, int count, int precision, hipStream_t stream)
+{
+    if (count <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_mfma_consts, dim3((count + 63) / 64), dim3(64), 0, stream, tmpl, count, precision);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma_level2(const Waveform &w, const TmplDev *tmpl, int precision, const float4 *binfo,
+                              const SurvOut &so, const CandOut &co, int cus, hipStream_t stream)
+{
+    const dim3 grid(static_cast<unsigned>((cus > 0 ? cus : 256) * 8));
+    if (precision == TMX_PREC_FP16)
+        hipLaunchKernelGGL(k_level2<TMX_PREC_FP16>, grid, dim3(256), 0, stream, w, tmpl, binfo, so, co);
+    else
+        hipLaunchKernelGGL(k_level2<TMX_PREC_BF16X3>, grid, dim3(256), 0, stream, w, tmpl, binfo, so, co);
     return hipGetLastError();
 }
 

@@ -232,7 +232,43 @@ __global__ void k_block_prefix(const double2 *__restrict__ p, const double2 *__r
     }
 }
 
+// Largest |x0 - c_b| over the len samples of each block b (x0 = x with
+// missing samples as 0, the zero padding past n included), into binfo[b].w
+// (float bits, non-negative: an integer max), after k_block_prefix set c_b.
+__global__ void k_block_absmax(const float *__restrict__ x, int block, int len, float4 *__restrict__ binfo)
+{
+    const uint64_t b = blockIdx.y;
+    const uint64_t J0 = b * static_cast<uint64_t>(block);
+    const float c = binfo[b].x;
+    float mx = 0.0f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) {
+        float v = x[J0 + static_cast<uint64_t>(i)];
+        v = v == kMissing ? 0.0f : v;
+        mx = fmaxf(mx, fabsf(v - c));
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(reinterpret_cast<unsigned int *>(&binfo[b].w), __float_as_uint(mx));
+}
+
 }  // namespace
+
+hipError_t launch_block_absmax(const float *x, uint64_t n, int block, int len, float4 *binfo, hipStream_t stream)
+{
+    const uint64_t blocks = (n + block - 1) / block;
+    if (blocks == 0)
+        return hipErrorInvalidValue;
+    unsigned gx = static_cast<unsigned>((len + 255) / 256 < 16 ? (len + 255) / 256 : 16);
+    if (gx == 0)
+        gx = 1;
+    for (uint64_t b0 = 0; b0 < blocks; b0 += 65535) {
+        const uint64_t nb = blocks - b0 < 65535 ? blocks - b0 : 65535;
+        hipLaunchKernelGGL(k_block_absmax, dim3(gx, static_cast<unsigned>(nb)), dim3(256), 0, stream,
+                           x + b0 * block, block, len, binfo + b0);
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_block_prefix(const double2 *p, const double2 *pl, uint64_t n, int block, int len, float4 *binfo,
                                float2 *rb, hipStream_t stream)

@@ -43,11 +43,14 @@ hipError_t launch_seed_bound(const uint64_t *keys, const uint32_t *offsets, Tmpl
 hipError_t launch_knn_select(const uint64_t *keys, const uint32_t *offsets, const TmplDev *tmpl, int T, int max_k,
                              uint64_t *out_pos, float *out_d, hipStream_t stream);
 // mfma.hip
-hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, const float *qz, int precision, void *a_frag,
-                            hipStream_t stream);
+hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, int max_kpad, const float *qz, int precision,
+                            void *a_frag, hipStream_t stream);
 hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, const int32_t *group_begin, int groups,
                             int precision, const void *a_frag, const float2 *rb, const float4 *binfo, int rb_len,
-                            int mode, int block_stride, const CandOut &co, hipStream_t stream);
+                            int mode, int block_stride, const SurvOut &so, const CandOut &co, hipStream_t stream);
+hipError_t launch_mfma_consts(TmplDev *tmpl, int count, int precision, hipStream_t stream);
+hipError_t launch_mfma_level2(const Waveform &w, const TmplDev *tmpl, int precision, const float4 *binfo,
+                              const SurvOut &so, const CandOut &co, int cus, hipStream_t stream);
 int mfma_group_size();
 uint64_t mfma_block_windows();
 int64_t mfma_frag_count(int kpad);
@@ -200,6 +203,7 @@ double fft_seconds_per_window(int L)
 
 constexpr int kFusedChunk = 16;    // templates per fused FFT launch
 constexpr int kFftPieceBatch = 64;  // pieces / templates per forward transform launch
+constexpr uint64_t kSurvQueueMin = 1u << 22;   // level-1 survivors a lane's matrix-core pass starts with (16 B each)
 constexpr uint64_t kChunkQueueMin = 1u << 20;  // chunk records a lane's fused pass starts with (280 B each)
 constexpr int kFftMinLength = 160;  // shortest template the cost model sends to the FFT form
 
@@ -377,6 +381,8 @@ int build_block_prefix(tmx_ctx *ctx, Lane &L, int need)
     LANE_HIP(launch_block_prefix(ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n, nw, need, ctx->binfo.as<float4>(), ctx->rb.as<float2>(),
                                  L.stream),
              "block prefix");
+    LANE_HIP(launch_block_absmax(ctx->x.as<float>(), ctx->n, nw, need, ctx->binfo.as<float4>(), L.stream),
+             "block maxima");
     LANE_HIP(hipStreamSynchronize(L.stream), "block prefix");
     ctx->rb_len = need;
     ctx->rb_n = ctx->n;
@@ -635,7 +641,11 @@ int upload_and_prepare(Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const 
              "prepare kernel");
     if (cfg.precision != TMX_PREC_FP32 && plan.Tg > 0) {
         LANE_HIP(L.afrag.reserve(16 * static_cast<size_t>(plan.afrag_len + 64)), "hipMalloc MFMA operands");
-        LANE_HIP(launch_mfma_pack(L.tmpl.as<TmplDev>(), plan.Tg, L.qz.as<float>(), cfg.precision, L.afrag.p, L.stream),
+        int max_kpad = 0;
+        for (int d = 0; d < plan.Tg; d++)
+            max_kpad = std::max(max_kpad, plan.tmpl[d].kpad);
+        LANE_HIP(launch_mfma_pack(L.tmpl.as<TmplDev>(), plan.Tg, max_kpad, L.qz.as<float>(), cfg.precision, L.afrag.p,
+                                  L.stream),
                  "MFMA operand packing");
     }
     return fft_prepare(L, plan, fcls);
@@ -699,6 +709,27 @@ CandOut cand_out(Lane &L)
 {
     return CandOut{L.cand.as<Cand>(), L.cand_val.as<float>(), L.cand_lo.as<float>(), L.cand_hi.as<float>(),
                    L.cand_count.as<unsigned long long>(), L.cand_cap};
+}
+
+SurvOut surv_out(Lane &L)
+{
+    return SurvOut{L.surv.as<SurvRec>(), L.surv_n.as<unsigned long long>(), L.surv_cap};
+}
+
+// The matrix-core pass's queue of level-1 survivors, its counter cleared.
+int reserve_surv_queue(tmx_ctx *ctx, Lane &L)
+{
+    const uint64_t want = std::max<uint64_t>({L.surv_need, ctx->surv_hint.load(), kSurvQueueMin});
+    if (want > L.surv_cap) {
+        L.surv_cap = 0;
+        LANE_HIP(L.surv.reserve(sizeof(SurvRec) * want), "hipMalloc survivor queue");
+        L.surv_cap = want;
+        if (L.surv_need > 0)
+            raise_hint(ctx->surv_hint, want);
+    }
+    LANE_HIP(L.surv_n.reserve(sizeof(unsigned long long)), "hipMalloc survivor queue");
+    LANE_HIP(launch_zero_u64(L.surv_n.as<unsigned long long>(), L.stream), "reset survivor counter");
+    return 0;
 }
 
 // The chunk queue of the fused passes of one candidate pass: room for every
@@ -877,10 +908,13 @@ int autotune_groups(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, t
                 return rc;
             const int groups = static_cast<int>(plan.group_begin.size()) - 1;
             LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
+            if (int rc = reserve_surv_queue(ctx, L))
+                return rc;
+            LANE_HIP(launch_mfma_consts(L.tmpl.as<TmplDev>(), plan.Tg, cfg.precision, L.stream), "level-1 constants");
             LANE_HIP(hipEventRecord(L.ev0, L.stream), "event");
             LANE_HIP(launch_scan_mfma(w, L.tmpl.as<TmplDev>(), plan.T, L.groups.as<int32_t>(), groups, cfg.precision,
                                       L.afrag.p, ctx->rb.as<float2>(), ctx->binfo.as<float4>(), ctx->rb_len,
-                                      kModeMain, S, cand_out(L), L.stream),
+                                      kModeMain, S, surv_out(L), cand_out(L), L.stream),
                      "MFMA scan (tuning)");
             LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
             LANE_HIP(hipEventSynchronize(L.ev1), "tuning");
@@ -928,7 +962,7 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
     LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
     const int groups = static_cast<int>(plan.group_begin.size()) - 1;
-    bool inner = false, fft_any = false;
+    bool inner = false, fft_any = false, mfma = false;
     for (int d = 0; d < plan.T; d++)
         (d < plan.Tg ? inner : fft_any) |= !plan.tmpl[d].ea;
     LANE_HIP(hipEventRecord(L.ev0, L.stream), "event");
@@ -938,10 +972,18 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
                                       groups, mode, block_stride, cand_out(L), L.stream),
                      "FP32 scan");
         } else {
+            if (int rc = reserve_surv_queue(ctx, L))
+                return rc;
+            LANE_HIP(launch_mfma_consts(L.tmpl.as<TmplDev>(), plan.Tg, cfg.precision, L.stream), "level-1 constants");
             LANE_HIP(launch_scan_mfma(w, L.tmpl.as<TmplDev>(), plan.T, L.groups.as<int32_t>(), groups, cfg.precision,
                                       L.afrag.p, ctx->rb.as<float2>(), ctx->binfo.as<float4>(), ctx->rb_len, mode,
-                                      block_stride, cand_out(L), L.stream),
+                                      block_stride, surv_out(L), cand_out(L), L.stream),
                      "MFMA scan");
+            if (mode == kModeMain)
+                LANE_HIP(launch_mfma_level2(w, L.tmpl.as<TmplDev>(), cfg.precision, ctx->binfo.as<float4>(),
+                                            surv_out(L), cand_out(L), L.cus, L.stream),
+                         "MFMA level 2");
+            mfma = true;
         }
     }
     bool fused = false;
@@ -967,8 +1009,11 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
                                 0, 0, cand_out(L), nullptr, L.stream),
                  "early-abandoning scan");
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
-    unsigned long long c = 0, nchunk = 0;
+    unsigned long long c = 0, nchunk = 0, nsurv = 0;
     LANE_HIP(hipMemcpyAsync(&c, L.cand_count.p, sizeof(c), hipMemcpyDeviceToHost, L.stream), "read counter");
+    if (mfma && mode == kModeMain)
+        LANE_HIP(hipMemcpyAsync(&nsurv, L.surv_n.p, sizeof(nsurv), hipMemcpyDeviceToHost, L.stream),
+                 "read survivor counter");
     if (fused)
         LANE_HIP(hipMemcpyAsync(&nchunk, L.chunk_n.as<unsigned long long>() + 1, sizeof(nchunk),
                                 hipMemcpyDeviceToHost, L.stream),
@@ -976,6 +1021,9 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     LANE_HIP(hipStreamSynchronize(L.stream), "candidate pass");
     // A main pass whose chunk queue overflowed lost candidates: rerun.
     L.chunk_need = (fused && mode == kModeMain && nchunk > L.chunk_cap) ? nchunk + (nchunk >> 3) + 1024 : 0;
+    // Likewise a matrix-core pass whose survivor queue overflowed.
+    L.surv_need = nsurv > L.surv_cap ? nsurv + (nsurv >> 3) + 1024 : 0;
+    L.last_survivors = nsurv;
     LANE_HIP(hipEventElapsedTime(&L.last_pass_ms, L.ev0, L.ev1), "event time");
     *count = c;
     return 0;
@@ -1005,9 +1053,9 @@ int run_main(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan,
         int rc = run_pass(ctx, L, cfg, plan, fcls, kModeMain, 1, count);
         if (rc)
             return rc;
-        if (L.chunk_need > 0) {
+        if (L.chunk_need > 0 || L.surv_need > 0) {
             (*retries)++;
-            continue;  // the next pass reserves the larger chunk queue
+            continue;  // the next pass reserves the larger queues
         }
         if (*count > limit)
             return L.fail("more than " + std::to_string(limit) + " candidate windows in one pass", kRcSplit);

@@ -16,6 +16,19 @@ constexpr float kMissing = -100000.0f;
 // Early-abandon block of the reference kernel (ZnormSearch.hpp:65).
 constexpr int kBlock = 32;
 
+// FP32 constants of the matrix-core form's level-1 test for one template,
+// computed on the device once its bound is final (k_mfma_consts).
+struct L1Consts {
+    float gmin;    // (Q2 + m - bound - E_ref cap) / 2, shrunk: the dt / sigma-scaled margin
+    float Kc;      // gmin / sqrt(m) less the relative dot-product error terms
+    float eta_sq;  // eta * sqrt(Q2): relative error of the dot product
+    float eabs_q;  // FP16: template values that may flush, per sqrt(sum x~^2)
+    float sqrt_m, rsqrt_m, inv_mf;
+    float absS;    // |sum q|, grown
+    float exq;     // FP16: staged samples that may flush, per unit of the block's scale
+    float pad;
+};
+
 // Per-template record on the device.  The z-normalised samples live in one
 // packed float array (q_off), the block order in one packed int array
 // (order_off).  Q2 = sum q^2 and S = sum q in FP64 feed the candidate filter.
@@ -33,6 +46,7 @@ struct TmplDev {
     int32_t merge;      // exclusion zone of the answers (merge offset)
     int32_t ea;         // 1: the early-abandoning scan takes this template (the other forms skip it)
     int64_t out_off;    // first kNN output slot of this template
+    L1Consts l1;        // matrix-core level-1 constants (device-computed per pass)
 };
 
 // One (template, FFT piece) pair of the FFT form.
@@ -56,6 +70,22 @@ struct CandOut {
     float *lo;
     float *hi;
     unsigned long long *count;
+    uint64_t cap;
+};
+
+// Level-1 survivors of the matrix-core pass, for its rigorous second level:
+// the window, its dot product (real units) and the block's FP16 scale (1 for
+// split BF16).
+struct SurvRec {
+    uint32_t t;  // template index within the call
+    uint32_t j;  // window start
+    float dt;    // dot product of the centred window with the template
+    float scale; // FP16 scale of the block (bounds the flushed-operand error)
+};
+
+struct SurvOut {
+    SurvRec *rec;
+    unsigned long long *count;  // may exceed cap: the pass must be rerun with more room
     uint64_t cap;
 };
 
@@ -252,6 +282,10 @@ size_t prefix_work_len(uint64_t n);
 // `len` entries per block.
 hipError_t launch_block_prefix(const double2 *p, const double2 *pl, uint64_t n, int block, int len, float4 *binfo,
                                float2 *rb, hipStream_t stream);
+
+// Largest |x0 - c_b| over each block's len samples into binfo[b].w (after
+// launch_block_prefix on the same stream).
+hipError_t launch_block_absmax(const float *x, uint64_t n, int block, int len, float4 *binfo, hipStream_t stream);
 
 // Reference template preparation: z-norm (ZnormSearch.hpp:33-62), block
 // order (:72-90), plus Q2 and S.  One workgroup per template.
