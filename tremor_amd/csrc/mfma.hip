@@ -69,13 +69,14 @@ constexpr int kGroup = 8;                     // templates per workgroup
 constexpr int kStage = kNW + kKC;             // staged samples per chunk
 
 // LDS layout of the staged tile (16-bit elements): rows of 32 elements (one
-// column of the Hankel view), 8-element chunks permuted within a row by
-// (row / 2) mod 4.  The 16-byte reads of 8 consecutive lanes (consecutive
-// rows) then land in 8 distinct 16-byte bank groups, without padding.  The
-// permutation touches bits 3-4 only and reads bits 6-7, so it commutes with
-// adding multiples of 1024: the column tiles of a wavefront share one
+// column of the Hankel view), the four 16-byte chunks of a row permuted by
+// (row / 4) mod 4.  A 16-byte read by 16 consecutive lanes (16 consecutive
+// rows, any chunk) then covers all 64 banks once: each row residue mod 16
+// takes its own 16-byte slot of the 256 bytes the banks serve per cycle.
+// The permutation touches bits 3-4 only and reads bits 7-8, so it commutes
+// with adding multiples of 1024: the column tiles of a wavefront share one
 // address plus immediate offsets.
-__device__ inline int swz(int e) { return e ^ (((e >> 6) & 3) << 3); }
+__device__ inline int swz(int e) { return e ^ (((e >> 7) & 3) << 3); }
 
 constexpr int kTileBytesBf16 = 2 * kStage * 2;
 constexpr int kTileBytesF16 = kStage * 2;
@@ -747,6 +748,7 @@ __global__ __launch_bounds__(256) void k_level2(Waveform w, const TmplDev *__res
 
 int mfma_group_size() { return kGroup; }
 uint64_t mfma_block_windows() { return kNW; }
+int mfma_block_waves() { return kWaves; }
 // K of a template: its m + 31 Toeplitz columns, padded to whole pairs of
 // k-steps (the K loop runs k-steps in pairs).
 int mfma_kpad(int m) { return ((m + 31 + 31) / 32) * 32; }

@@ -53,6 +53,7 @@ hipError_t launch_mfma_level2(const Waveform &w, const TmplDev *tmpl, int precis
                               const SurvOut &so, const CandOut &co, int cus, hipStream_t stream);
 int mfma_group_size();
 uint64_t mfma_block_windows();
+int mfma_block_waves();
 int64_t mfma_frag_count(int kpad);
 int mfma_kpad(int m);
 // fft.hip
@@ -1212,7 +1213,9 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
         const uint64_t blocks = waveform_blocks(ctx, cfg);
         const uint64_t target = cfg.seed_target > 0 ? static_cast<uint64_t>(cfg.seed_target) : 1024;
         const uint64_t block_target = cfg.seed_target > 0 ? target : 512;
-        uint64_t stride = (blocks * 4) / block_target;
+        // One entry per wavefront of a sampled block.
+        const uint64_t per_block = cfg.precision == TMX_PREC_FP32 ? 4 : static_cast<uint64_t>(mfma_block_waves());
+        uint64_t stride = (blocks * per_block) / block_target;
         if (stride < 1)
             stride = 1;
         std::vector<int> fft_strides;
