@@ -67,6 +67,7 @@ constexpr int kNW = kWaves * kNN * kTileWin;  // windows per workgroup
 constexpr int kKC = 2048;                     // K samples staged per chunk
 constexpr int kGroup = 8;                     // templates per workgroup
 constexpr int kStage = kNW + kKC;             // staged samples per chunk
+constexpr uint32_t kTileB = 4;                // blocks per template group in the dispatch order (per XCD)
 
 // LDS layout of the staged tile (16-bit elements): rows of 32 elements (one
 // column of the Hankel view), the four 16-byte chunks of a row permuted by
@@ -242,12 +243,15 @@ __global__ __launch_bounds__(kThreads, 1) void k_scan_mfma(Waveform w, const Tmp
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int col = lane & 31, h = lane >> 5;
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
-    // on each XCD, consecutive workgroups take every template group of one
-    // waveform block, so the block's samples and prefix sums are shared in
-    // that XCD's L2 instead of being fetched once per group.
+    // on each XCD, consecutive workgroups take kTileB consecutive blocks (of
+    // that XCD's share) for one template group, then the next group, so the
+    // workgroups in flight on an XCD (one per CU) hold a few blocks' samples
+    // and prefix sums and a few groups' operands, both L2-resident.
     const uint32_t xcd = blockIdx.x & 7u, k8 = blockIdx.x >> 3;
-    const int g = static_cast<int>(k8 % static_cast<uint32_t>(groups));
-    const uint32_t bidx = (k8 / static_cast<uint32_t>(groups)) * 8u + xcd;
+    const uint32_t per_tile = kTileB * static_cast<uint32_t>(groups);
+    const uint32_t rem = k8 % per_tile;
+    const int g = static_cast<int>(rem / kTileB);
+    const uint32_t bidx = ((k8 / per_tile) * kTileB + rem % kTileB) * 8u + xcd;
     if (bidx >= nblocks)
         return;
     const uint64_t blk = static_cast<uint64_t>(bidx) * static_cast<uint64_t>(block_stride);
@@ -778,7 +782,8 @@ hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, c
         return hipSuccess;
     const uint64_t blocks = (w.n + kNW - 1) / kNW;
     const uint64_t launched = (blocks + block_stride - 1) / block_stride;
-    const uint64_t total = ((launched + 7) / 8) * 8 * static_cast<uint64_t>(groups);
+    const uint64_t per_xcd = ((launched + 7) / 8 + kTileB - 1) / kTileB * kTileB;
+    const uint64_t total = per_xcd * 8 * static_cast<uint64_t>(groups);
     if (total >= (1ull << 32))
         return hipErrorInvalidConfiguration;
     const dim3 grid(static_cast<unsigned>(total));
