@@ -76,6 +76,18 @@ def lib():
         L.orc_scan_threshold.restype = ctypes.c_int64
         L.orc_scan_threshold.argtypes = [_f32p, ctypes.c_int64, _f32p, _i32p, _i64p, ctypes.c_int, ctypes.c_float,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int64, _i32p, _u64p, _f32p]
+        _u8p = ctypes.POINTER(ctypes.c_uint8)
+        L.orc_sax_breakpoints.argtypes = [_f32p]
+        L.orc_paa.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _f32p]
+        L.orc_sax_words.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u8p]
+        L.orc_lower_bounds.argtypes = [_f32p, _f32p, _u8p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _f32p]
+        L.orc_scan_knn_lb.restype = ctypes.c_int
+        L.orc_scan_knn_lb.argtypes = [_f32p, ctypes.c_int64, _f32p, _i32p, _i64p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64p, _f32p, _f64p]
+        L.orc_scan_threshold_lb.restype = ctypes.c_int64
+        L.orc_scan_threshold_lb.argtypes = [_f32p, ctypes.c_int64, _f32p, _i32p, _i64p, ctypes.c_int,
+                                            ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int64, _i32p, _u64p, _f32p]
         _lib = L
     return _lib
 
@@ -205,23 +217,25 @@ def pack_templates(templates):
     return np.ascontiguousarray(flat), lengths, offsets
 
 
-def scan_knn(raw, templates, k: int, merge: int = 0, threads: int = 0):
-    """SkipSequentialScan::search_knn (one rank, sequential method).  Returns
-    (positions [T,k] uint64, distances [T,k] float32, seconds [T])."""
+def scan_knn(raw, templates, k: int, merge: int = 0, threads: int = 0, segments: int = 0):
+    """SkipSequentialScan::search_knn (one rank; segments > 0: the
+    skip-sequential method with that many PAA segments).  Returns (positions
+    [T,k] uint64, distances [T,k] float32, seconds [T])."""
     raw = np.ascontiguousarray(raw, dtype=np.float32)
     flat, lengths, offsets = pack_templates(templates)
     T = len(templates)
     op = np.zeros((T, k), dtype=np.uint64)
     od = np.zeros((T, k), dtype=np.float32)
     sec = np.zeros(T, dtype=np.float64)
-    lib().orc_scan_knn(_p(raw, _f32p), raw.size, _p(flat, _f32p), _p(lengths, _i32p), _p(offsets, _i64p), T, k,
-                       merge, threads or os.cpu_count(), _p(op, _u64p), _p(od, _f32p), _p(sec, _f64p))
+    lib().orc_scan_knn_lb(_p(raw, _f32p), raw.size, _p(flat, _f32p), _p(lengths, _i32p), _p(offsets, _i64p), T, k,
+                          merge, threads or os.cpu_count(), segments, _p(op, _u64p), _p(od, _f32p), _p(sec, _f64p))
     return op, od, sec
 
 
-def scan_threshold(raw, templates, rho: float, merge: int = 0, threads: int = 0):
-    """SkipSequentialScan::search_threshold (one rank, sequential method).
-    Returns (template ids int32, positions uint64, distances float32)."""
+def scan_threshold(raw, templates, rho: float, merge: int = 0, threads: int = 0, segments: int = 0):
+    """SkipSequentialScan::search_threshold (one rank; segments > 0: the
+    skip-sequential method).  Returns (template ids int32, positions uint64,
+    distances float32)."""
     raw = np.ascontiguousarray(raw, dtype=np.float32)
     flat, lengths, offsets = pack_templates(templates)
     T = len(templates)
@@ -230,12 +244,54 @@ def scan_threshold(raw, templates, rho: float, merge: int = 0, threads: int = 0)
         ot = np.zeros(cap, dtype=np.int32)
         op = np.zeros(cap, dtype=np.uint64)
         od = np.zeros(cap, dtype=np.float32)
-        rows = lib().orc_scan_threshold(_p(raw, _f32p), raw.size, _p(flat, _f32p), _p(lengths, _i32p),
-                                        _p(offsets, _i64p), T, float(rho), merge, threads or os.cpu_count(), cap,
-                                        _p(ot, _i32p), _p(op, _u64p), _p(od, _f32p))
+        rows = lib().orc_scan_threshold_lb(_p(raw, _f32p), raw.size, _p(flat, _f32p), _p(lengths, _i32p),
+                                           _p(offsets, _i64p), T, float(rho), merge, threads or os.cpu_count(),
+                                           segments, cap, _p(ot, _i32p), _p(op, _u64p), _p(od, _f32p))
         if rows <= cap:
             return ot[:rows], op[:rows], od[:rows]
         cap = int(rows)
+
+
+# ----------------------------------------------------------- iSAX bounds
+
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+def sax_breakpoints() -> np.ndarray:
+    """The 255 breakpoints of cardinality 256 (float32)."""
+    bp = np.zeros(255, dtype=np.float32)
+    lib().orc_sax_breakpoints(_p(bp, _f32p))
+    return bp
+
+
+def paa(z, segments: int) -> np.ndarray:
+    """paa_from_ts (SAX.cpp:73-85) of a z-normalised template."""
+    z = np.ascontiguousarray(z, dtype=np.float32)
+    out = np.zeros(segments, dtype=np.float32)
+    lib().orc_paa(_p(z, _f32p), z.size, segments, _p(out, _f32p))
+    return out
+
+
+def sax_words(raw, m: int, segments: int, threads: int = 0) -> np.ndarray:
+    """The iSAX word of every window (SkipSequential.cpp:155-190 with lower
+    bounds): uint8 [W, segments]."""
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    W = raw.size - m + 1
+    out = np.zeros((max(W, 0), segments), dtype=np.uint8)
+    lib().orc_sax_words(_p(raw, _f32p), raw.size, m, segments, threads or os.cpu_count(), _p(out, _u8))
+    return out
+
+
+def lower_bounds(paa_q, words, m: int) -> np.ndarray:
+    """SkipSequentialScan::lower_bound (SkipSequential.cpp:200-211) of every
+    word against the template's PAA."""
+    paa_q = np.ascontiguousarray(paa_q, dtype=np.float32)
+    words = np.ascontiguousarray(words, dtype=np.uint8)
+    bp = sax_breakpoints()
+    out = np.zeros(words.shape[0], dtype=np.float32)
+    lib().orc_lower_bounds(_p(bp, _f32p), _p(paa_q, _f32p), _p(words, _u8), words.shape[0], m, words.shape[1],
+                           _p(out, _f32p))
+    return out
 
 
 # ------------------------------------------------------------- FP64 oracle

@@ -16,6 +16,7 @@
  *   - two-pass FP64 window statistics     SkipSequential.cpp:154-190
  *   - threshold scan + merge              SkipSequential.cpp:223-294
  *   - kNN scan (bound set + candidates)   SkipSequential.cpp:296-393
+ *   - iSAX words and lower bound          SkipSequential.cpp:155-211, SAX.cpp:24-85,112-180,362-490
  *
  * Floating-point contraction.  The reference is built with GCC at -O2 with FMA
  * available (Makefile:5-6); GCC contracts every a*b+c on this path into one
@@ -25,6 +26,7 @@
  * it.  The fusions are pinned against the reference header compiled with the
  * reference's flags (oracle/_ref, tests/test_oracle_ref.py).
  */
+#define _GNU_SOURCE /* M_SQRT1_2 */
 #include <float.h>
 #include <math.h>
 #include <stdint.h>
@@ -352,6 +354,175 @@ void orc_bsf_insert_offset(orc_bsf *q, float data, int64_t position, int merge_o
 
 /* ---------- scans -------------------------------------------------------- */
 
+/* iSAX breakpoints of cardinality 256, the sub-table of SAXBreakpoints.hpp at
+ * offset (c - 1)(c - 2) / 2: the standard normal quantiles Phi^-1(i / 256),
+ * i = 1..255, rounded to float.  The lower half is found by bisection on
+ * Phi(x) = erfc(-x / sqrt 2) / 2 in FP64 down to adjacent doubles; the median
+ * is 0 and the upper half mirrors the lower (Phi^-1(1 - p) = -Phi^-1(p)). */
+void orc_sax_breakpoints(float *bp)
+{
+    bp[127] = 0.0f;
+    for (int i = 1; i < 128; i++) {
+        const double p = (double)i / 256.0;
+        double lo = -10.0, hi = 10.0;
+        for (;;) {
+            const double mid = 0.5 * (lo + hi);
+            if (!(mid > lo && mid < hi))
+                break;
+            if (0.5 * erfc(-mid * M_SQRT1_2) < p)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        bp[i - 1] = (float)(0.5 * (lo + hi));
+        bp[255 - i] = -bp[i - 1];
+    }
+}
+
+/* SAX.cpp:73-85 (paa_from_ts): FP32 sums in index order, then one division. */
+void orc_paa(const float *z, int m, int segments, float *paa)
+{
+    const int len = m / segments;
+    for (int s = 0; s < segments; s++) {
+        float acc = 0.0f;
+        for (int i = 0; i < len; i++)
+            acc += z[s * len + i];
+        paa[s] = acc / (float)len;
+    }
+}
+
+/* SAX.cpp:46-66 (the bsearch with its compare of SAX.cpp:7-22): the symbol of
+ * a PAA value is the number of breakpoints strictly below it (255 above the
+ * last; 0 for NaN). */
+static uint8_t sax_symbol(const float *bp, float v)
+{
+    int c = 0;
+    for (int i = 0; i < 255; i++)
+        c += bp[i] < v;
+    return (uint8_t)c;
+}
+
+/* SkipSequential.cpp:155-190 with lower_bounds: for window id (first sample
+ * not MISSING), the two-pass FP64 mean and deviation (sigma <= 1e-12 -> 1),
+ * the samples (v - mean) / sigma rounded to float, their PAA and its symbols
+ * (sax_from_ts, SAX.cpp:24-70).  Windows starting with MISSING keep zeros. */
+void orc_sax_words(const float *raw, int64_t n, int m, int segments, int threads, uint8_t *out)
+{
+    const int64_t windows = n - m + 1;
+    if (windows <= 0 || segments <= 0 || m % segments != 0)
+        return;
+    float bp[255];
+    orc_sax_breakpoints(bp);
+#pragma omp parallel num_threads(threads)
+    {
+        float *z = (float *)malloc(sizeof(float) * (size_t)m);
+        float *paa = (float *)malloc(sizeof(float) * (size_t)segments);
+#pragma omp for schedule(static)
+        for (int64_t id = 0; id < windows; id++) {
+            const float *w = raw + id;
+            uint8_t *o = out + (size_t)id * segments;
+            memset(o, 0, (size_t)segments);
+            if (w[0] == ORC_MISSING)
+                continue;
+            double sum = 0.0;
+            for (int d = 0; d < m; d++)
+                sum += w[d] == ORC_MISSING ? 0.0 : (double)w[d];
+            const double mean = sum / m;
+            double squares = 0.0;
+            for (int d = 0; d < m; d++) {
+                const double v = (w[d] == ORC_MISSING ? 0.0 : (double)w[d]) - mean;
+                squares = fma(v, v, squares);
+            }
+            const double sd = sqrt(squares / m);
+            const double div = sd <= 1e-12 ? 1.0 : sd;
+            for (int d = 0; d < m; d++) {
+                const double v = w[d] == ORC_MISSING ? 0.0 : (double)w[d];
+                z[d] = (float)((v - mean) / div);
+            }
+            orc_paa(z, m, segments, paa);
+            for (int s = 0; s < segments; s++)
+                o[s] = sax_symbol(bp, paa[s]);
+        }
+        free(z);
+        free(paa);
+    }
+}
+
+/* The lower bound of SkipSequential.cpp:200-211: per segment the distance of
+ * the template's PAA value from the word's symbol region [bp[v - 1], bp[v]]
+ * (MINVAL / MAXVAL = -/+2e6 at the ends, iSAXTypes.hpp:17-18), all at full
+ * cardinality.  16 segments (minidist_paa_to_isax_rawa_SIMD, SAX.cpp:362-490):
+ * FP32 squares summed as two DPPS of each 8-lane half, ((s0 + s1) + (s2 +
+ * s3)), the halves' sums added lane-wise, then lanes 0 and 4; otherwise
+ * (minidist_paa_to_isax, SAX.cpp:112-180) the FP32 difference widened to FP64
+ * and accumulated with one fused multiply-add per segment into the FP32 sum.
+ * Then times m / segments in FP32. */
+float orc_lower_bound(const float *bp, const float *paa, const uint8_t *word, int m, int segments)
+{
+    float d[64];
+    for (int s = 0; s < segments && s < 64; s++) {
+        const int v = word[s];
+        const float lo = v == 0 ? -2000000.0f : bp[v - 1];
+        const float hi = v == 255 ? 2000000.0f : bp[v];
+        d[s] = lo > paa[s] ? lo - paa[s] : (hi < paa[s] ? hi - paa[s] : 0.0f);
+    }
+    const float ratio = (float)m / (float)segments;
+    if (segments == 16) {
+        float q[16];
+        for (int s = 0; s < 16; s++)
+            q[s] = d[s] * d[s];
+        float h[4];
+        for (int g = 0; g < 4; g++)
+            h[g] = (q[4 * g] + q[4 * g + 1]) + (q[4 * g + 2] + q[4 * g + 3]);
+        const float lane0 = h[0] + h[2], lane4 = h[1] + h[3];
+        return (lane0 + lane4) * ratio;
+    }
+    float dist = 0.0f;
+    for (int s = 0; s < segments; s++) {
+        const double e = (double)d[s];
+        dist = (float)fma(e, e, (double)dist);
+    }
+    return ratio * dist;
+}
+
+/* orc_lower_bound for `count` consecutive words. */
+void orc_lower_bounds(const float *bp, const float *paa, const uint8_t *words, int64_t count, int m, int segments,
+                      float *out)
+{
+    for (int64_t i = 0; i < count; i++)
+        out[i] = orc_lower_bound(bp, paa, words + (size_t)i * segments, m, segments);
+}
+
+/* Per-template lower-bound state of the skip-sequential scan (NULL words: no
+ * bound, the sequential method). */
+typedef struct {
+    const uint8_t *words;
+    float paa[64];
+    float bp[255];
+    int m, segments;
+} orc_lb;
+
+static int lb_setup(orc_lb *lb, const float *raw, int64_t n, int m, int segments, int threads, const float *qz)
+{
+    lb->words = NULL;
+    if (segments <= 0 || segments > 64 || m % segments != 0 || n - m + 1 <= 0)
+        return 0;
+    uint8_t *w = (uint8_t *)malloc((size_t)(n - m + 1) * (size_t)segments);
+    orc_sax_words(raw, n, m, segments, threads, w);
+    orc_sax_breakpoints(lb->bp);
+    orc_paa(qz, m, segments, lb->paa);
+    lb->words = w;
+    lb->m = m;
+    lb->segments = segments;
+    return 1;
+}
+
+static int lb_skip(const orc_lb *lb, int64_t id, float bound)
+{
+    return lb->words &&
+           orc_lower_bound(lb->bp, lb->paa, lb->words + (size_t)id * lb->segments, lb->m, lb->segments) > bound;
+}
+
 /* Prepared template: SkipSequential.cpp:213-221 (z-norm + block order; the PAA
  * only feeds the lower bounds, which the sequential method does not use). */
 static void prepare(const float *t, int m, float *qz, int *order)
@@ -371,6 +542,18 @@ int orc_scan_knn(const float *raw, int64_t n, const float *templates, const int3
                  const int64_t *offsets, int count, int k, int merge_offset, int threads, uint64_t *out_pos,
                  float *out_d, double *seconds)
 {
+    return orc_scan_knn_lb(raw, n, templates, lengths, offsets, count, k, merge_offset, threads, 0, out_pos, out_d,
+                           seconds);
+}
+
+/* The same with lower_bounds = true and `segments` PAA segments (the
+ * skip-sequential method, SkipSequential.cpp:321): a window is skipped when
+ * its lower bound exceeds the bound of the moment; templates whose length is
+ * not a multiple of `segments` are scanned without bounds. */
+int orc_scan_knn_lb(const float *raw, int64_t n, const float *templates, const int32_t *lengths,
+                    const int64_t *offsets, int count, int k, int merge_offset, int threads, int segments,
+                    uint64_t *out_pos, float *out_d, double *seconds)
+{
     const int bound_size = merge_offset > 0 ? 2 * k - 1 : k;
     for (int t = 0; t < count; t++) {
         const int m = lengths[t];
@@ -389,6 +572,8 @@ int orc_scan_knn(const float *raw, int64_t n, const float *templates, const int3
         float *qz = (float *)malloc(sizeof(float) * (size_t)m);
         int *order = (int *)malloc(sizeof(int) * (size_t)(m / ORC_BLOCK + 1));
         prepare(templates + offsets[t], m, qz, order);
+        orc_lb lb;
+        lb_setup(&lb, raw, n, m, segments, threads, qz);
 
         double t0 = 0.0;
 #ifdef _OPENMP
@@ -406,7 +591,7 @@ int orc_scan_knn(const float *raw, int64_t n, const float *templates, const int3
                 float bound;
 #pragma omp atomic read
                 bound = best.knn[bound_size - 1];
-                if (raw[id] == ORC_MISSING)
+                if (raw[id] == ORC_MISSING || lb_skip(&lb, id, bound))
                     continue;
                 const float dist = orc_dist(qz, raw, id, m, mean[id], sd[id], bound, order, &blocks);
                 if (dist > bound)
@@ -449,6 +634,7 @@ int orc_scan_knn(const float *raw, int64_t n, const float *templates, const int3
         free(pp);
         free(dd);
         free(cand);
+        free((void *)lb.words);
         orc_bsf_free(&best);
         free(qz);
         free(order);
@@ -467,6 +653,16 @@ int64_t orc_scan_threshold(const float *raw, int64_t n, const float *templates, 
                            const int64_t *offsets, int count, float rho, int merge_offset, int threads,
                            int64_t cap, int32_t *out_t, uint64_t *out_pos, float *out_d)
 {
+    return orc_scan_threshold_lb(raw, n, templates, lengths, offsets, count, rho, merge_offset, threads, 0, cap,
+                                 out_t, out_pos, out_d);
+}
+
+/* The same with lower_bounds = true (SkipSequential.cpp:256): a window whose
+ * lower bound exceeds cc_to_ed(rho, m) is skipped. */
+int64_t orc_scan_threshold_lb(const float *raw, int64_t n, const float *templates, const int32_t *lengths,
+                              const int64_t *offsets, int count, float rho, int merge_offset, int threads,
+                              int segments, int64_t cap, int32_t *out_t, uint64_t *out_pos, float *out_d)
+{
     int64_t rows = 0;
     for (int t = 0; t < count; t++) {
         const int m = lengths[t];
@@ -480,6 +676,8 @@ int64_t orc_scan_threshold(const float *raw, int64_t n, const float *templates, 
         float *qz = (float *)malloc(sizeof(float) * (size_t)m);
         int *order = (int *)malloc(sizeof(int) * (size_t)(m / ORC_BLOCK + 1));
         prepare(templates + offsets[t], m, qz, order);
+        orc_lb lb;
+        lb_setup(&lb, raw, n, m, segments, threads, qz);
         size_t cap_h = 1024, nh = 0;
         orc_cand *hits = (orc_cand *)malloc(sizeof(orc_cand) * cap_h);
 #pragma omp parallel num_threads(threads)
@@ -489,7 +687,7 @@ int64_t orc_scan_threshold(const float *raw, int64_t n, const float *templates, 
             orc_cand *local = (orc_cand *)malloc(sizeof(orc_cand) * lcap);
 #pragma omp for schedule(dynamic, 65536)
             for (int64_t id = 0; id < windows; id++) {
-                if (raw[id] == ORC_MISSING)
+                if (raw[id] == ORC_MISSING || lb_skip(&lb, id, bound))
                     continue;
                 const float dist = orc_dist(qz, raw, id, m, mean[id], sd[id], bound, order, &blocks);
                 if (dist <= bound) {
@@ -530,6 +728,7 @@ int64_t orc_scan_threshold(const float *raw, int64_t n, const float *templates, 
             has_last = 1;
         }
         free(hits);
+        free((void *)lb.words);
         free(qz);
         free(order);
         free(mean);

@@ -38,6 +38,19 @@ int64_t orc_scan_threshold(const float *raw, int64_t n, const float *templates, 
                            const int64_t *offsets, int count, float rho, int merge_offset, int threads,
                            int64_t cap, int32_t *out_t, uint64_t *out_pos, float *out_d);
 
+void orc_sax_breakpoints(float *bp);
+void orc_paa(const float *z, int m, int segments, float *paa);
+void orc_sax_words(const float *raw, int64_t n, int m, int segments, int threads, uint8_t *out);
+float orc_lower_bound(const float *bp, const float *paa, const uint8_t *word, int m, int segments);
+void orc_lower_bounds(const float *bp, const float *paa, const uint8_t *words, int64_t count, int m, int segments,
+                      float *out);
+int orc_scan_knn_lb(const float *raw, int64_t n, const float *templates, const int32_t *lengths,
+                    const int64_t *offsets, int count, int k, int merge_offset, int threads, int segments,
+                    uint64_t *out_pos, float *out_d, double *seconds);
+int64_t orc_scan_threshold_lb(const float *raw, int64_t n, const float *templates, const int32_t *lengths,
+                              const int64_t *offsets, int count, float rho, int merge_offset, int threads,
+                              int segments, int64_t cap, int32_t *out_t, uint64_t *out_pos, float *out_d);
+
 #ifdef __cplusplus
 }
 #endif
