@@ -7,9 +7,10 @@
 //   search_threshold  writes <output>_<rank>.csv: "Query, Counter, Position, Cross Correlation"
 //                                                                    (SkipSequential.cpp:230-290)
 // One process drives one GPU: the waveform forms a single partition (rank 0).
-// threads, paa_segments and lower_bounds are accepted for call-site
-// compatibility; the device evaluates every window and needs no iSAX
-// summaries, and the answers equal the sequential method's.
+// threads is accepted for call-site compatibility.  lower_bounds (with
+// paa_segments) puts the reference's iSAX lower bound on the
+// early-abandoning scan (SkipSequential.cpp:200-211): templates on that scan
+// skip the windows it rules out; the answers equal the sequential method's.
 #ifndef TREMOR_AMD_SEQUENTIAL_SCAN_HPP
 #define TREMOR_AMD_SEQUENTIAL_SCAN_HPP
 

@@ -87,6 +87,14 @@ typedef struct {
                                 together would exceed it is split into halves of its templates, run
                                 one after the other (same answers); one template alone exceeding it
                                 is an error */
+    int32_t lower_bounds;    /* 1 = the skip-sequential method's iSAX lower bound on the early-abandoning
+                                scan (SkipSequentialScan's lower_bounds, SkipSequential.cpp:200-211,
+                                :256, :321): a window whose bound exceeds the template's bound is
+                                skipped; the iSAX words of each template length are built on first
+                                use and kept; templates whose length is not a multiple of
+                                paa_segments scan without bounds.  0 = default (off) */
+    int32_t paa_segments;    /* PAA segments of those bounds, 1..64 (SkipSequentialScan's
+                                paa_segments); default 16 */
 } tmx_cfg;
 
 typedef struct {
@@ -113,6 +121,7 @@ typedef struct {
                                 samples every fft_block - fft_lmax + 1 windows; 0: none) */
     int32_t splits;          /* times the call's templates were split in two because their
                                 candidates exceeded max_candidates (0: one pass) */
+    int64_t lb_skipped;      /* windows the lower bound skipped in the early-abandoning scan */
 } tmx_stats;
 
 typedef struct {

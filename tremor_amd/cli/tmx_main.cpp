@@ -38,6 +38,7 @@ struct Options {
     int precision = TMX_PREC_BF16X3;
     int fft_mode = TMX_FFT_AUTO;
     int scan_mode = TMX_SCAN_AUTO;
+    bool scan_mode_given = false;
     int device = 0;
 };
 
@@ -54,7 +55,8 @@ void usage(const char *prog)
         "  --output PATH              kNN CSV, or threshold prefix (<PATH>_0.csv)\n"
         "  --benchmark-output PATH    timings CSV (default <output>.timings.csv)\n"
         "  --no-benchmark-report      skip the timings CSV\n"
-        "  --method sequential        (skip-sequential is accepted: same answers)\n"
+        "  --method sequential|skip-sequential  (skip-sequential: every template on the\n"
+        "                             early-abandoning scan with the iSAX lower bound; same answers)\n"
         "  --precision bf16x3|fp16|fp32  candidate pass on the GPU (answers identical)\n"
         "  --fft-mode auto|off|always     FFT form of the candidate pass (default auto: cost model)\n"
         "  --scan-mode auto|off|always    early-abandoning scan form (default auto: cost model)\n"
@@ -157,6 +159,8 @@ Options parse(int argc, char **argv)
             if (mode < 0)
                 throw std::invalid_argument(a + " must be auto, off or always");
             (a == "--fft-mode" ? o.fft_mode : o.scan_mode) = mode;
+            if (a == "--scan-mode")
+                o.scan_mode_given = true;
         } else if (a == "--precision") {
             const std::string p = value(i, a);
             if (p == "bf16x3")
@@ -269,7 +273,10 @@ int run(const Options &o)
 
     tremor_amd::SequentialScan scan(o.workers, o.paa, o.merge, o.replication_groups, o.method != "sequential",
                                     o.precision, o.device);
-    scan.set_forms(o.fft_mode, o.scan_mode);
+    // The skip-sequential method is the bounded early-abandoning scan for
+    // every template (unless --scan-mode says otherwise).
+    const bool sss = o.method == "skip-sequential";
+    scan.set_forms(o.fft_mode, sss && !o.scan_mode_given ? TMX_SCAN_ALWAYS : o.scan_mode);
     const double t_build = now_s();
     scan.build(o.dataset, n, static_cast<int>(o.window));
     const double build_s = now_s() - t_build;

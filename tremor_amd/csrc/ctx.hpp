@@ -88,8 +88,10 @@ struct Lane {
     DevMem Y, pairs, qspec, qmax, qin, ids, slot;
     // fused FFT form: queue of chunks for the per-window pass
     DevMem chunk_rec, chunk_dt, chunk_n;
-    // early-abandoning scan: its templates and the sample counters
-    DevMem scan_ids, scan_visit;
+    // early-abandoning scan: its templates and the sample counters; the
+    // templates' PAA and the count of windows the lower bound skipped
+    DevMem scan_ids, scan_visit, lb_paa, lb_skip;
+    uint64_t last_lb_skipped = 0;
     uint64_t chunk_cap = 0;
     uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
     // matrix-core form: queue of level-1 survivors for the rigorous second level
@@ -184,6 +186,12 @@ struct FftShape {
     int B, Lmax;
 };
 
+// iSAX words of every window of one length (skip-sequential lower bound).
+struct SaxWords {
+    int m = 0, segments = 0;
+    DevMem words;  // uint8 [n - m + 1][segments]
+};
+
 // Piece spectra of the resident waveform for one shape (B, Lmax).
 struct FftClass {
     int B = 0, Lmax = 0, step = 0, bins = 0;
@@ -241,6 +249,10 @@ struct tmx_ctx {
     uint64_t rb_n = 0;
     // FFT piece spectra, one class per block length
     std::vector<std::unique_ptr<tmx::FftClass>> fft;
+    // iSAX words per (template length, segments), oldest first, and the
+    // breakpoints of the lower bounds
+    std::vector<std::unique_ptr<tmx::SaxWords>> sax;
+    tmx::DevMem sax_bp;
     // serving loop (null when not running)
     std::unique_ptr<tmx::Server, tmx::ServerDelete> server;
     // lanes of stopped serving loops, kept with their grown buffers and FFT

@@ -36,7 +36,7 @@
 // The reference's partial sums only grow (fused products of squares onto
 // non-negative accumulators), so its final distance is at least this bound
 // whichever block the test fires at.
-#include "tmx_internal.hpp"
+#include "isax.hpp"
 
 namespace tmx {
 namespace {
@@ -55,11 +55,15 @@ __device__ inline double sq(double v) { return v * v; }
 // ids[0..count): the templates of this pass (blockIdx.y).  Main mode: the
 // windows [256 blockIdx.x, + 256) of each; sample mode: sample_n windows
 // spread evenly over the template's windows, visited[t] += blocks visited,
-// visited[T + t] += windows sampled.
+// visited[T + t] += windows sampled.  A template with a lower bound (lb_seg >
+// 0) skips every window whose bound exceeds its own bound before reading a
+// sample (SkipSequential.cpp:256, :321); main mode counts them in *skipped.
 __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev *__restrict__ tmpl, int T,
                                                       const float *__restrict__ qz, const int32_t *__restrict__ order,
                                                       const int32_t *__restrict__ ids, int mode, uint64_t sample_n,
-                                                      CandOut co, unsigned long long *__restrict__ visited)
+                                                      CandOut co, unsigned long long *__restrict__ visited,
+                                                      const float *__restrict__ bp_g, const float *__restrict__ paa_g,
+                                                      unsigned long long *__restrict__ skipped)
 {
     constexpr double u = 5.9604644775390625e-08;
     constexpr double u64 = 1.1102230246251565e-16;
@@ -67,6 +71,16 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
     const int m = tmpl[t].m;
     if (static_cast<uint64_t>(m) > w.n)
         return;
+    const int lb_seg = tmpl[t].lb_seg;
+    __shared__ float bp[256];
+    __shared__ float paa[kMaxSegments];
+    if (lb_seg > 0) {
+        for (int i = threadIdx.x; i < 255; i += kThreads)
+            bp[i] = bp_g[i];
+        if (threadIdx.x < lb_seg)
+            paa[threadIdx.x] = paa_g[tmpl[t].lb_paa + threadIdx.x];
+        __syncthreads();
+    }
     const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
     uint64_t j;
@@ -81,6 +95,20 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
     }
     if (w.x[j] == kMissing)
         return;  // never searched (SkipSequential.cpp:256)
+    if (lb_seg > 0) {
+        const float lb = sax_lower_bound(bp, paa, tmpl[t].lb_words + j * static_cast<uint64_t>(lb_seg), m, lb_seg);
+        if (lb > tmpl[t].bound) {
+            if (mode == kModeSample)
+                atomicAdd(&visited[T + t], 1ull);  // sampled, no block visited
+            else if (skipped) {
+                // one atomic per wavefront: the lanes skipping here
+                const unsigned long long mask = __ballot(1);
+                if (__lane_id() == __ffsll(static_cast<long long>(mask)) - 1)
+                    atomicAdd(skipped, static_cast<unsigned long long>(__popcll(mask)));
+            }
+            return;
+        }
+    }
     const int nb = m / kBlock;
     const double md = static_cast<double>(m);
     const double invm = 1.0 / md;
@@ -187,7 +215,8 @@ __global__ void k_mark_ea(TmplDev *__restrict__ tmpl, const int32_t *__restrict_
 // or a sample pass (sample_n windows each) accumulating the blocks visited.
 hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const float *qz, const int32_t *order,
                           const int32_t *ids, int count, uint64_t max_windows, int sample, uint64_t sample_n,
-                          const CandOut &co, unsigned long long *visited, hipStream_t stream)
+                          const CandOut &co, unsigned long long *visited, const float *bp, const float *paa,
+                          unsigned long long *skipped, hipStream_t stream)
 {
     if (count <= 0)
         return hipSuccess;
@@ -199,7 +228,7 @@ hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const f
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_scan_ea, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(count)), dim3(kThreads), 0,
                        stream, w, tmpl, T, qz, order, ids, sample ? kModeSample : static_cast<int>(kModeMain),
-                       sample_n, co, visited);
+                       sample_n, co, visited, bp, paa, skipped);
     return hipGetLastError();
 }
 

@@ -28,8 +28,6 @@ SequentialScan::SequentialScan(int threads, int paa_segments, int merge_offset, 
     : merge_offset_(merge_offset)
 {
     (void)threads;
-    (void)paa_segments;
-    (void)lower_bounds;
     if (replication_groups < 0 || replication_groups > 1)
         throw std::invalid_argument("--replication-groups must be 0 or 1 on one GPU");
     if (merge_offset < 0)
@@ -38,6 +36,14 @@ SequentialScan::SequentialScan(int threads, int paa_segments, int merge_offset, 
     tmx_cfg_default(&cfg);
     cfg.device = device;
     cfg.precision = precision;
+    // lower_bounds: the iSAX bound on the early-abandoning scan (answers
+    // unchanged; it only skips windows that cannot qualify).
+    if (lower_bounds) {
+        if (paa_segments < 1 || paa_segments > 64)
+            throw std::invalid_argument("--paa-segments must be in 1..64 with lower bounds");
+        cfg.lower_bounds = 1;
+        cfg.paa_segments = paa_segments;
+    }
     if (tmx_create(&ctx_, &cfg) != 0)
         throw std::runtime_error("no usable HIP device " + std::to_string(device));
 }

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "isax.hpp"
 #include "filter.hpp"
 
 namespace tmx {
@@ -92,7 +93,8 @@ hipError_t noalloc_distance_batch(int device, const float *x, uint64_t n, const 
 hipError_t launch_zero_u64(unsigned long long *p, hipStream_t stream);
 hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const float *qz, const int32_t *order,
                           const int32_t *ids, int count, uint64_t max_windows, int sample, uint64_t sample_n,
-                          const CandOut &co, unsigned long long *visited, hipStream_t stream);
+                          const CandOut &co, unsigned long long *visited, const float *bp, const float *paa,
+                          unsigned long long *skipped, hipStream_t stream);
 hipError_t launch_mark_ea(TmplDev *tmpl, const int32_t *ids, int count, hipStream_t stream);
 hipError_t launch_rho_err(const Cand *cand, const float *est, const float *exact, uint64_t count,
                           const TmplDev *tmpl, unsigned int *out_bits, hipStream_t stream);
@@ -239,6 +241,23 @@ void set_groups(CallPlan &plan, int G)
     plan.group_begin.push_back(plan.Tg);
 }
 
+// A template length the skip-sequential lower bound applies to: a whole
+// number of PAA segments (the reference requires it of every query,
+// tremor_main.cpp:391-402) and at most the waveform.
+bool lb_eligible(const tmx_cfg &cfg, int m, uint64_t n)
+{
+    return cfg.paa_segments >= 1 && cfg.paa_segments <= kMaxSegments && m % cfg.paa_segments == 0 &&
+           static_cast<uint64_t>(m) <= n;
+}
+
+bool plan_has_lb(const CallPlan &plan)
+{
+    for (const auto &t : plan.tmpl)
+        if (t.lb_seg > 0)
+            return true;
+    return false;
+}
+
 int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count, CallPlan &plan,
                std::vector<float> &raw_packed)
 {
@@ -297,6 +316,16 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
     plan.raw_len = static_cast<int64_t>(raw_packed.size());
     plan.order_len = ooff;
     plan.afrag_len = aoff;
+    // Skip-sequential lower bounds: templates whose length is a whole number
+    // of segments; their PAA at d * kMaxSegments in the lane's array, their
+    // words bound once the resident words exist (bind_sax).
+    for (int d = 0; d < count; d++) {
+        TmplDev &t = plan.tmpl[d];
+        if (cfg.lower_bounds && lb_eligible(cfg, t.m, ctx->n)) {
+            t.lb_seg = cfg.paa_segments;
+            t.lb_paa = static_cast<int64_t>(d) * kMaxSegments;
+        }
+    }
     // Path choice: templates are sorted by length and the FFT form wins for
     // long templates, so the FFT set is a suffix of the device order.
     plan.Tg = count;
@@ -358,8 +387,26 @@ FftClass *find_fft_class(tmx_ctx *ctx, FftShape sh)
     return nullptr;
 }
 
+SaxWords *find_sax(tmx_ctx *ctx, int m, int segments)
+{
+    for (auto &w : ctx->sax)
+        if (w->m == m && w->segments == segments)
+            return w.get();
+    return nullptr;
+}
+
+bool sax_ready(tmx_ctx *ctx, const CallPlan &plan)
+{
+    for (const auto &t : plan.tmpl)
+        if (t.lb_seg > 0 && !find_sax(ctx, t.m, t.lb_seg))
+            return false;
+    return !plan_has_lb(plan) || ctx->sax_bp.p;
+}
+
 bool resident_ready(tmx_ctx *ctx, const tmx_cfg &cfg, const CallPlan &plan)
 {
+    if (!sax_ready(ctx, plan))
+        return false;
     if (cfg.precision != TMX_PREC_FP32 && plan.Tg > 0 &&
         !(ctx->rb_n == ctx->n && ctx->rb_len >= block_prefix_len(plan)))
         return false;
@@ -503,8 +550,44 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
     return 0;
 }
 
+// iSAX words for every lower-bound template length of the plan that has
+// none yet (SkipSequential.cpp:155-190, on the device), keeping at most
+// kSaxKept lengths (the oldest are dropped first; no batch runs meanwhile).
+constexpr size_t kSaxKept = 8;
+
+int build_sax(tmx_ctx *ctx, Lane &L, const CallPlan &plan)
+{
+    if (!plan_has_lb(plan))
+        return 0;
+    if (!ctx->sax_bp.p) {
+        float bp[255];
+        sax_breakpoints(bp);
+        LANE_HIP(ctx->sax_bp.reserve(sizeof(bp)), "hipMalloc breakpoints");
+        LANE_HIP(hipMemcpy(ctx->sax_bp.p, bp, sizeof(bp), hipMemcpyHostToDevice), "copy breakpoints");
+    }
+    for (const auto &t : plan.tmpl) {
+        if (t.lb_seg <= 0 || find_sax(ctx, t.m, t.lb_seg))
+            continue;
+        while (ctx->sax.size() >= kSaxKept)
+            ctx->sax.erase(ctx->sax.begin());
+        auto w = std::make_unique<SaxWords>();
+        w->m = t.m;
+        w->segments = t.lb_seg;
+        const uint64_t W = ctx->n - static_cast<uint64_t>(t.m) + 1;
+        LANE_HIP(w->words.reserve(W * static_cast<uint64_t>(t.lb_seg) + 16), "hipMalloc iSAX words");
+        LANE_HIP(launch_sax_words(ctx->x.as<float>(), ctx->n, t.m, t.lb_seg, ctx->sax_bp.as<float>(),
+                                  w->words.as<uint8_t>(), L.stream),
+                 "iSAX words");
+        LANE_HIP(hipStreamSynchronize(L.stream), "iSAX words");
+        ctx->sax.push_back(std::move(w));
+    }
+    return 0;
+}
+
 int build_resident(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan)
 {
+    if (const int rc = build_sax(ctx, L, plan))
+        return rc;
     if (cfg.precision != TMX_PREC_FP32 && plan.Tg > 0) {
         const int rc = build_block_prefix(ctx, L, block_prefix_len(plan));
         if (rc)
@@ -640,6 +723,11 @@ int upload_and_prepare(Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const 
     LANE_HIP(launch_prepare(L.raw.as<float>(), L.tmpl.as<TmplDev>(), T, L.qz.as<float>(), L.order.as<int32_t>(),
                             L.stream),
              "prepare kernel");
+    if (plan_has_lb(plan)) {
+        LANE_HIP(L.lb_paa.reserve(sizeof(float) * kMaxSegments * static_cast<size_t>(T)), "hipMalloc template PAA");
+        LANE_HIP(launch_tmpl_paa(L.tmpl.as<TmplDev>(), T, L.qz.as<float>(), L.lb_paa.as<float>(), L.stream),
+                 "template PAA");
+    }
     if (cfg.precision != TMX_PREC_FP32 && plan.Tg > 0) {
         LANE_HIP(L.afrag.reserve(16 * static_cast<size_t>(plan.afrag_len + 64)), "hipMalloc MFMA operands");
         int max_kpad = 0;
@@ -1004,14 +1092,23 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
                 return rc;
         }
     }
+    const bool lb_count = mode == kModeMain && !plan.scan_ids.empty() && plan_has_lb(plan);
+    if (lb_count) {
+        LANE_HIP(L.lb_skip.reserve(sizeof(unsigned long long)), "hipMalloc skip counter");
+        LANE_HIP(launch_zero_u64(L.lb_skip.as<unsigned long long>(), L.stream), "reset skip counter");
+    }
     if (mode == kModeMain && !plan.scan_ids.empty())
         LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
                                 L.scan_ids.as<int32_t>(), static_cast<int>(plan.scan_ids.size()), plan.scan_windows,
-                                0, 0, cand_out(L), nullptr, L.stream),
+                                0, 0, cand_out(L), nullptr, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
+                                lb_count ? L.lb_skip.as<unsigned long long>() : nullptr, L.stream),
                  "early-abandoning scan");
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
-    unsigned long long c = 0, nchunk = 0, nsurv = 0;
+    unsigned long long c = 0, nchunk = 0, nsurv = 0, nskip = 0;
     LANE_HIP(hipMemcpyAsync(&c, L.cand_count.p, sizeof(c), hipMemcpyDeviceToHost, L.stream), "read counter");
+    if (lb_count)
+        LANE_HIP(hipMemcpyAsync(&nskip, L.lb_skip.p, sizeof(nskip), hipMemcpyDeviceToHost, L.stream),
+                 "read skip counter");
     if (mfma && mode == kModeMain)
         LANE_HIP(hipMemcpyAsync(&nsurv, L.surv_n.p, sizeof(nsurv), hipMemcpyDeviceToHost, L.stream),
                  "read survivor counter");
@@ -1025,6 +1122,8 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     // Likewise a matrix-core pass whose survivor queue overflowed.
     L.surv_need = nsurv > L.surv_cap ? nsurv + (nsurv >> 3) + 1024 : 0;
     L.last_survivors = nsurv;
+    if (mode == kModeMain)
+        L.last_lb_skipped = nskip;
     LANE_HIP(hipEventElapsedTime(&L.last_pass_ms, L.ev0, L.ev1), "event time");
     *count = c;
     return 0;
@@ -1133,8 +1232,9 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
     for (int d = 0; d < plan.T; d++) {
         if (static_cast<uint64_t>(plan.tmpl[d].m) > ctx->n)
             continue;
-        // The scan visits at least one block per window.
-        if (cfg.scan_mode == TMX_SCAN_ALWAYS ||
+        // The scan visits at least one block per window, unless a lower
+        // bound skips windows before any block.
+        if (cfg.scan_mode == TMX_SCAN_ALWAYS || plan.tmpl[d].lb_seg > 0 ||
             planned_seconds_per_window(cfg, plan, d) >= 0.9 * scan_seconds_per_window(1.0))
             ids.push_back(d);
     }
@@ -1153,7 +1253,8 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
         Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
         LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
                                 L.scan_ids.as<int32_t>(), static_cast<int>(ids.size()), 0, 1, kScanSample,
-                                cand_out(L), L.scan_visit.as<unsigned long long>(), L.stream),
+                                cand_out(L), L.scan_visit.as<unsigned long long>(), ctx->sax_bp.as<float>(),
+                                L.lb_paa.as<float>(), nullptr, L.stream),
                  "scan sample");
         std::vector<unsigned long long> visit(2 * static_cast<size_t>(plan.T));
         LANE_HIP(hipMemcpyAsync(visit.data(), L.scan_visit.p, vb, hipMemcpyDeviceToHost, L.stream), "read scan sample");
@@ -1193,6 +1294,9 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     std::vector<const FftClass *> fcls;
     for (const auto &g : plan.fft)
         fcls.push_back(find_fft_class(ctx, g.shape));
+    for (auto &t : plan.tmpl)
+        if (t.lb_seg > 0)
+            t.lb_words = find_sax(ctx, t.m, t.lb_seg)->words.as<uint8_t>();
     int rc = upload_and_prepare(L, cfg, plan, raw_packed, fcls);
     if (rc)
         return rc;
@@ -1246,6 +1350,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     uint64_t ncand = 0;
     if ((rc = run_main(ctx, L, cfg, plan, fcls, &ncand, &st.retries)))
         return rc;
+    st.lb_skipped = static_cast<int64_t>(L.last_lb_skipped);
     st.kernel_ms_main = L.last_pass_ms;
     st.flops_main = plan.flops;
     st.fft_templates = 0;
@@ -1506,6 +1611,8 @@ int tmx_cfg_default(tmx_cfg *cfg)
     cfg->verify_seq = 0;
     cfg->autotune = 1;
     cfg->max_candidates = 0;
+    cfg->lower_bounds = 0;
+    cfg->paa_segments = 16;
     return 0;
 }
 
@@ -1520,7 +1627,8 @@ int tmx_create(tmx_ctx **out, const tmx_cfg *cfg)
     else
         tmx_cfg_default(&ctx->cfg);
     if (ctx->cfg.precision < TMX_PREC_FP32 || ctx->cfg.precision > TMX_PREC_FP16 ||
-        ctx->cfg.fft_mode < TMX_FFT_AUTO || ctx->cfg.fft_mode > TMX_FFT_ALWAYS) {
+        ctx->cfg.fft_mode < TMX_FFT_AUTO || ctx->cfg.fft_mode > TMX_FFT_ALWAYS ||
+        (ctx->cfg.lower_bounds && (ctx->cfg.paa_segments < 1 || ctx->cfg.paa_segments > kMaxSegments))) {
         delete ctx;
         return -1;
     }
@@ -1611,6 +1719,7 @@ int tmx_load_waveform(tmx_ctx *ctx, const float *x, uint64_t n)
     ctx->n = 0;
     ctx->rb_n = 0;
     ctx->fft.clear();
+    ctx->sax.clear();
     const size_t wl_len = prefix_work_len(n);
     if ((e = ctx->x.reserve(sizeof(float) * (n + pad))) != hipSuccess)
         return hip_err(e, "hipMalloc waveform");

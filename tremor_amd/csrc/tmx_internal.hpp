@@ -47,6 +47,10 @@ struct TmplDev {
     int32_t ea;         // 1: the early-abandoning scan takes this template (the other forms skip it)
     int64_t out_off;    // first kNN output slot of this template
     L1Consts l1;        // matrix-core level-1 constants (device-computed per pass)
+    int32_t lb_seg;     // skip-sequential lower bound: PAA segments (0 = none)
+    int32_t lb_pad;
+    int64_t lb_paa;     // offset of the template's PAA in the lane's PAA array
+    const uint8_t *lb_words;  // resident iSAX words of this length (window j at j * lb_seg)
 };
 
 // One (template, FFT piece) pair of the FFT form.
