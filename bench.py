@@ -40,9 +40,11 @@ def _args():
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--scan-steps", type=int, default=1, help="steps of the plain FP32 scan (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
-    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3", "c4"],
-                    help="c1: batched uneven templates (the headline); c2: threshold at scale; "
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "c4"],
+                    help="c0: plain small workload, skip-sequential bound vs sequential scan; "
+                         "c1: batched uneven templates (the headline); c2: threshold at scale; "
                          "c3: serving loop latency; c4: long templates, FFT vs GEMM forms")
+    ap.add_argument("--segments", type=int, default=16, help="c0: PAA segments of the lower bound")
     ap.add_argument("--fft", default="auto", choices=["auto", "off", "always"])
     ap.add_argument("--rate", type=float, default=200.0, help="c3: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="c3: seconds of arrivals (batched loop)")
@@ -250,6 +252,63 @@ def bench_long(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_skip_sequential(a):
+    """Plain small workload (c0 shape): n = 10^6, 16 templates of 256 samples
+    cut from events, kNN k = 10 and threshold rho >= 0.8.  The
+    early-abandoning scan for every template, without and with the iSAX lower
+    bound (the reference's sequential and skip-sequential methods), and the
+    cost model's own choice; all with exact verification and the same
+    answers.  The first bounded call builds the iSAX words of the length."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    w = synth.make("c0", n=a.n if a.n != 10**8 else None)
+    k, rho = 10, 0.8
+    out = {"config": {"workload": f"plain small workload: n={w.x.size}, {len(w.templates)} templates m=256, "
+                                  f"kNN k={k}, threshold rho>={rho}", "segments": a.segments}}
+    legs = (("sequential", "always", False), ("skip_sequential", "always", True), ("cost_model", "auto", False))
+    res, answers = {}, {}
+    for name, scan, lb in legs:
+        with Context(precision=a.prec, scan=scan, lower_bounds=lb, paa_segments=a.segments) as ctx:
+            ctx.load_waveform(w.x)
+            t0 = time.perf_counter()
+            ctx.knn(w.templates, k, 0)  # first call: words of the length, tuning
+            first = time.perf_counter() - t0
+            row = {"first_call_ms": 1e3 * first}
+            for kind in ("knn", "threshold"):
+                t0 = time.perf_counter()
+                for _ in range(a.steps):
+                    if kind == "knn":
+                        pos, d, st = ctx.knn(w.templates, k, 0)
+                        ans = (pos.copy(), d.copy())
+                    else:
+                        r = ctx.threshold(w.templates, rho, 0)
+                        st = r.stats
+                        ans = (r.tmpl.copy(), r.pos.copy(), r.d.copy())
+                el = (time.perf_counter() - t0) / a.steps
+                row[kind] = {"ms_per_step": 1e3 * el, "comparisons_per_s": st["comparisons"] / el,
+                             "kernel_ms_main": st["kernel_ms_main"], "scan_templates": st["scan_templates"],
+                             "scan_blocks_per_window": st["scan_blocks"], "lb_skipped": st["lb_skipped"],
+                             "lb_skipped_frac": st["lb_skipped"] / max(1, st["comparisons"]),
+                             "candidates": st["candidates"]}
+                answers[(name, kind)] = ans
+                print(f"[bench] c0 {name} {kind}: {1e3 * el:.2f} ms/step, main {st['kernel_ms_main']:.2f} ms, "
+                      f"scan templates {st['scan_templates']}, skipped by the bound {st['lb_skipped']}",
+                      file=sys.stderr, flush=True)
+            res[name] = row
+
+    def same(x, y):
+        return all(np.array_equal(u.view(np.uint32) if u.dtype == np.float32 else u,
+                                  v.view(np.uint32) if v.dtype == np.float32 else v) for u, v in zip(x, y))
+
+    agree = all(same(answers[("sequential", kd)], answers[(nm, kd)])
+                for nm in ("skip_sequential", "cost_model") for kd in ("knn", "threshold"))
+    out.update({"metric": METRIC, "unit": "comparisons/s", "n_gpus": 1, "forms": res, "same_answers": bool(agree),
+                "steps": a.steps, "higher_is_better": True,
+                "value": res["skip_sequential"]["knn"]["comparisons_per_s"],
+                "dtype": "fp32 early-abandoning scan + iSAX bound + exact verify", "data": "synthetic"})
+    print(json.dumps(out), flush=True)
+
+
 def _serve(ctx, stream, lanes, max_batch, label):
     """Replays the stream's arrival times against the serving loop; returns
     latency percentiles and throughput from the library's own timestamps."""
@@ -362,6 +421,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    if a.config == "c0":
+        return bench_skip_sequential(a)
     if a.config == "c4":
         return bench_long(a)
     if a.config == "c3":

@@ -43,6 +43,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kModeSample = 2;  // count the blocks each sampled window visits
+constexpr int kTableSegments = 16;  // lower bounds over at most this many segments use a gap table
+constexpr uint32_t kBlocksPerTemplate = 2048;  // main mode: workgroups per template (windows in a stride loop)
 
 __device__ inline double gamma32(double k)
 {
@@ -52,61 +54,37 @@ __device__ inline double gamma32(double k)
 
 __device__ inline double sq(double v) { return v * v; }
 
-// ids[0..count): the templates of this pass (blockIdx.y).  Main mode: the
-// windows [256 blockIdx.x, + 256) of each; sample mode: sample_n windows
-// spread evenly over the template's windows, visited[t] += blocks visited,
-// visited[T + t] += windows sampled.  A template with a lower bound (lb_seg >
-// 0) skips every window whose bound exceeds its own bound before reading a
-// sample (SkipSequential.cpp:256, :321); main mode counts them in *skipped.
-__global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev *__restrict__ tmpl, int T,
-                                                      const float *__restrict__ qz, const int32_t *__restrict__ order,
-                                                      const int32_t *__restrict__ ids, int mode, uint64_t sample_n,
-                                                      CandOut co, unsigned long long *__restrict__ visited,
-                                                      const float *__restrict__ bp_g, const float *__restrict__ paa_g,
-                                                      unsigned long long *__restrict__ skipped)
+// One window of the early-abandoning scan (index i of the template's
+// windows, or of its sample); true when the lower bound skipped it.
+__device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__restrict__ tmpl, int T, int t, int m,
+                                            const float *__restrict__ qz, const int32_t *__restrict__ order, int mode,
+                                            uint64_t sample_n, const CandOut &co,
+                                            unsigned long long *__restrict__ visited, const float *bp,
+                                            const float *paa, const float *gap, int lb_seg, uint64_t i)
 {
     constexpr double u = 5.9604644775390625e-08;
     constexpr double u64 = 1.1102230246251565e-16;
-    const int t = ids[blockIdx.y];
-    const int m = tmpl[t].m;
-    if (static_cast<uint64_t>(m) > w.n)
-        return;
-    const int lb_seg = tmpl[t].lb_seg;
-    __shared__ float bp[256];
-    __shared__ float paa[kMaxSegments];
-    if (lb_seg > 0) {
-        for (int i = threadIdx.x; i < 255; i += kThreads)
-            bp[i] = bp_g[i];
-        if (threadIdx.x < lb_seg)
-            paa[threadIdx.x] = paa_g[tmpl[t].lb_paa + threadIdx.x];
-        __syncthreads();
-    }
     const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
     uint64_t j;
     if (mode == kModeSample) {
         if (i >= sample_n)
-            return;
+            return false;
         j = (i * W) / sample_n;
     } else {
         if (i >= W)
-            return;
+            return false;
         j = i;
     }
     if (w.x[j] == kMissing)
-        return;  // never searched (SkipSequential.cpp:256)
+        return false;  // never searched (SkipSequential.cpp:256)
     if (lb_seg > 0) {
-        const float lb = sax_lower_bound(bp, paa, tmpl[t].lb_words + j * static_cast<uint64_t>(lb_seg), m, lb_seg);
+        const uint8_t *word = tmpl[t].lb_words + j * static_cast<uint64_t>(lb_seg);
+        const float lb = lb_seg == 16 ? sax_lower_bound16_table(gap, word, m)
+                                      : sax_lower_bound(bp, paa, word, m, lb_seg);
         if (lb > tmpl[t].bound) {
             if (mode == kModeSample)
                 atomicAdd(&visited[T + t], 1ull);  // sampled, no block visited
-            else if (skipped) {
-                // one atomic per wavefront: the lanes skipping here
-                const unsigned long long mask = __ballot(1);
-                if (__lane_id() == __ffsll(static_cast<long long>(mask)) - 1)
-                    atomicAdd(skipped, static_cast<unsigned long long>(__popcll(mask)));
-            }
-            return;
+            return true;
         }
     }
     const int nb = m / kBlock;
@@ -130,7 +108,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
             cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN, -INFINITY,
                      INFINITY);
         }
-        return;
+        return false;
     }
     // sigma = sqrt(V_exact / m) with |V_exact - V| <= eV, rV = eV / V <= 1/4:
     // |sigma / sig - 1| <= rV (sqrt(1 + x) - 1 <= x / 2, 1 - sqrt(1 - x) <= x).
@@ -183,10 +161,10 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
     if (mode == kModeSample) {
         atomicAdd(&visited[t], static_cast<unsigned long long>(abandoned ? b : nb + 1));
         atomicAdd(&visited[T + t], 1ull);
-        return;
+        return false;
     }
     if (abandoned)
-        return;
+        return false;
     float ts = 0.0f;
     for (int k = nb * kBlock; k < m; k++) {
         float rv = r[k];
@@ -200,6 +178,62 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
     const double hi = (1.0 + g_ref) * sq((rs * (1.0 + g_acc + 1.01 * u) + Tm) / (1.0 - u));
     if (!(lo > bound))
         cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), s, lo, hi);
+    return false;
+}
+
+// ids[0..count): the templates of this pass (blockIdx.y).  Main mode: the
+// windows [256 blockIdx.x, + 256) of each; sample mode: sample_n windows
+// spread evenly over the template's windows, visited[t] += blocks visited,
+// visited[T + t] += windows sampled.  A template with a lower bound (lb_seg >
+// 0) skips every window whose bound exceeds its own bound before reading a
+// sample (SkipSequential.cpp:256, :321); main mode counts them in *skipped.
+__global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev *__restrict__ tmpl, int T,
+                                                      const float *__restrict__ qz, const int32_t *__restrict__ order,
+                                                      const int32_t *__restrict__ ids, int mode, uint64_t sample_n,
+                                                      CandOut co, unsigned long long *__restrict__ visited,
+                                                      const float *__restrict__ bp_g, const float *__restrict__ paa_g,
+                                                      unsigned long long *__restrict__ skipped)
+{
+    const int t = ids[blockIdx.y];
+    const int m = tmpl[t].m;
+    if (static_cast<uint64_t>(m) > w.n)
+        return;
+    const int lb_seg = tmpl[t].lb_seg;
+    // Lower bound: the breakpoints and the template's PAA; up to 16 segments
+    // also every (segment, symbol) gap of the template's PAA value from the
+    // symbol's region, so a window's bound is 16 table reads.
+    __shared__ float bp[256];
+    __shared__ float paa[kMaxSegments];
+    __shared__ float gap[kTableSegments * 256];
+    if (lb_seg > 0) {
+        for (int i = threadIdx.x; i < 255; i += kThreads)
+            bp[i] = bp_g[i];
+        if (threadIdx.x < lb_seg)
+            paa[threadIdx.x] = paa_g[tmpl[t].lb_paa + threadIdx.x];
+        __syncthreads();
+        if (lb_seg <= kTableSegments) {
+            for (int e = threadIdx.x; e < lb_seg * 256; e += kThreads)
+                gap[e] = sax_gap(bp, paa[e >> 8], e & 255);
+            __syncthreads();
+        }
+    }
+    const uint64_t stride = mode == kModeSample ? 0 : static_cast<uint64_t>(gridDim.x) * kThreads;
+    unsigned long long nskip = 0;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;;) {
+        nskip += scan_window(w, tmpl, T, t, m, qz, order, mode, sample_n, co, visited, bp, paa, gap, lb_seg, i) ? 1 : 0;
+        if (mode == kModeSample)
+            break;
+        i += stride;
+        if (i >= w.n - static_cast<uint64_t>(m) + 1)
+            break;
+    }
+    // Windows the bound skipped: one atomic per wavefront.
+    if (skipped && lb_seg > 0) {
+        for (int off = 32; off > 0; off >>= 1)
+            nskip += __shfl_xor(nskip, off);
+        if ((threadIdx.x & 63) == 0 && nskip)
+            atomicAdd(skipped, nskip);
+    }
 }
 
 __global__ void k_mark_ea(TmplDev *__restrict__ tmpl, const int32_t *__restrict__ ids, int count)
@@ -221,7 +255,9 @@ hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const f
     if (count <= 0)
         return hipSuccess;
     const uint64_t win = sample ? sample_n : max_windows;
-    const uint64_t gx = (win + kThreads - 1) / kThreads;
+    uint64_t gx = (win + kThreads - 1) / kThreads;
+    if (!sample && gx > kBlocksPerTemplate)
+        gx = kBlocksPerTemplate;  // the windows in a stride loop
     if (gx == 0)
         return hipSuccess;
     if (gx > 0x7fffffffull || count > 65535)

@@ -67,6 +67,27 @@ __device__ inline float sax_lower_bound(const float *bp, const float *paa, const
     return ratio * dist;
 }
 
+// The 16-segment bound from a table of the template's gaps, gap[256 s + v]
+// = sax_gap(bp, paa[s], v): the same FP32 squares and the same sums.
+__device__ inline float sax_lower_bound16_table(const float *gap, const uint8_t *word, int m)
+{
+    const float ratio = static_cast<float>(m) / 16.0f;
+    const uint4 wv = *reinterpret_cast<const uint4 *>(word);
+    const uint32_t wd[4] = {wv.x, wv.y, wv.z, wv.w};
+    float h[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        float q[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const float d = gap[256 * (4 * g + e) + static_cast<int>((wd[g] >> (8 * e)) & 0xffu)];
+            q[e] = d * d;
+        }
+        h[g] = (q[0] + q[1]) + (q[2] + q[3]);
+    }
+    return ((h[0] + h[2]) + (h[1] + h[3])) * ratio;
+}
+
 // The 255 breakpoints of cardinality 256 (SAXBreakpoints.hpp, its
 // sub-table at (c - 1)(c - 2) / 2): the standard normal quantiles
 // Phi^-1(i / 256) rounded to float (host).
