@@ -41,7 +41,13 @@ constexpr int kN1 = 256;     // length of the first-step transforms
 constexpr int kCols = 16;    // columns (first-step transforms) per workgroup of kernel A
 constexpr int kElems = 4096;  // complex values per workgroup of either kernel
 
-__device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// Complex product with fused multiply-adds: one rounded product and one
+// rounded fused sum per component (relative error at most 2u of |a||b|, inside
+// the per-level term below).
+__device__ inline float2 cmul(float2 a, float2 b)
+{
+    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
+}
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
@@ -77,7 +83,9 @@ __device__ __forceinline__ void dft_reg(float2 *v, const float2 *tw)
 #pragma unroll
             for (int j = 0; j < len / 2; j++) {
                 const float2 a = v[i + j];
-                const float2 b = j == 0 ? v[i + j + len / 2] : cmul(tw[j * (S / len)], v[i + j + len / 2]);
+                const float2 x = v[i + j + len / 2];
+                // w^{S/4} = i exactly: a swap instead of a table product.
+                const float2 b = j == 0 ? x : 4 * j == len ? make_float2(-x.y, x.x) : cmul(tw[j * (S / len)], x);
                 v[i + j] = cadd(a, b);
                 v[i + j + len / 2] = csub(a, b);
             }
@@ -199,6 +207,39 @@ struct PairFilter {
     double Q2, S, bound, eabs, eta;
     float gmin, b0, eta_sq, sqrt_m, rsqrt_m, absS, mf, inv_mf;
 };
+
+// m sigma^2 of a window from its prefix pairs ra (start) and rz (end), and
+// whether it is judged (the FP32 interval of V is narrow and the window's
+// mean small enough next to its spread for the level-1 bound to hold).
+__device__ inline float window_var(const PairFilter &F, float c, float2 ra, float2 rz, bool *judge)
+{
+    const float S1 = rz.x - ra.x, S2 = rz.y - ra.y;
+    const float eS1 = 2.0f * F.bi.y + 1.2e-7f * (fabsf(ra.x) + fabsf(rz.x) + fabsf(S1));
+    const float eS2 = 2.0f * F.bi.z + 1.2e-7f * (fabsf(ra.y) + fabsf(rz.y) + fabsf(S2));
+    const float V = S2 - S1 * S1 * F.inv_mf;
+    const float eV =
+        eS2 + (2.0f * fabsf(S1) * eS1 + eS1 * eS1) * F.inv_mf + 2.4e-7f * (fabsf(S2) + S1 * S1 * F.inv_mf);
+    const float aD = fabsf(S1 * F.inv_mf) + eS1 * F.inv_mf;
+    const float amu = fabsf(c) + aD;
+    *judge = V > 1e3f * eV && V > F.mf * 1e-20f && !(amu * amu * F.mf > 1e6f * V);
+    return V;
+}
+
+// The level-1 test of the main pass on one window (dt: its FFT dot product):
+// false only for a window whose distance provably exceeds the bound.
+__device__ inline bool window_survives(const PairFilter &F, float c, float2 ra, float2 rz, float dt)
+{
+    bool judge;
+    const float V = window_var(F, c, ra, rz, &judge);
+    if (!judge || F.gmin <= 0.0f)
+        return true;
+    const float S1 = rz.x - ra.x;
+    const float eS1 = 2.0f * F.bi.y + 1.2e-7f * (fabsf(ra.x) + fabsf(rz.x) + fabsf(S1));
+    const float aD = fabsf(S1 * F.inv_mf) + eS1 * F.inv_mf;
+    const float sV = V * __frsqrt_rn(V);
+    const float b = F.b0 + F.eta_sq * (sV + F.sqrt_m * aD) + aD * F.absS;
+    return dt >= F.gmin * sV * F.rsqrt_m * 0.999f - b * 1.001f - 1e-30f;
+}
 
 // Kernel B.  Work item (pair, row block): rows r1 = RB block + row, RB =
 // 4096 / N2.  FILTER: the filter on the outputs y[2r], y[2r+1] (the windows
@@ -327,18 +368,16 @@ __global__ __launch_bounds__(kThreads) void k_large_b(Waveform w, const TmplDev 
     F.inv_mf = 1.0f / F.mf;
     const float c = F.bi.x;
     const float invB = 1.0f / static_cast<float>(B);
-    float best_g = -INFINITY;
-    uint32_t best_j = 0xffffffffu;
-    float best_dt = 0.0f;
     // Element (row, r2) holds outputs n = 2 r and 2 r + 1, r = r10 + row +
     // 256 r2; consecutive threads take consecutive rows (runs of 2 RB
-    // consecutive windows).  Window i = n - (m - 1) of the piece, j = base + i.
-    // Four elements (eight windows) at a time: their prefix pairs are read
-    // first (indices clamped into the piece's windows, so every read is in
-    // bounds: i + m <= step - 1 + Lmax = B < rb_len), then filtered.
+    // consecutive windows).  Window i = n - (m - 1) of the piece, j = base + i,
+    // valid for 0 <= i < iend.  Four elements (eight windows) at a time: their
+    // prefix pairs are read first (indices clamped into the piece's windows,
+    // so every read is in bounds: i + m <= step - 1 + Lmax = B < rb_len), then
+    // filtered.
     static_assert((RB * N2) % (4 * kThreads) == 0, "whole rounds of four elements");
-    for (int e0 = tid; e0 < RB * N2; e0 += 4 * kThreads) {
-        float2 ra4[4][2], rz4[4][2];
+    const int iend = F.base >= F.W ? 0 : static_cast<int>(min(static_cast<uint64_t>(step), F.W - F.base));
+    const auto load8 = [&](int e0, float2 (&ra4)[4][2], float2 (&rz4)[4][2]) {
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int e = e0 + u * kThreads;
@@ -351,57 +390,117 @@ __global__ __launch_bounds__(kThreads) void k_large_b(Waveform w, const TmplDev 
                 rz4[u][h] = F.rbp[i + F.m];
             }
         }
+    };
+    if (mode == kModeMain) {
+        // A cheap rejection shared by a thread's eight windows of a round: the
+        // level-1 test with its error terms (eS1, eS2, eV, aD, amu) replaced by
+        // upper bounds from the largest prefix pair magnitudes among the eight
+        // (|S1| <= |ra.x| + |rz.x|, |S2| <= |ra.y| + |rz.y|), with relative
+        // slack of at least 1e-5 over the FP32 roundings of both evaluations.
+        // Larger error terms make every step of that test more conservative,
+        // so a window rejected here fails window_survives too; the others take
+        // window_survives itself, and the survivors are exactly its survivors.
+        const double Kd = static_cast<double>(F.gmin) * static_cast<double>(F.rsqrt_m) * 0.999 * (1.0 - 1e-5) -
+                          1.001 * static_cast<double>(F.eta_sq) * (1.0 + 1e-5);
+        const bool cheap = F.gmin > 0.0f && Kd > 0.0;
+        const float Kc = static_cast<float>(Kd * (1.0 - 1e-6));  // slope in sqrt(V), rounded down
+        const float slope_aD = F.eta_sq * F.sqrt_m + F.absS;
+        for (int e0 = tid; e0 < RB * N2; e0 += 4 * kThreads) {
+            float2 ra4[4][2], rz4[4][2];
+            load8(e0, ra4, rz4);
+            float ax = 0.0f, bx = 0.0f, ay = 0.0f, by = 0.0f;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-        const int e = e0 + u * kThreads;
-        const int row = e % RB, r2 = e / RB;
-        const float2 zz = d[row * kPad + at(r2)];
-        const int r = r10 + row + kN1 * r2;
+            for (int u = 0; u < 4; u++)
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int i = 2 * r + h - (F.m - 1);
-            if (i < 0 || i >= step)
-                continue;
-            const uint64_t j = F.base + static_cast<uint64_t>(i);
-            if (j >= F.W)
-                continue;
-            const float dt = (h == 0 ? zz.x : zz.y) * invB;
-            const float2 ra = ra4[u][h], rz = rz4[u][h];
-            const float S1 = rz.x - ra.x, S2 = rz.y - ra.y;
-            const float eS1 = 2.0f * F.bi.y + 1.2e-7f * (fabsf(ra.x) + fabsf(rz.x) + fabsf(S1));
-            const float eS2 = 2.0f * F.bi.z + 1.2e-7f * (fabsf(ra.y) + fabsf(rz.y) + fabsf(S2));
-            const float V = S2 - S1 * S1 * F.inv_mf;
-            const float eV =
-                eS2 + (2.0f * fabsf(S1) * eS1 + eS1 * eS1) * F.inv_mf + 2.4e-7f * (fabsf(S2) + S1 * S1 * F.inv_mf);
-            const float aD = fabsf(S1 * F.inv_mf) + eS1 * F.inv_mf;
-            const float amu = fabsf(c) + aD;
-            const bool judge = V > 1e3f * eV && V > F.mf * 1e-20f && !(amu * amu * F.mf > 1e6f * V);
-            if (mode == kModeMain) {
-                bool survive = !judge || F.gmin <= 0.0f;
-                if (!survive) {
-                    const float sV = V * __frsqrt_rn(V);
-                    const float b = F.b0 + F.eta_sq * (sV + F.sqrt_m * aD) + aD * F.absS;
-                    survive = dt >= F.gmin * sV * F.rsqrt_m * 0.999f - b * 1.001f - 1e-30f;
+                for (int h = 0; h < 2; h++) {
+                    ax = fmaxf(ax, fabsf(ra4[u][h].x));
+                    bx = fmaxf(bx, fabsf(rz4[u][h].x));
+                    ay = fmaxf(ay, fabsf(ra4[u][h].y));
+                    by = fmaxf(by, fabsf(rz4[u][h].y));
                 }
-                if (survive && w.x[j] != kMissing) {
+            const float Mx = (ax + bx) * 1.0001f, My = (ay + by) * 1.0001f;
+            const float eS1 = 2.0001f * F.bi.y + 2.41e-7f * Mx;
+            const float eS2 = 2.0001f * F.bi.z + 2.41e-7f * My;
+            const float aD = (Mx + eS1) * F.inv_mf;
+            const float eV = eS2 + (2.0f * Mx * eS1 + eS1 * eS1) * F.inv_mf + 2.41e-7f * (My + Mx * Mx * F.inv_mf);
+            const float amu = fabsf(c) + aD;
+            const float VT =
+                cheap ? fmaxf(fmaxf(1e3f * eV, F.mf * 1e-20f), amu * amu * F.mf * 1e-6f) * 1.00001f : INFINITY;
+            const float Cg = (1.001f * (F.b0 + slope_aD * aD) + 1e-30f) * 1.00001f;
+            uint32_t need = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u * kThreads;
+                const int row = e % RB, r2 = e / RB;
+                const float2 zz = d[row * kPad + at(r2)];
+                const int r = r10 + row + kN1 * r2;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int i = 2 * r + h - (F.m - 1);
+                    const float2 ra = ra4[u][h], rz = rz4[u][h];
+                    const float S1 = rz.x - ra.x, S2 = rz.y - ra.y;
+                    const float V = S2 - S1 * S1 * F.inv_mf;
+                    const float dt = (h == 0 ? zz.x : zz.y) * invB;
+                    const bool reject = V > VT && dt < V * __frsqrt_rn(V) * Kc - Cg;
+                    if (i >= 0 && i < iend && !reject)
+                        need |= 1u << (2 * u + h);
+                }
+            }
+            // The undecided windows one at a time: the full test, then the
+            // FP64 interval (level 2) for its survivors.
+            while (need) {
+                const int bit = __builtin_ctz(need);
+                need &= need - 1;
+                const int e = e0 + (bit >> 1) * kThreads, h = bit & 1;
+                const int row = e % RB, r2 = e / RB;
+                const int i = 2 * (r10 + row + kN1 * r2) + h - (F.m - 1);
+                const float2 zz = d[row * kPad + at(r2)];
+                const float dt = (h == 0 ? zz.x : zz.y) * invB;
+                const uint64_t j = F.base + static_cast<uint64_t>(i);
+                if (window_survives(F, c, F.rbp[i], F.rbp[i + F.m], dt) && w.x[j] != kMissing) {
                     const DistInterval iv =
                         dist_interval(w, j, F.m, F.Q2, F.S, c, static_cast<double>(dt), F.eta, F.eabs);
                     if (!(iv.lo > F.bound))  // NaN (undecidable): a candidate
                         cand_put(co, wave_claim(co.count), static_cast<uint32_t>(F.t), static_cast<uint32_t>(j),
                                  static_cast<float>(iv.est), iv.lo, iv.hi);
                 }
-            } else if (judge && V > 0.0f) {
-                const float g = dt * __frsqrt_rn(V);
-                if (g > best_g) {
-                    best_g = g;
-                    best_j = static_cast<uint32_t>(j);
-                    best_dt = dt;
+            }
+        }
+        return;
+    }
+    // Seeding: per thread the judged window of the largest dt / sqrt(V).
+    float best_g = -INFINITY;
+    uint32_t best_j = 0xffffffffu;
+    float best_dt = 0.0f;
+    for (int e0 = tid; e0 < RB * N2; e0 += 4 * kThreads) {
+        float2 ra4[4][2], rz4[4][2];
+        load8(e0, ra4, rz4);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = e0 + u * kThreads;
+            const int row = e % RB, r2 = e / RB;
+            const float2 zz = d[row * kPad + at(r2)];
+            const int r = r10 + row + kN1 * r2;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int i = 2 * r + h - (F.m - 1);
+                if (i < 0 || i >= iend)
+                    continue;
+                bool judge;
+                const float V = window_var(F, c, ra4[u][h], rz4[u][h], &judge);
+                if (judge && V > 0.0f) {
+                    const float dt = (h == 0 ? zz.x : zz.y) * invB;
+                    const float g = dt * __frsqrt_rn(V);
+                    if (g > best_g) {
+                        best_g = g;
+                        best_j = static_cast<uint32_t>(F.base + static_cast<uint64_t>(i));
+                        best_dt = dt;
+                    }
                 }
             }
         }
-        }
     }
-    if (mode == kModeSeed) {
+    {
         float ub = INFINITY;
         if (best_j != 0xffffffffu && w.x[best_j] != kMissing) {
             const DistInterval iv =
