@@ -51,6 +51,18 @@ __device__ inline float2 cmul(float2 a, float2 b)
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
+// Reads through a buffer resource: a 32-bit per-thread byte offset plus a
+// per-read constant in a scalar register, no 64-bit address per read.
+__device__ inline __amdgpu_buffer_rsrc_t rsrc_of(const float2 *p, uint32_t count)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(p), static_cast<short>(0),
+                                             static_cast<int>(count * 8u), 0x00020000);
+}
+__device__ inline float2 ld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 constexpr int rev_bits(int i, int bits)
 {
     int r = 0;
@@ -137,33 +149,47 @@ __global__ __launch_bounds__(kThreads) void k_large_a(const FftPair *__restrict_
         return;
     const int pair = item / kBlocks;
     const int k20 = (item % kBlocks) * kCols;
-    // The compact twiddle tables after E (E has B = 2 M entries).
+    // The compact twiddle tables after E (E has B = 2 M entries), read into
+    // registers first so that their reads overlap the spectra's.
+    static_assert(kN1 == kThreads && M / 512 <= kThreads, "one table entry per thread");
     const float2 *lt = E + 2 * M;
-    for (int t = tid; t < kN1; t += kThreads)
-        tw[t] = lt[t];
-    for (int t = tid; t < 512; t += kThreads)
-        ta[t] = lt[kN1 + t];
-    for (int t = tid; t < M / 512; t += kThreads)
-        tb[t] = lt[kN1 + 512 + t];
-    const float2 *X = nullptr, *H = nullptr;
+    const float2 t_tw = lt[tid], t_ta0 = lt[kN1 + tid], t_ta1 = lt[kN1 + kThreads + tid];
+    const float2 t_tb = lt[kN1 + 512 + min(tid, M / 512 - 1)];  // no branch around a read
     if (SPEC) {
         const FftPair pr = pairs[pair];
-        X = spectra + static_cast<uint64_t>(pr.p) * (M + 1);
-        H = qspec + static_cast<uint64_t>(slot[pr.t]) * (M + 1);
-    }
-#pragma unroll 4
-    for (int e = tid; e < kCols * kN1; e += kThreads) {
-        const int c = e % kCols, k1 = e / kCols;
-        const int k = k20 + c + N2 * k1;
-        float2 z;
-        if (SPEC) {
-            const int km = M - k;  // bin M (the last) for k = 0
-            z = zval(cmul(X[k], H[k]), cmul(X[km], H[km]), E[k]);
-        } else {
-            z = zin[static_cast<uint64_t>(pair) * M + k];
+        const __amdgpu_buffer_rsrc_t Xr = rsrc_of(spectra + static_cast<uint64_t>(pr.p) * (M + 1), M + 1);
+        const __amdgpu_buffer_rsrc_t Hr = rsrc_of(qspec + static_cast<uint64_t>(slot[pr.t]) * (M + 1), M + 1);
+        const __amdgpu_buffer_rsrc_t Er = rsrc_of(E, M);
+        // Element (c, k1), c = tid % 16, k1 = tid / 16 + 16 q: bin k = k20 + c
+        // + N2 k1 at byte offset vk + q 128 N2, and its mirror M - k (bin M,
+        // the last, for k = 0) at vm + (15 - q) 128 N2.
+        const int c = tid % kCols, k1_0 = tid / kCols;
+        const uint32_t vk = static_cast<uint32_t>(k20 + c + N2 * k1_0) * 8u;
+        const uint32_t vm = static_cast<uint32_t>(16 * N2 - k20 - c - N2 * k1_0) * 8u;
+        // Two rounds of eight elements: all forty reads of a round are issued
+        // before the first is used.
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+#pragma unroll
+            for (int q = 8 * half; q < 8 * half + 8; q++) {
+                const uint32_t sk = static_cast<uint32_t>(q) * 128u * N2;
+                const uint32_t sm = static_cast<uint32_t>(15 - q) * 128u * N2;
+                const float2 z = zval(cmul(ld2(Xr, vk, sk), ld2(Hr, vk, sk)), cmul(ld2(Xr, vm, sm), ld2(Hr, vm, sm)),
+                                      ld2(Er, vk, sk));
+                d[c * kPad + k1_0 + 16 * q] = z;
+            }
         }
-        d[c * kPad + k1] = z;
+    } else {
+        for (int e = tid; e < kCols * kN1; e += kThreads) {
+            const int c = e % kCols, k1 = e / kCols;
+            d[c * kPad + k1] = zin[static_cast<uint64_t>(pair) * M + k20 + c + N2 * k1];
+        }
     }
+    tw[tid] = t_tw;
+    ta[tid] = t_ta0;
+    ta[kThreads + tid] = t_ta1;
+    if (tid < M / 512)
+        tb[tid] = t_tb;
     __syncthreads();
     // 256 = 16 x 16: radix-16 over n2 for each (column, n1), element n1 + 16 n2.
     {
@@ -278,12 +304,27 @@ __global__ __launch_bounds__(kThreads) void k_large_b(Waveform w, const TmplDev 
         return;  // the whole workgroup: no barrier is skipped by part of it
     const int pair = item / kBlocks;
     const int r10 = (item % kBlocks) * RB;
-    for (int t = tid; t < N2; t += kThreads)
-        tw[t] = E[2 * M + kN1 + 512 + M / 512 + t];
-    const float2 *src = U + static_cast<uint64_t>(pair) * M;
-    for (int e = tid; e < RB * N2; e += kThreads) {
-        const int row = e / N2, k2 = e % N2;
-        d[row * kPad + at(k2)] = src[(r10 + row) * N2 + k2];
+    // The row block is RB N2 = 4096 consecutive values of U: sixteen reads
+    // per thread (and the twiddle table's), all issued before any is used.
+    constexpr int kPer = RB * N2 / kThreads;
+    const __amdgpu_buffer_rsrc_t Ur = rsrc_of(U + static_cast<uint64_t>(pair) * M + r10 * N2, RB * N2);
+    float2 uv[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; q++)
+        uv[q] = ld2(Ur, static_cast<uint32_t>(tid) * 8u, static_cast<uint32_t>(q) * kThreads * 8u);
+    const float2 *lt = E + 2 * M + kN1 + 512 + M / 512;
+    float2 tv[(N2 + kThreads - 1) / kThreads];
+#pragma unroll
+    for (int q = 0; q < (N2 + kThreads - 1) / kThreads; q++)
+        tv[q] = lt[min(tid + q * kThreads, N2 - 1)];  // no branch around a read
+#pragma unroll
+    for (int q = 0; q < (N2 + kThreads - 1) / kThreads; q++)
+        if (tid + q * kThreads < N2)
+            tw[tid + q * kThreads] = tv[q];
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+        const int e = tid + q * kThreads;
+        d[(e / N2) * kPad + at(e % N2)] = uv[q];
     }
     __syncthreads();
     // N2 = 16 Q: radix-16 over n2 for each (row, n1 < Q): RB Q = 256 items.
@@ -441,9 +482,9 @@ __global__ __launch_bounds__(kThreads) void k_large_b(Waveform w, const TmplDev 
                     const float S1 = rz.x - ra.x, S2 = rz.y - ra.y;
                     const float V = S2 - S1 * S1 * F.inv_mf;
                     const float dt = (h == 0 ? zz.x : zz.y) * invB;
-                    const bool reject = V > VT && dt < V * __frsqrt_rn(V) * Kc - Cg;
-                    if (i >= 0 && i < iend && !reject)
-                        need |= 1u << (2 * u + h);
+                    // Evaluated for every window (no branch); a NaN compares false.
+                    const bool keep = !(V > VT) | !(dt < V * __frsqrt_rn(V) * Kc - Cg);
+                    need |= static_cast<uint32_t>(keep & (i >= 0) & (i < iend)) << (2 * u + h);
                 }
             }
             // The undecided windows one at a time: the full test, then the
