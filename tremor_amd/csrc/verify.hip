@@ -207,12 +207,21 @@ __device__ bool wave_stats(const float *w, int m, float &mean_f, float &sd_f)
     constexpr double u = 1.1102230246251565e-16;
     const int lane = threadIdx.x & 63;
     const double md = static_cast<double>(m);
+    // Lane sums in index order; eight reads of a round issued before the
+    // first is added (the order of the additions is unchanged).
+    constexpr int kU = 8;
     double s = 0.0, a = 0.0;
-    for (int d = lane; d < m; d += 64) {
-        const float v = w[d];
-        const double dv = v == kMissing ? 0.0 : static_cast<double>(v);
-        s += dv;
-        a += fabs(dv);
+    for (int c0 = lane; c0 < m; c0 += 64 * kU) {
+        float v[kU];
+#pragma unroll
+        for (int k = 0; k < kU; k++)
+            v[k] = c0 + 64 * k < m ? w[c0 + 64 * k] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kU; k++) {
+            const double dv = v[k] == kMissing ? 0.0 : static_cast<double>(v[k]);
+            s += dv;
+            a += fabs(dv);
+        }
     }
     // Every lane takes lane 0's sums, so the decisions below are uniform.
     s = __shfl(wave_sum_f64(s), 0);
@@ -242,10 +251,18 @@ __device__ bool wave_stats(const float *w, int m, float &mean_f, float &sd_f)
     const double mu = 0.5 * (ml + mh);
     const double W = 0.5 * (mh - ml) * (1.0 + 4.0 * u);
     double q = 0.0;
-    for (int d = lane; d < m; d += 64) {
-        const float v = w[d];
-        const double dv = (v == kMissing ? 0.0 : static_cast<double>(v)) - mu;
-        q = __builtin_fma(dv, dv, q);
+    for (int c0 = lane; c0 < m; c0 += 64 * kU) {
+        float v[kU];
+#pragma unroll
+        for (int k = 0; k < kU; k++)
+            v[k] = c0 + 64 * k < m ? w[c0 + 64 * k] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kU; k++) {
+            if (c0 + 64 * k < m) {
+                const double dv = (v[k] == kMissing ? 0.0 : static_cast<double>(v[k])) - mu;
+                q = __builtin_fma(dv, dv, q);
+            }
+        }
     }
     q = __shfl(wave_sum_f64(q), 0);
     const double e2 = 2.01 * u + gp;
