@@ -79,3 +79,37 @@ def test_one_template_over_the_limit_is_an_error(work):
         # the same error again (nothing left behind by the failed call)
         with pytest.raises(TmxError, match="one template has more candidate windows"):
             ctx.threshold(tm[:1], 0.3, 0)
+
+
+def test_more_templates_than_one_batch(oracle):
+    """A call of 70,000 templates runs as batches of at most 65,535 in caller
+    order: every template's answer equals the one it gets in a call of its
+    own group of ten, across the batch seam, and matches the oracle."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    rng = np.random.default_rng(91)
+    n = 1 << 13
+    x = synth.bandpassed_noise(n, rng)
+    count, m = 70000, 32
+    starts = rng.integers(0, n - m, size=count)
+    tm = [x[s:s + m] + rng.normal(0.0, 0.1, m).astype(np.float32) for s in starts]
+    seam = list(range(65530, 65540))
+    _log("many templates", count=count, m=m, n=n)
+    with Context(precision="bf16x3", fft="off", scan="off") as ctx:
+        ctx.load_waveform(x)
+        pos, d, _ = ctx.knn(tm, 3, 4)
+        r = ctx.threshold(tm, 0.97, 0)
+        _log("many templates: groups of ten")
+        for lo in (0, 65530, 69990):
+            ids = list(range(lo, lo + 10))
+            sub = [tm[i] for i in ids]
+            ps, ds, _ = ctx.knn(sub, 3, 4)
+            assert np.array_equal(pos[lo:lo + 10], ps) and np.array_equal(d[lo:lo + 10].view(np.uint32),
+                                                                         ds.view(np.uint32))
+            rs = ctx.threshold(sub, 0.97, 0)
+            keep = (r.tmpl >= lo) & (r.tmpl < lo + 10)
+            assert np.array_equal(r.tmpl[keep] - lo, rs.tmpl) and np.array_equal(r.pos[keep], rs.pos)
+            assert np.array_equal(r.d[keep].view(np.uint32), rs.d.view(np.uint32))
+    sub = [tm[i] for i in seam]
+    po, do, _ = oracle.scan_knn(x, sub, 3, 4)
+    assert np.array_equal(pos[seam], po) and np.array_equal(d[seam].view(np.uint32), do.view(np.uint32))

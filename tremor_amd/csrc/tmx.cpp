@@ -1536,6 +1536,7 @@ void add_stats(tmx_stats &a, const tmx_stats &b)
     a.seconds_tune += b.seconds_tune;
     a.fft_lmax = std::max(a.fft_lmax, b.fft_lmax);
     a.splits += b.splits;
+    a.lb_skipped += b.lb_skipped;
 }
 
 }  // namespace
@@ -1583,19 +1584,41 @@ int check_common(tmx_ctx *ctx, const float *templates, const int32_t *lengths, c
         return ctx->fail("template count must be positive");
     if (!templates || !lengths || !offsets)
         return ctx->fail("null template arguments");
-    if (count > 65535)
-        return ctx->fail("at most 65535 templates per call");
     return 0;
 }
 
-// Runs a synchronous batch on the direct lane; errors land in ctx->err.
+// Most templates one batch takes (the per-template launches index templates
+// by the grid's second dimension).
+constexpr int kBatchTemplates = 65535;
+
+// Runs a synchronous call on the direct lane as batches of at most
+// kBatchTemplates templates in caller order (every template's answer is the
+// same in any batch); errors land in ctx->err.
 int direct_batch(tmx_ctx *ctx, const std::vector<Query> &q, std::vector<QueryOut> &out, tmx_stats *stats)
 {
     Lane &L = ctx->direct;
-    const int rc = run_batch(ctx, L, ctx->cfg, q.data(), static_cast<int>(q.size()), out, stats);
-    if (rc)
-        ctx->fail(L.err, rc);
-    return rc;
+    const size_t count = q.size();
+    out.clear();
+    out.reserve(count);
+    tmx_stats total{};
+    for (size_t c0 = 0; c0 < count; c0 += kBatchTemplates) {
+        const int cnt = static_cast<int>(std::min<size_t>(kBatchTemplates, count - c0));
+        std::vector<QueryOut> part;
+        tmx_stats st{};
+        const int rc = run_batch(ctx, L, ctx->cfg, q.data() + c0, cnt, part, &st);
+        if (rc) {
+            ctx->fail(L.err, rc);
+            return rc;
+        }
+        out.insert(out.end(), std::make_move_iterator(part.begin()), std::make_move_iterator(part.end()));
+        if (c0 == 0)
+            total = st;
+        else
+            add_stats(total, st);
+    }
+    if (stats)
+        *stats = total;
+    return 0;
 }
 
 }  // namespace
