@@ -61,6 +61,8 @@ def _args():
     ap.add_argument("--long-subset", type=int, default=8, help="c4: templates of the GEMM and scan legs")
     ap.add_argument("--legs", default="fft,fft_subset,gemm_subset,scan_subset,auto",
                     help="c4: comma-separated legs to run (the default runs all and compares their answers)")
+    ap.add_argument("--launch", default="auto", choices=["auto", "grid", "queue"],
+                    help="c1: matrix-core launch form (auto: timed by the autotuner with the group size)")
     ap.add_argument("--group", type=int, default=0, choices=[0, 4, 8, 16],
                     help="c1: templates per matrix-core workgroup, fixed (0: autotuned on the warmup call)")
     return ap.parse_args()
@@ -448,7 +450,7 @@ def main():
 
     # The headline measures the matrix-core form (every template on MFMA);
     # the cost model's own choice is measured after it.
-    ctx = Context(precision=a.prec, fft="off", scan="off", autotune=a.group if a.group else True)
+    ctx = Context(precision=a.prec, fft="off", scan="off", autotune=a.group if a.group else True, launch=a.launch)
     t0 = time.time()
     ctx.load_waveform(w.x)
     load_s = time.time() - t0
@@ -558,7 +560,10 @@ def main():
         "max_abs_rho_err": last["max_abs_rho_err"],
         "phase_ms": {k: 1e3 * last[k] for k in ("seconds_seed", "seconds_main", "seconds_verify", "seconds_select")},
         "autotune": {"templates_per_workgroup": last["tuned_group"],
-                     "note": "timed on the first (warmup) call among 4, 8, 16 and kept for the context"},
+                     "launch": {1: "grid", 2: "queue"}.get(last["tuned_launch"], "?"),
+                     "note": "timed on the first (warmup) call among groups 4, 8, 16 x launch forms grid "
+                             "(one workgroup per block and group) and queue (persistent, per-XCD work queues), "
+                             "kept for the context"},
     }
     print(json.dumps(out), flush=True)
 

@@ -68,6 +68,15 @@ enum tmx_scan_mode {
     TMX_SCAN_ALWAYS = 2  /* every template */
 };
 
+/* Launch form of the matrix-core pass (no counterpart in the reference: its
+ * CPU scan has no launches).  Answers do not depend on it. */
+enum tmx_launch {
+    TMX_LAUNCH_AUTO = 0,  /* timed with the group sizes (tmx_cfg.autotune = 1), else the grid */
+    TMX_LAUNCH_GRID = 1,  /* one workgroup per (waveform block, template group) item */
+    TMX_LAUNCH_QUEUE = 2  /* one persistent workgroup per CU claiming items from per-XCD work
+                             queues, stealing from the other XCDs' queues once its own is empty */
+};
+
 typedef struct {
     int32_t device;          /* HIP device ordinal */
     int32_t precision;       /* enum tmx_precision */
@@ -79,9 +88,10 @@ typedef struct {
     int32_t verify_seq;      /* 1 = the exact recomputation takes every window's statistics by the
                                 sequential sums (tests, diagnostics); 0 = default (a rigorous
                                 parallel shortcut, sequential only where it cannot decide) */
-    int32_t autotune;        /* 1 = default: the templates per matrix-core workgroup are timed on a
-                                sampled pass the first time a batch shape is seen and the fastest
-                                kept for the context; 0 = fixed (8); 4, 8 or 16 = that size, fixed */
+    int32_t autotune;        /* 1 = default: the templates per matrix-core workgroup (4, 8, 16) and
+                                the launch form (mfma_launch) are timed on a sampled pass the first
+                                time a batch shape is seen and the fastest kept for the context;
+                                0 = fixed (8); 4, 8 or 16 = that size, fixed */
     int64_t max_candidates;  /* most candidate windows one candidate pass may hold; 0 = default
                                 (2^31 - 1, less if device memory is short).  A call whose templates
                                 together would exceed it is split into halves of its templates, run
@@ -95,6 +105,7 @@ typedef struct {
                                 paa_segments scan without bounds.  0 = default (off) */
     int32_t paa_segments;    /* PAA segments of those bounds, 1..64 (SkipSequentialScan's
                                 paa_segments); default 16 */
+    int32_t mfma_launch;     /* enum tmx_launch; default TMX_LAUNCH_AUTO */
 } tmx_cfg;
 
 typedef struct {
@@ -122,6 +133,7 @@ typedef struct {
     int32_t splits;          /* times the call's templates were split in two because their
                                 candidates exceeded max_candidates (0: one pass) */
     int64_t lb_skipped;      /* windows the lower bound skipped in the early-abandoning scan */
+    int32_t tuned_launch;    /* enum tmx_launch of the matrix-core pass of this call */
 } tmx_stats;
 
 typedef struct {

@@ -414,6 +414,43 @@ def test_autotuned_group_size(oracle):
         assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
 
 
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16"])
+def test_persistent_launch_equals_grid(oracle, prec):
+    """The matrix-core pass as one persistent workgroup per CU claiming
+    (block, group) items from per-XCD queues returns the same answers, bit for
+    bit, as one workgroup per item, for every group size, on a waveform of
+    many blocks and uneven template lengths; the autotuner reports the form it
+    chose, and the answers are the oracle's."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(2_000_000, 59, events=60)
+    lengths = [int(v) for v in rng.integers(150, 2000, size=40)]
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    res = {}
+    for launch in ("grid", "queue"):
+        for g in (4, 8, 16):
+            with Context(precision=prec, fft="off", scan="off", autotune=g, launch=launch) as ctx:
+                ctx.load_waveform(x)
+                _log("persistent-launch", prec=prec, launch=launch, group=g)
+                pos, d, st = ctx.knn(tm, 20, 8)
+                assert st["tuned_launch"] == (1 if launch == "grid" else 2) and st["tuned_group"] == g
+                r = ctx.threshold(tm, 0.75, 0)
+                res[(launch, g)] = (pos, d, r)
+    p0, d0, r0 = res[("grid", 8)]
+    for key, (p, d, r) in res.items():
+        assert np.array_equal(p, p0) and np.array_equal(d.view(np.uint32), d0.view(np.uint32)), key
+        assert np.array_equal(r.tmpl, r0.tmpl) and np.array_equal(r.pos, r0.pos), key
+        assert np.array_equal(r.d.view(np.uint32), r0.d.view(np.uint32)), key
+    with Context(precision=prec, fft="off", scan="off") as ctx:
+        ctx.load_waveform(x)
+        p, d, st = ctx.knn(tm, 20, 8)
+        assert st["tuned_launch"] in (1, 2) and st["tuned_group"] in (4, 8, 16)
+        assert np.array_equal(p, p0) and np.array_equal(d.view(np.uint32), d0.view(np.uint32))
+    sub = list(range(0, 40, 5))
+    po, do, _ = oracle.scan_knn(x, [tm[i] for i in sub], 20, 8)
+    assert np.array_equal(p0[sub], po) and np.array_equal(d0[sub].view(np.uint32), do.view(np.uint32))
+
+
 def test_fft_shape_groups_in_one_batch(oracle):
     """One batch whose FFT templates fall in every overlap-save shape (the
     four fused ones and a hipFFT length): each group runs in its own shape,

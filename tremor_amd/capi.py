@@ -40,7 +40,7 @@ class Cfg(ctypes.Structure):
                 ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32), ("fft_mode", ctypes.c_int32),
                 ("scan_mode", ctypes.c_int32), ("verify_seq", ctypes.c_int32), ("autotune", ctypes.c_int32),
                 ("max_candidates", ctypes.c_int64), ("lower_bounds", ctypes.c_int32),
-                ("paa_segments", ctypes.c_int32)]
+                ("paa_segments", ctypes.c_int32), ("mfma_launch", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -52,7 +52,7 @@ class Stats(ctypes.Structure):
                 ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64),
                 ("scan_templates", ctypes.c_int32), ("scan_blocks", ctypes.c_double),
                 ("tuned_group", ctypes.c_int32), ("seconds_tune", ctypes.c_double), ("fft_lmax", ctypes.c_int32),
-                ("splits", ctypes.c_int32), ("lb_skipped", ctypes.c_int64)]
+                ("splits", ctypes.c_int32), ("lb_skipped", ctypes.c_int64), ("tuned_launch", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -191,7 +191,7 @@ class Context:
     def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
                  seed_target: int = 0, verbose: bool = False, fft: str = "auto", scan: str = "auto",
                  verify_seq: bool = False, autotune: bool | int = True, max_candidates: int = 0,
-                 lower_bounds: bool = False, paa_segments: int = 16):
+                 lower_bounds: bool = False, paa_segments: int = 16, launch: str = "auto"):
         L = lib()
         cfg = Cfg()
         L.tmx_cfg_default(ctypes.byref(cfg))
@@ -209,6 +209,7 @@ class Context:
         # the skip-sequential method's iSAX lower bound on the early-abandoning scan
         cfg.lower_bounds = 1 if lower_bounds else 0
         cfg.paa_segments = int(paa_segments)
+        cfg.mfma_launch = {"auto": 0, "grid": 1, "queue": 2}[launch]
         h = ctypes.c_void_p()
         rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:

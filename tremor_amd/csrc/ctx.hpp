@@ -96,6 +96,7 @@ struct Lane {
     uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
     // matrix-core form: queue of level-1 survivors for the rigorous second level
     DevMem surv, surv_n;
+    DevMem queue;  // the persistent matrix-core launch's per-XCD work counters
     uint64_t surv_cap = 0;
     uint64_t surv_need = 0;  // > surv_cap after a main pass that overflowed the queue
     uint64_t last_survivors = 0;  // level-1 survivors of the last main pass
@@ -265,7 +266,7 @@ struct tmx_ctx {
     // autotuner: templates per matrix-core workgroup chosen per batch shape
     // (precision, template count, mean padded length), guarded by tune_mu
     std::mutex tune_mu;
-    std::map<uint32_t, int> tuned_group;
+    std::map<uint32_t, int> tuned_group;  // shape -> group size | launch form << 8
 
     int fail(const std::string &msg, int code = -1)
     {

@@ -225,33 +225,38 @@ __device__ inline DistInterval survivor_interval(const Waveform &w, uint64_t j, 
     return iv;
 }
 
-template <int PREC>
-__global__ __launch_bounds__(kThreads, 1) void k_scan_mfma(Waveform w, const TmplDev *__restrict__ tmpl,
-                                                           const int32_t *__restrict__ group_begin,
-                                                           const Frag *__restrict__ afrag, int mode,
-                                                           int block_stride, int groups, uint32_t nblocks,
-                                                           const float2 *__restrict__ rb,
-                                                           const float4 *__restrict__ binfo, int rb_len,
-                                                           SurvOut so, CandOut co)
+// Work item k8 of XCD xcd's share: on each XCD, consecutive items take
+// kTileB consecutive blocks (of that XCD's share) for one template group, then
+// the next group, so the items in flight on an XCD (one per CU) hold a few
+// blocks' samples and prefix sums and a few groups' operands, both
+// L2-resident.
+__device__ inline void block_of_item(uint32_t xcd, uint32_t k8, int groups, uint32_t *bidx, int *g)
 {
-    constexpr bool kSplit = PREC != TMX_PREC_FP16;
-    constexpr int kLds = kSplit ? kTileBytesBf16 : kTileBytesF16;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[kLds];
-    uint16_t *const lhi = reinterpret_cast<uint16_t *>(lds);
-    uint16_t *const llo = lhi + kStage;
-    const int tid = threadIdx.x;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int col = lane & 31, h = lane >> 5;
-    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
-    // on each XCD, consecutive workgroups take kTileB consecutive blocks (of
-    // that XCD's share) for one template group, then the next group, so the
-    // workgroups in flight on an XCD (one per CU) hold a few blocks' samples
-    // and prefix sums and a few groups' operands, both L2-resident.
-    const uint32_t xcd = blockIdx.x & 7u, k8 = blockIdx.x >> 3;
     const uint32_t per_tile = kTileB * static_cast<uint32_t>(groups);
     const uint32_t rem = k8 % per_tile;
-    const int g = static_cast<int>(rem / kTileB);
-    const uint32_t bidx = ((k8 / per_tile) * kTileB + rem % kTileB) * 8u + xcd;
+    *g = static_cast<int>(rem / kTileB);
+    *bidx = ((k8 / per_tile) * kTileB + rem % kTileB) * 8u + xcd;
+}
+
+// The matrix-core pass over one (block, template group) item; lds: the
+// workgroup's staged tile.
+template <int PREC>
+__device__ __forceinline__ void scan_item(unsigned char *lds, const Waveform &w, const TmplDev *__restrict__ tmpl,
+                                          const int32_t *__restrict__ group_begin, const Frag *__restrict__ afrag,
+                                          int mode, int block_stride, uint32_t nblocks, const float2 *__restrict__ rb,
+                                          const float4 *__restrict__ binfo, int rb_len, const SurvOut &so,
+                                          const CandOut &co, uint32_t bidx, int g)
+{
+    constexpr bool kSplit = PREC != TMX_PREC_FP16;
+    uint16_t *const lhi = reinterpret_cast<uint16_t *>(lds);
+    uint16_t *const llo = lhi + kStage;
+    // Opaque per item: in the persistent form nothing derived from the
+    // thread index is hoisted out of the item loop into registers the K loop
+    // needs.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int col = lane & 31, h = lane >> 5;
     if (bidx >= nblocks)
         return;
     const uint64_t blk = static_cast<uint64_t>(bidx) * static_cast<uint64_t>(block_stride);
@@ -725,6 +730,76 @@ __global__ __launch_bounds__(kThreads, 1) void k_scan_mfma(Waveform w, const Tmp
     }
 }
 
+__global__ void k_zero_queue(uint32_t *queue) { queue[threadIdx.x] = 0u; }
+
+// One workgroup per (block, group) item: workgroups are dispatched
+// round-robin over the 8 XCDs, so workgroup L runs item L / 8 of XCD L mod 8.
+template <int PREC>
+__global__ __launch_bounds__(kThreads, 1) void k_scan_mfma(Waveform w, const TmplDev *__restrict__ tmpl,
+                                                           const int32_t *__restrict__ group_begin,
+                                                           const Frag *__restrict__ afrag, int mode,
+                                                           int block_stride, int groups, uint32_t nblocks,
+                                                           const float2 *__restrict__ rb,
+                                                           const float4 *__restrict__ binfo, int rb_len,
+                                                           SurvOut so, CandOut co)
+{
+    constexpr int kLds = PREC != TMX_PREC_FP16 ? kTileBytesBf16 : kTileBytesF16;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kLds];
+    uint32_t bidx;
+    int g;
+    block_of_item(blockIdx.x & 7u, blockIdx.x >> 3, groups, &bidx, &g);
+    scan_item<PREC>(lds, w, tmpl, group_begin, afrag, mode, block_stride, nblocks, rb, binfo, rb_len, so, co, bidx,
+                    g);
+}
+
+// Persistent form: one workgroup per CU, each claiming items from its XCD's
+// queue (queue[x] counts the items of XCD x claimed so far; items beyond
+// `items` mean that queue is empty) and, once that is empty, from the other
+// XCDs' queues in turn, until every queue is empty.  Every workgroup ends
+// when all eight counters have passed `items`.
+template <int PREC>
+__global__ __launch_bounds__(kThreads, 1) void k_scan_mfma_queue(Waveform w, const TmplDev *__restrict__ tmpl,
+                                                                 const int32_t *__restrict__ group_begin,
+                                                                 const Frag *__restrict__ afrag, int mode,
+                                                                 int block_stride, int groups, uint32_t nblocks,
+                                                                 const float2 *__restrict__ rb,
+                                                                 const float4 *__restrict__ binfo, int rb_len,
+                                                                 SurvOut so, CandOut co, uint32_t *queue,
+                                                                 uint32_t items)
+{
+    constexpr int kLds = PREC != TMX_PREC_FP16 ? kTileBytesBf16 : kTileBytesF16;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kLds];
+    __shared__ uint32_t s_item[2];  // alternate slots: a slot is rewritten two barriers after it was read
+    const uint32_t xcd = blockIdx.x & 7u;
+    for (int it = 0;; it++) {
+        if (threadIdx.x == 0) {
+            uint32_t got = 0xffffffffu;
+            for (uint32_t s = 0; s < 8u; s++) {
+                const uint32_t x = (xcd + s) & 7u;
+                if (__hip_atomic_load(&queue[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= items)
+                    continue;
+                const uint32_t k8 = atomicAdd(&queue[x], 1u);
+                if (k8 < items) {
+                    got = k8 << 3 | x;
+                    break;
+                }
+            }
+            s_item[it & 1] = got;
+        }
+        // Also keeps the next item's staging from overwriting the tile while
+        // a wavefront still reads the previous item's.
+        __syncthreads();
+        const uint32_t got = s_item[it & 1];
+        if (got == 0xffffffffu)
+            return;
+        uint32_t bidx;
+        int g;
+        block_of_item(got & 7u, got >> 3, groups, &bidx, &g);
+        scan_item<PREC>(lds, w, tmpl, group_begin, afrag, mode, block_stride, nblocks, rb, binfo, rb_len, so, co,
+                        bidx, g);
+    }
+}
+
 // Level 2: the rigorous FP64 interval of every level-1 survivor; windows
 // whose lower bound reaches the template's bound become candidates.
 template <int PREC>
@@ -775,7 +850,8 @@ hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, int max_kpad, const 
 
 hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, const int32_t *group_begin, int groups,
                             int precision, const void *a_frag, const float2 *rb, const float4 *binfo, int rb_len,
-                            int mode, int block_stride, const SurvOut &so, const CandOut &co, hipStream_t stream)
+                            int mode, int block_stride, const SurvOut &so, const CandOut &co, int persistent,
+                            uint32_t *queue, hipStream_t stream)
 {
     (void)count;
     if (groups <= 0 || w.n == 0)
@@ -786,8 +862,26 @@ hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, c
     const uint64_t total = per_xcd * 8 * static_cast<uint64_t>(groups);
     if (total >= (1ull << 32))
         return hipErrorInvalidConfiguration;
-    const dim3 grid(static_cast<unsigned>(total));
     const Frag *a = static_cast<const Frag *>(a_frag);
+    if (persistent > 0) {
+        // One workgroup per CU (a multiple of 8, so every XCD has its share);
+        // the queues start empty of claims.
+        if (!queue)
+            return hipErrorInvalidValue;
+        const unsigned wgs = static_cast<unsigned>((persistent + 7) / 8 * 8);
+        const uint32_t items = static_cast<uint32_t>(per_xcd * static_cast<uint64_t>(groups));
+        hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(8), 0, stream, queue);
+        if (precision == TMX_PREC_FP16)
+            hipLaunchKernelGGL(k_scan_mfma_queue<TMX_PREC_FP16>, dim3(wgs), dim3(kThreads), 0, stream, w, tmpl,
+                               group_begin, a, mode, block_stride, groups, static_cast<uint32_t>(launched), rb, binfo,
+                               rb_len, so, co, queue, items);
+        else
+            hipLaunchKernelGGL(k_scan_mfma_queue<TMX_PREC_BF16X3>, dim3(wgs), dim3(kThreads), 0, stream, w, tmpl,
+                               group_begin, a, mode, block_stride, groups, static_cast<uint32_t>(launched), rb, binfo,
+                               rb_len, so, co, queue, items);
+        return hipGetLastError();
+    }
+    const dim3 grid(static_cast<unsigned>(total));
     if (precision == TMX_PREC_FP16)
         hipLaunchKernelGGL(k_scan_mfma<TMX_PREC_FP16>, grid, dim3(kThreads), 0, stream, w, tmpl, group_begin, a, mode,
                            block_stride, groups, static_cast<uint32_t>(launched), rb, binfo, rb_len, so, co);

@@ -46,9 +46,12 @@ hipError_t launch_knn_select(const uint64_t *keys, const uint32_t *offsets, cons
 // mfma.hip
 hipError_t launch_mfma_pack(const TmplDev *tmpl, int count, int max_kpad, const float *qz, int precision,
                             void *a_frag, hipStream_t stream);
+// persistent > 0: that many workgroups claim the items from per-XCD queues
+// (queue: 8 counters); else one workgroup per item.
 hipError_t launch_scan_mfma(const Waveform &w, const TmplDev *tmpl, int count, const int32_t *group_begin, int groups,
                             int precision, const void *a_frag, const float2 *rb, const float4 *binfo, int rb_len,
-                            int mode, int block_stride, const SurvOut &so, const CandOut &co, hipStream_t stream);
+                            int mode, int block_stride, const SurvOut &so, const CandOut &co, int persistent,
+                            uint32_t *queue, hipStream_t stream);
 hipError_t launch_mfma_consts(TmplDev *tmpl, int count, int precision, hipStream_t stream);
 hipError_t launch_mfma_level2(const Waveform &w, const TmplDev *tmpl, int precision, const float4 *binfo,
                               const SurvOut &so, const CandOut &co, int cus, hipStream_t stream);
@@ -123,6 +126,7 @@ struct CallPlan {
     std::vector<int> perm;          // device index -> query index
     std::vector<TmplDev> tmpl;      // device order
     std::vector<int32_t> group_begin;
+    int launch = TMX_LAUNCH_GRID;   // matrix-core launch form (tmx_launch)
     int64_t raw_len = 0, order_len = 0;
     int64_t afrag_len = 0;          // 16-byte fragments
     int64_t comparisons = 0;
@@ -923,20 +927,29 @@ int run_fft_pass(tmx_ctx *ctx, Lane &L, const CallPlan &plan, const CallPlan::Ff
     return 0;
 }
 
-// ---- autotuner: templates per matrix-core workgroup -------------------------
+// ---- autotuner: templates per matrix-core workgroup and launch form -------
 //
 // The group size trades workgroups (a larger grid) against the per-workgroup
 // work shared by a group (the block's largest |x~|, its statistics reads and
-// staging).  The first batch of a shape - precision, matrix-core template
-// count and mean padded length, each in powers of two - times every
-// candidate on a strided sample of blocks of the main pass, with the batch's
-// real bounds (after seeding), and the context keeps the fastest for that
-// shape.  Answers do not depend on the group size.
+// staging); the launch form is one workgroup per (block, group) item, or one
+// persistent workgroup per CU claiming items from per-XCD queues (with
+// stealing across XCDs at the end).  The first batch of a shape - precision,
+// matrix-core template count and mean padded length, each in powers of two -
+// times every (size, form) candidate on a strided sample of blocks of the
+// main pass, with the batch's real bounds (after seeding), and the context
+// keeps the fastest for that shape.  Answers depend on neither.
 
 uint64_t waveform_blocks(tmx_ctx *ctx, const tmx_cfg &cfg);
 
 constexpr int kTuneGroups[] = {4, 8, 16};
+constexpr int kTuneLaunch[] = {TMX_LAUNCH_GRID, TMX_LAUNCH_QUEUE};
 constexpr uint64_t kTuneBlocks = 512;  // blocks per timed sample (every S-th block)
+
+// Workgroups of the persistent form for this plan (0: one per item).
+int persistent_wgs(const Lane &L, const CallPlan &plan)
+{
+    return plan.launch == TMX_LAUNCH_QUEUE ? (L.cus > 0 ? L.cus : 256) : 0;
+}
 
 uint32_t tune_key(const tmx_cfg &cfg, const CallPlan &plan)
 {
@@ -963,11 +976,16 @@ int upload_groups(Lane &L, const CallPlan &plan)
 
 int autotune_groups(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
 {
+    LANE_HIP(L.queue.reserve(8 * sizeof(uint32_t)), "hipMalloc work queues");
     if (cfg.precision == TMX_PREC_FP32) {
         st.tuned_group = 1;
+        st.tuned_launch = TMX_LAUNCH_GRID;
         return 0;
     }
+    // A launch form asked for, else the grid unless tuned below.
+    plan.launch = cfg.mfma_launch == TMX_LAUNCH_QUEUE ? TMX_LAUNCH_QUEUE : TMX_LAUNCH_GRID;
     st.tuned_group = mfma_group_size();
+    st.tuned_launch = plan.launch;
     int inner = 0;
     for (int d = 0; d < plan.Tg; d++)
         inner += plan.tmpl[d].ea ? 0 : 1;
@@ -978,16 +996,19 @@ int autotune_groups(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, t
             set_groups(plan, g);
             return upload_groups(L, plan);
         }
-    // Up to the smallest candidate every size gives one group: nothing to choose.
+    // Up to the smallest candidate every size gives one group: only the
+    // launch form could differ, and one group leaves little to balance.
     if (!cfg.autotune || inner <= kTuneGroups[0])
         return 0;
     const uint32_t key = tune_key(cfg, plan);
-    int G = 0;
+    int G = 0, F = 0;
     {
         std::lock_guard<std::mutex> lk(ctx->tune_mu);
         const auto it = ctx->tuned_group.find(key);
-        if (it != ctx->tuned_group.end())
-            G = it->second;
+        if (it != ctx->tuned_group.end()) {
+            G = it->second & 0xff;
+            F = it->second >> 8;
+        }
     }
     if (G == 0) {
         const double t0 = now_s();
@@ -1000,29 +1021,39 @@ int autotune_groups(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, t
             if (int rc = upload_groups(L, plan))
                 return rc;
             const int groups = static_cast<int>(plan.group_begin.size()) - 1;
-            LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
-            if (int rc = reserve_surv_queue(ctx, L))
-                return rc;
-            LANE_HIP(launch_mfma_consts(L.tmpl.as<TmplDev>(), plan.Tg, cfg.precision, L.stream), "level-1 constants");
-            LANE_HIP(hipEventRecord(L.ev0, L.stream), "event");
-            LANE_HIP(launch_scan_mfma(w, L.tmpl.as<TmplDev>(), plan.T, L.groups.as<int32_t>(), groups, cfg.precision,
-                                      L.afrag.p, ctx->rb.as<float2>(), ctx->binfo.as<float4>(), ctx->rb_len,
-                                      kModeMain, S, surv_out(L), cand_out(L), L.stream),
-                     "MFMA scan (tuning)");
-            LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
-            LANE_HIP(hipEventSynchronize(L.ev1), "tuning");
-            float ms = 0.0f;
-            LANE_HIP(hipEventElapsedTime(&ms, L.ev0, L.ev1), "event time");
-            if (G == 0 || ms < best) {
-                best = ms;
-                G = g;
+            for (const int f : kTuneLaunch) {
+                if (cfg.mfma_launch != TMX_LAUNCH_AUTO && f != cfg.mfma_launch)
+                    continue;
+                plan.launch = f;
+                LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
+                if (int rc = reserve_surv_queue(ctx, L))
+                    return rc;
+                LANE_HIP(launch_mfma_consts(L.tmpl.as<TmplDev>(), plan.Tg, cfg.precision, L.stream),
+                         "level-1 constants");
+                LANE_HIP(hipEventRecord(L.ev0, L.stream), "event");
+                LANE_HIP(launch_scan_mfma(w, L.tmpl.as<TmplDev>(), plan.T, L.groups.as<int32_t>(), groups,
+                                          cfg.precision, L.afrag.p, ctx->rb.as<float2>(), ctx->binfo.as<float4>(),
+                                          ctx->rb_len, kModeMain, S, surv_out(L), cand_out(L),
+                                          persistent_wgs(L, plan), L.queue.as<uint32_t>(), L.stream),
+                         "MFMA scan (tuning)");
+                LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
+                LANE_HIP(hipEventSynchronize(L.ev1), "tuning");
+                float ms = 0.0f;
+                LANE_HIP(hipEventElapsedTime(&ms, L.ev0, L.ev1), "event time");
+                if (G == 0 || ms < best) {
+                    best = ms;
+                    G = g;
+                    F = f;
+                }
             }
         }
         std::lock_guard<std::mutex> lk(ctx->tune_mu);
-        ctx->tuned_group[key] = G;
+        ctx->tuned_group[key] = G | F << 8;
         st.seconds_tune = now_s() - t0;
     }
     st.tuned_group = G;
+    st.tuned_launch = F;
+    plan.launch = F;
     set_groups(plan, G);
     return upload_groups(L, plan);
 }
@@ -1070,7 +1101,8 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
             LANE_HIP(launch_mfma_consts(L.tmpl.as<TmplDev>(), plan.Tg, cfg.precision, L.stream), "level-1 constants");
             LANE_HIP(launch_scan_mfma(w, L.tmpl.as<TmplDev>(), plan.T, L.groups.as<int32_t>(), groups, cfg.precision,
                                       L.afrag.p, ctx->rb.as<float2>(), ctx->binfo.as<float4>(), ctx->rb_len, mode,
-                                      block_stride, surv_out(L), cand_out(L), L.stream),
+                                      block_stride, surv_out(L), cand_out(L), persistent_wgs(L, plan),
+                                      L.queue.as<uint32_t>(), L.stream),
                      "MFMA scan");
             if (mode == kModeMain)
                 LANE_HIP(launch_mfma_level2(w, L.tmpl.as<TmplDev>(), cfg.precision, ctx->binfo.as<float4>(),
@@ -1533,6 +1565,7 @@ void add_stats(tmx_stats &a, const tmx_stats &b)
                     std::max(1, a.scan_templates + b.scan_templates);
     a.scan_templates += b.scan_templates;
     a.tuned_group = std::max(a.tuned_group, b.tuned_group);
+    a.tuned_launch = std::max(a.tuned_launch, b.tuned_launch);
     a.seconds_tune += b.seconds_tune;
     a.fft_lmax = std::max(a.fft_lmax, b.fft_lmax);
     a.splits += b.splits;
@@ -1639,6 +1672,7 @@ int tmx_cfg_default(tmx_cfg *cfg)
     cfg->scan_mode = TMX_SCAN_AUTO;
     cfg->verify_seq = 0;
     cfg->autotune = 1;
+    cfg->mfma_launch = TMX_LAUNCH_AUTO;
     cfg->max_candidates = 0;
     cfg->lower_bounds = 0;
     cfg->paa_segments = 16;
@@ -1657,6 +1691,7 @@ int tmx_create(tmx_ctx **out, const tmx_cfg *cfg)
         tmx_cfg_default(&ctx->cfg);
     if (ctx->cfg.precision < TMX_PREC_FP32 || ctx->cfg.precision > TMX_PREC_FP16 ||
         ctx->cfg.fft_mode < TMX_FFT_AUTO || ctx->cfg.fft_mode > TMX_FFT_ALWAYS ||
+        ctx->cfg.mfma_launch < TMX_LAUNCH_AUTO || ctx->cfg.mfma_launch > TMX_LAUNCH_QUEUE ||
         (ctx->cfg.lower_bounds && (ctx->cfg.paa_segments < 1 || ctx->cfg.paa_segments > kMaxSegments))) {
         delete ctx;
         return -1;
