@@ -68,6 +68,16 @@ enum tmx_scan_mode {
     TMX_SCAN_ALWAYS = 2  /* every template */
 };
 
+/* Whether templates take the iSAX index (TREMOR's index search: leaves in
+ * key order whose region bound passes, then the windows whose own word bound
+ * passes, all exactly verified), decided once the kNN bounds are known. */
+enum tmx_index_mode {
+    TMX_INDEX_OFF = 0,    /* never (default) */
+    TMX_INDEX_AUTO = 1,   /* templates whose index leaves at most 1% of their windows to visit (the
+                             reference's rule: leave the index when more of them pass the bound) */
+    TMX_INDEX_ALWAYS = 2  /* every eligible template (length a multiple of 16; 16 PAA segments) */
+};
+
 /* Launch form of the matrix-core pass (no counterpart in the reference: its
  * CPU scan has no launches).  Answers do not depend on it. */
 enum tmx_launch {
@@ -106,6 +116,12 @@ typedef struct {
     int32_t paa_segments;    /* PAA segments of those bounds, 1..64 (SkipSequentialScan's
                                 paa_segments); default 16 */
     int32_t mfma_launch;     /* enum tmx_launch; default TMX_LAUNCH_AUTO */
+    int32_t index_mode;      /* enum tmx_index_mode; default TMX_INDEX_OFF.  The index of a template
+                                length (16-segment iSAX words, their keys sorted on the device, leaves
+                                of at most leaf_size windows) is built on first use and kept with the
+                                words; requires paa_segments = 16 when lower_bounds is on */
+    int32_t leaf_size;       /* windows per index leaf before it splits (the reference's --leaf-size);
+                                default 2000 */
 } tmx_cfg;
 
 typedef struct {
@@ -134,6 +150,9 @@ typedef struct {
                                 candidates exceeded max_candidates (0: one pass) */
     int64_t lb_skipped;      /* windows the lower bound skipped in the early-abandoning scan */
     int32_t tuned_launch;    /* enum tmx_launch of the matrix-core pass of this call */
+    int32_t index_templates; /* templates that took the iSAX index */
+    int64_t index_windows;   /* windows in their leaves whose region bound passed (before the
+                                windows' own bounds) */
 } tmx_stats;
 
 typedef struct {
@@ -220,6 +239,19 @@ enum tmx_fft_probe_kind {
     TMX_FFT_PROBE_INVERSE = 1
 };
 int tmx_fft_probe(tmx_ctx *ctx, int32_t B, int32_t kind, const float *in, int32_t count, float *out);
+
+/* Diagnostics for tests of the iSAX index (its leaves are internal in the
+ * reference: iSAXIndex.cpp's tree): the index of template length m with
+ * leaves of at most leaf_size windows, built on first use as a search builds
+ * it (m a multiple of 16, m <= n).  Copies the W = n - m + 1 window ids in key
+ * order into ids (ids_cap >= W), and each leaf's first position in that order,
+ * window count and depth (key bits its windows share), in key order, into the
+ * leaf arrays (leaf_cap entries; the leaves beyond are not copied); *nleaves
+ * gets the leaf count.  Also copies the 16 symbols of every window into words
+ * when words is not NULL (W x 16 bytes). */
+int tmx_index_probe(tmx_ctx *ctx, int32_t m, int32_t leaf_size, uint32_t *ids, uint64_t ids_cap,
+                    uint32_t *leaf_start, uint32_t *leaf_count, int32_t *leaf_depth, int64_t leaf_cap,
+                    int64_t *nleaves, uint8_t *words);
 
 /* Waits for all device work of the context's synchronous calls. */
 int tmx_sync(tmx_ctx *ctx);

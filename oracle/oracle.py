@@ -294,6 +294,77 @@ def lower_bounds(paa_q, words, m: int) -> np.ndarray:
     return out
 
 
+# ------------------------------------------------------------ iSAX index
+
+IX_PLANES = 4      # symbol bits per segment in the key
+IX_ROOT_DEPTH = 16  # one bit per segment: the reference's 2^16 root nodes
+
+
+def isax_keys(words) -> np.ndarray:
+    """Interleaved keys of 16-segment words (uint8 [W, 16]): bit plane p (0 =
+    most significant) of segment s's symbol at key bit 63 - (16 p + s), p < 4."""
+    w = np.asarray(words, dtype=np.uint8).astype(np.uint64)
+    key = np.zeros(w.shape[0], dtype=np.uint64)
+    for p in range(IX_PLANES):
+        for s in range(16):
+            key |= ((w[:, s] >> np.uint64(7 - p)) & np.uint64(1)) << np.uint64(63 - (16 * p + s))
+    return key
+
+
+def isax_index(words, leaf_size: int):
+    """The index build restated (indexing.cpp:24-281's tree, split in turn
+    over the segments): window ids stably sorted by key, then the leaves top
+    down - the root level holds the runs of equal top-16 key bits, and a node
+    of more than leaf_size windows splits on its next key bit until 64 bits
+    are known.  Returns (ids, leaves: int64 rows (start, count, depth) in key
+    order)."""
+    keys = isax_keys(words)
+    ids = np.argsort(keys, kind="stable").astype(np.uint32)
+    sk = keys[ids]
+    top = sk >> np.uint64(64 - IX_ROOT_DEPTH)
+    starts = np.flatnonzero(np.r_[True, top[1:] != top[:-1]]) if sk.size else np.zeros(0, dtype=np.int64)
+    ends = np.r_[starts[1:], sk.size]
+    stack = [(int(a), int(b - a), IX_ROOT_DEPTH) for a, b in zip(starts, ends)]
+    leaves = []
+    while stack:
+        start, count, depth = stack.pop()
+        if not (depth < 64 and count > leaf_size):
+            leaves.append((start, count, depth))
+            continue
+        bit = np.uint64(1) << np.uint64(63 - depth)
+        lo = start + int(np.searchsorted((sk[start:start + count] & bit) != 0, True))
+        if lo > start:
+            stack.append((start, lo - start, depth + 1))
+        if lo < start + count:
+            stack.append((lo, start + count - lo, depth + 1))
+    leaves.sort()
+    return ids, np.array(leaves, dtype=np.int64).reshape(-1, 3)
+
+
+def isax_leaf_bound(paa_q, key: int, depth: int, m: int) -> np.float32:
+    """minidist_paa_to_isax (SAX.cpp:112-170) of a leaf: segment s known to
+    b = ceil((depth - s) / 16) bits (at most 4) spans the symbols [v 2^(8-b),
+    (v + 1) 2^(8-b) - 1] of its top bits v; the PAA value's distance from
+    their breakpoint range (MINVAL / MAXVAL = -/+2e6 at the ends), squared in
+    FP32 and summed in segment order, times m / 16."""
+    bp = sax_breakpoints()
+    f = np.float32
+    dist = f(0.0)
+    for s in range(16):
+        b = min(IX_PLANES, (depth - s + 15) // 16) if depth > s else 0
+        v = 0
+        for p in range(b):
+            v = (v << 1) | ((int(key) >> (63 - (16 * p + s))) & 1)
+        lo_sym = v << (8 - b)
+        hi_sym = lo_sym + (1 << (8 - b)) - 1
+        lo = f(-2000000.0) if lo_sym == 0 else bp[lo_sym - 1]
+        hi = f(2000000.0) if hi_sym == 255 else bp[hi_sym]
+        q = f(paa_q[s])
+        d = f(lo - q) if lo > q else (f(hi - q) if hi < q else f(0.0))
+        dist = f(dist + f(d * d))
+    return f(f(f(m) / f(16.0)) * dist)
+
+
 # ------------------------------------------------------------- FP64 oracle
 
 def pearson_fp64(raw, template) -> np.ndarray:

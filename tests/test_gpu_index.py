@@ -1,0 +1,115 @@
+"""GPU parity of the iSAX index (TREMOR's index: indexing.cpp:24-281 build,
+Tremor.cpp:1195-1518 leaf-order search, SAX.cpp:112-170 node mindist).
+
+Bar: the device build equals the oracle's restatement exactly (the words, the
+window ids in key order, every leaf's start, count and depth), and searches
+through the index return the oracle scan's answers bit for bit - kNN and
+threshold, merge 0 and > 0, missing samples, lengths the index cannot take
+beside ones it does - with the reference's 1% rule choosing the index where
+its leaves prune.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MISSING = np.float32(-100000.0)
+
+
+def _log(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _wave(n, seed, events=30):
+    from tremor_amd import synth
+    rng = np.random.default_rng(seed)
+    x = synth.bandpassed_noise(n, rng)
+    onsets, _ = synth.inject_events(x, events, rng)
+    return x, onsets, rng
+
+
+@pytest.mark.parametrize("m,leaf_size", [(64, 300), (256, 2000)])
+def test_index_build_equals_oracle(oracle, m, leaf_size):
+    from tremor_amd.capi import Context
+    x, _, _ = _wave(1 << 17, 71)
+    x[5000:5100] = MISSING
+    _log("index build", m=m, leaf_size=leaf_size)
+    with Context() as ctx:
+        ctx.load_waveform(x)
+        ids, leaves, words = ctx.index_probe(m, leaf_size)
+    want_words = oracle.sax_words(x, m, 16)
+    assert np.array_equal(words, want_words)
+    want_ids, want_leaves = oracle.isax_index(want_words, leaf_size)
+    assert np.array_equal(ids, want_ids)
+    assert np.array_equal(leaves, want_leaves)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16"])
+def test_index_answers_equal_scan(oracle, prec):
+    """Every eligible template (lengths that are multiples of 16) through the
+    index, the others through the planned forms: the oracle's answers."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(1 << 18, 73)
+    x[70000:70400] = MISSING
+    lengths = [64, 96, 100, 128, 250, 256, 512]
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    eligible = sum(1 for m in lengths if m % 16 == 0)
+    with Context(precision=prec, index="always", leaf_size=500) as ctx:
+        ctx.load_waveform(x)
+        for k, merge in ((1, 0), (10, 16)):
+            _log("index knn", prec=prec, k=k, merge=merge)
+            pos, d, st = ctx.knn(tm, k, merge)
+            assert st["index_templates"] == eligible
+            po, do, _ = oracle.scan_knn(x, tm, k, merge)
+            assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
+        for rho, merge in ((0.8, 0), (0.6, 8)):
+            _log("index threshold", prec=prec, rho=rho, merge=merge)
+            r = ctx.threshold(tm, rho, merge)
+            assert r.stats["index_templates"] == eligible
+            tt, tp, td = oracle.scan_threshold(x, tm, rho, merge)
+            assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+            assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_index_auto_takes_pruning_templates(oracle):
+    """k = 1 on templates cut from events: the seed bound is tight, the
+    leaves prune, and the 1% rule sends the templates to the index; at a low
+    threshold nothing prunes and the rule keeps them on the planned forms."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(1 << 20, 79, events=40)
+    tm = synth.cut_templates(x, onsets, [128, 256, 512, 1024], rng)
+    W = [x.size - len(t) + 1 for t in tm]
+    with Context(index="auto", leaf_size=2000) as ctx:
+        ctx.load_waveform(x)
+        _log("index auto knn")
+        pos, d, st = ctx.knn(tm, 1, 0)
+        print("  stats", {k: st[k] for k in ("index_templates", "index_windows", "candidates")}, flush=True)
+        assert st["index_templates"] >= 1
+        assert st["index_windows"] <= 0.01 * sum(W)
+        po, do, _ = oracle.scan_knn(x, tm, 1, 0)
+        assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
+        _log("index auto low threshold")
+        r = ctx.threshold(tm, 0.2, 0)
+        assert r.stats["index_templates"] == 0
+        tt, tp, td = oracle.scan_threshold(x, tm, 0.2, 0)
+        assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+        assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_index_rebuilt_for_another_leaf_size(oracle):
+    """Contexts with different leaf sizes over the same waveform build their
+    own trees; the answers do not change."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(1 << 17, 83)
+    tm = synth.cut_templates(x, onsets, [256, 256, 320], rng)
+    po, do, _ = oracle.scan_knn(x, tm, 5, 0)
+    for leaf in (16, 2000, 100000):
+        with Context(index="always", leaf_size=leaf) as ctx:
+            ctx.load_waveform(x)
+            _log("index leaf size", leaf=leaf)
+            pos, d, st = ctx.knn(tm, 5, 0)
+            assert st["index_templates"] == 3
+            assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))

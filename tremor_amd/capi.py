@@ -26,7 +26,7 @@ EXPORTS = [
     "tmx_load_waveform",
     "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch",
     "tmx_distance_batch_noalloc", "tmx_window_stats",
-    "tmx_prepare_template", "tmx_fft_probe", "tmx_sync",
+    "tmx_prepare_template", "tmx_fft_probe", "tmx_index_probe", "tmx_sync",
     "tmx_server_cfg_default", "tmx_server_start", "tmx_server_stop", "tmx_submit", "tmx_poll", "tmx_wait",
     "tmx_result_free",
 ]
@@ -40,7 +40,8 @@ class Cfg(ctypes.Structure):
                 ("seed_target", ctypes.c_int32), ("verbose", ctypes.c_int32), ("fft_mode", ctypes.c_int32),
                 ("scan_mode", ctypes.c_int32), ("verify_seq", ctypes.c_int32), ("autotune", ctypes.c_int32),
                 ("max_candidates", ctypes.c_int64), ("lower_bounds", ctypes.c_int32),
-                ("paa_segments", ctypes.c_int32), ("mfma_launch", ctypes.c_int32)]
+                ("paa_segments", ctypes.c_int32), ("mfma_launch", ctypes.c_int32), ("index_mode", ctypes.c_int32),
+                ("leaf_size", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -52,7 +53,8 @@ class Stats(ctypes.Structure):
                 ("fft_templates", ctypes.c_int32), ("fft_block", ctypes.c_int32), ("verified", ctypes.c_int64),
                 ("scan_templates", ctypes.c_int32), ("scan_blocks", ctypes.c_double),
                 ("tuned_group", ctypes.c_int32), ("seconds_tune", ctypes.c_double), ("fft_lmax", ctypes.c_int32),
-                ("splits", ctypes.c_int32), ("lb_skipped", ctypes.c_int64), ("tuned_launch", ctypes.c_int32)]
+                ("splits", ctypes.c_int32), ("lb_skipped", ctypes.c_int64), ("tuned_launch", ctypes.c_int32),
+                ("index_templates", ctypes.c_int32), ("index_windows", ctypes.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -116,6 +118,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.tmx_window_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64, _f32p, _f32p]
     L.tmx_prepare_template.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int32, _f32p, _i32p]
     L.tmx_fft_probe.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _f32p, ctypes.c_int32, _f32p]
+    _u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.tmx_index_probe.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _u32p, ctypes.c_uint64, _u32p, _u32p,
+                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.POINTER(ctypes.c_uint8)]
     L.tmx_sync.argtypes = [ctypes.c_void_p]
     L.tmx_server_cfg_default.argtypes = [ctypes.POINTER(ServerCfg)]
     L.tmx_server_start.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerCfg)]
@@ -191,7 +197,8 @@ class Context:
     def __init__(self, device: int = 0, precision: str | int = "bf16x3", cand_capacity: int = 0,
                  seed_target: int = 0, verbose: bool = False, fft: str = "auto", scan: str = "auto",
                  verify_seq: bool = False, autotune: bool | int = True, max_candidates: int = 0,
-                 lower_bounds: bool = False, paa_segments: int = 16, launch: str = "auto"):
+                 lower_bounds: bool = False, paa_segments: int = 16, launch: str = "auto",
+                 index: str = "off", leaf_size: int = 2000):
         L = lib()
         cfg = Cfg()
         L.tmx_cfg_default(ctypes.byref(cfg))
@@ -210,6 +217,8 @@ class Context:
         cfg.lower_bounds = 1 if lower_bounds else 0
         cfg.paa_segments = int(paa_segments)
         cfg.mfma_launch = {"auto": 0, "grid": 1, "queue": 2}[launch]
+        cfg.index_mode = {"off": 0, "auto": 1, "always": 2}[index]
+        cfg.leaf_size = int(leaf_size)
         h = ctypes.c_void_p()
         rc = L.tmx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:
@@ -331,6 +340,27 @@ class Context:
         self._check(self._L.tmx_fft_probe(self._h, B, k, a.ctypes.data_as(_f32p), a.shape[0],
                                           out.ctypes.data_as(_f32p)))
         return out
+
+    def index_probe(self, m: int, leaf_size: int = 2000):
+        """The iSAX index of template length m as a search builds it: (window
+        ids in key order, leaves as an int64 array of rows (start, count,
+        depth) in key order, the windows' 16-segment words)."""
+        W = self.n - m + 1
+        u32 = ctypes.POINTER(ctypes.c_uint32)
+        nl = ctypes.c_int64(0)
+        ids = np.zeros(max(W, 1), dtype=np.uint32)
+        words = np.zeros((max(W, 1), 16), dtype=np.uint8)
+        self._check(self._L.tmx_index_probe(self._h, m, leaf_size, ids.ctypes.data_as(u32), ids.size, None, None,
+                                            None, 0, ctypes.byref(nl), words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        n = nl.value
+        st = np.zeros(max(n, 1), dtype=np.uint32)
+        ct = np.zeros(max(n, 1), dtype=np.uint32)
+        dp = np.zeros(max(n, 1), dtype=np.int32)
+        self._check(self._L.tmx_index_probe(self._h, m, leaf_size, None, 0, st.ctypes.data_as(u32),
+                                            ct.ctypes.data_as(u32), dp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            n, ctypes.byref(nl), None))
+        leaves = np.stack([st[:n], ct[:n], dp[:n]], axis=1).astype(np.int64)
+        return ids[:W], leaves, words[:W]
 
     def sync(self):
         self._check(self._L.tmx_sync(self._h))

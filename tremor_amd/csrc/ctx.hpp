@@ -97,6 +97,7 @@ struct Lane {
     // matrix-core form: queue of level-1 survivors for the rigorous second level
     DevMem surv, surv_n;
     DevMem queue;  // the persistent matrix-core launch's per-XCD work counters
+    DevMem ix_ids, ix_passed;  // templates on the iSAX index; windows of their passing leaves
     uint64_t surv_cap = 0;
     uint64_t surv_need = 0;  // > surv_cap after a main pass that overflowed the queue
     uint64_t last_survivors = 0;  // level-1 survivors of the last main pass
@@ -187,11 +188,26 @@ struct FftShape {
     int B, Lmax;
 };
 
-// iSAX words of every window of one length (skip-sequential lower bound).
+// The iSAX index of one length (index.hip): window ids and keys in key order
+// and the leaves.
+struct IxIndex {
+    DevMem keys;    // uint64 [W], sorted
+    DevMem ids;     // uint32 [W], the windows in key order
+    DevMem leaves;  // IxLeaf [nleaves], in key order
+    int64_t nleaves = 0;
+    int leaf_size = 0;
+};
+
+// iSAX words of every window of one length (skip-sequential lower bound and
+// the index), and the index once built.
 struct SaxWords {
     int m = 0, segments = 0;
     DevMem words;  // uint8 [n - m + 1][segments]
+    IxIndex ix;
 };
+
+// Builds the index of W windows from their 16-segment words.
+hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxIndex &ix, hipStream_t stream);
 
 // Piece spectra of the resident waveform for one shape (B, Lmax).
 struct FftClass {

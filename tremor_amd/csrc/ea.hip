@@ -236,11 +236,11 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
     }
 }
 
-__global__ void k_mark_ea(TmplDev *__restrict__ tmpl, const int32_t *__restrict__ ids, int count)
+__global__ void k_mark_ea(TmplDev *__restrict__ tmpl, const int32_t *__restrict__ ids, int count, int form)
 {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i < count)
-        tmpl[ids[i]].ea = 1;
+        tmpl[ids[i]].ea = form;
 }
 
 }  // namespace
@@ -268,12 +268,13 @@ hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const f
     return hipGetLastError();
 }
 
-hipError_t launch_mark_ea(TmplDev *tmpl, const int32_t *ids, int count, hipStream_t stream)
+// form: 1 the early-abandoning scan, 2 the iSAX index.
+hipError_t launch_mark_ea(TmplDev *tmpl, const int32_t *ids, int count, int form, hipStream_t stream)
 {
     if (count <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_mark_ea, dim3((count + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, tmpl, ids,
-                       count);
+                       count, form);
     return hipGetLastError();
 }
 

@@ -1,0 +1,271 @@
+// The iSAX index on the device (TREMOR's index, indexing.cpp:24-281 build;
+// Tremor.cpp:1195-1518 leaf-order search with node mindist and the record
+// pass): windows sorted by an interleaved iSAX key, cut top-down into leaves
+// of at most `leaf_size` windows, and a search that visits only the leaves
+// whose region's lower bound does not exceed a template's bound, then only
+// their windows whose own word's bound does not.
+//
+// Key of a window (16 PAA segments, symbols at cardinality 256): bit plane p
+// (0 = most significant) of segment s's symbol at key bit 63 - (16 p + s), for
+// p < 4.  A node at depth d holds the windows whose keys share the top d bits:
+// segment s is then known to ceil((d - s) / 16) bits (clamped to [0, 4]).  The
+// root level is depth 16 (one bit per segment, the reference's 2^16 root
+// nodes); a node splits on its next key bit while it holds more than
+// leaf_size windows (the reference's informed split picks the segment by the
+// data; here the segments split in turn, which keeps the tree a pure function
+// of the keys).  A node that cannot split (64 bits known) stays a leaf.
+#include <hipcub/hipcub.hpp>
+
+#include "ctx.hpp"
+#include "isax.hpp"
+
+namespace tmx {
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ __launch_bounds__(kThreads) void k_ix_keys(const uint8_t *__restrict__ words, uint64_t W,
+                                                      uint64_t *__restrict__ keys, uint32_t *__restrict__ ids)
+{
+    const uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (j >= W)
+        return;
+    const uint4 wv = *reinterpret_cast<const uint4 *>(words + 16 * j);
+    const uint32_t wd[4] = {wv.x, wv.y, wv.z, wv.w};
+    uint64_t key = 0;
+#pragma unroll
+    for (int p = 0; p < kIxPlanes; p++)
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const uint32_t sym = (wd[s >> 2] >> (8 * (s & 3))) & 0xffu;
+            key |= static_cast<uint64_t>((sym >> (7 - p)) & 1u) << (63 - (16 * p + s));
+        }
+    keys[j] = key;
+    ids[j] = static_cast<uint32_t>(j);
+}
+
+// One level of the top-down split: node i covers sorted keys [start, start +
+// count) sharing their top `depth` bits.  A leaf is appended to the leaf list;
+// a node that splits appends its non-empty children (the first index whose
+// next bit is set found by binary search: within a node that bit is
+// monotone in the sorted order).
+// counters: [0] next nodes, [1] leaves, [2] set when a list would exceed its
+// capacity (the entry is dropped and the host fails the build).
+__global__ __launch_bounds__(kThreads) void k_ix_split(const uint64_t *__restrict__ keys,
+                                                       const uint4 *__restrict__ nodes, uint32_t nn,
+                                                       uint4 *__restrict__ next, uint32_t next_cap,
+                                                       uint4 *__restrict__ leaves, uint32_t leaf_cap,
+                                                       uint32_t *__restrict__ counters, uint32_t leaf_size)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= nn)
+        return;
+    const uint4 nd = nodes[i];
+    const uint32_t start = nd.x, count = nd.y, depth = nd.z;
+    const auto put = [&](uint4 *list, uint32_t cap, uint32_t *n, uint4 v) {
+        const uint32_t slot = atomicAdd(n, 1u);
+        if (slot < cap)
+            list[slot] = v;
+        else
+            atomicOr(counters + 2, 1u);
+    };
+    if (!(depth < 64u && (depth < kIxRootDepth || count > leaf_size))) {
+        put(leaves, leaf_cap, counters + 1, nd);
+        return;
+    }
+    const uint64_t bit = 1ull << (63 - depth);
+    uint32_t lo = start, hi = start + count;  // first index with the bit set
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (keys[mid] & bit)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    if (lo > start)
+        put(next, next_cap, counters, make_uint4(start, lo - start, depth + 1, 0));
+    if (lo < start + count)
+        put(next, next_cap, counters, make_uint4(lo, start + count - lo, depth + 1, 0));
+}
+
+__global__ __launch_bounds__(kThreads) void k_ix_leaf_keys(const uint4 *__restrict__ leaves, uint32_t n,
+                                                           uint32_t *__restrict__ order_key,
+                                                           uint32_t *__restrict__ order_val)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) {
+        order_key[i] = leaves[i].x;
+        order_val[i] = i;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_ix_leaf_gather(const uint4 *__restrict__ leaves, uint32_t n,
+                                                             const uint32_t *__restrict__ order,
+                                                             const uint64_t *__restrict__ keys,
+                                                             IxLeaf *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) {
+        const uint4 nd = leaves[order[i]];
+        out[i] = IxLeaf{keys[nd.x], nd.x, nd.y, static_cast<int32_t>(nd.z), 0};
+    }
+}
+
+// Templates ids[blockIdx.y]: per leaf (one per thread), its bound against the
+// template's bound; the windows of the leaves that pass are counted
+// (fraction mode), or, main mode, each of their windows whose own word's bound
+// passes too and whose first sample is present becomes a candidate.
+template <bool EMIT>
+__global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDev *__restrict__ tmpl,
+                                                        const int32_t *__restrict__ ids,
+                                                        const float *__restrict__ bp_g, const float *__restrict__ paa_g,
+                                                        unsigned long long *__restrict__ passed, CandOut co)
+{
+    __shared__ float bp[256];
+    __shared__ float paa[16];
+    __shared__ uint32_t s_leaf[kThreads];
+    __shared__ uint32_t s_n;
+    const int tid = threadIdx.x;
+    const int d = ids[blockIdx.y];
+    const TmplDev &t = tmpl[d];
+    for (int i = tid; i < 255; i += kThreads)
+        bp[i] = bp_g[i];
+    if (tid < 16)
+        paa[tid] = paa_g[t.lb_paa + tid];
+    if (tid == 0)
+        s_n = 0;
+    __syncthreads();
+    const uint32_t leaf = blockIdx.x * kThreads + tid;
+    bool pass = false;
+    if (leaf < t.ix_nleaves) {
+        const IxLeaf L = t.ix_leaves[leaf];
+        pass = ix_pass(ix_leaf_bound(bp, paa, L.key, L.depth, t.m), t.bound);
+        if (pass && !EMIT)
+            atomicAdd(passed + d, static_cast<unsigned long long>(L.count));
+    }
+    if (!EMIT)
+        return;
+    if (pass)
+        s_leaf[atomicAdd(&s_n, 1u)] = leaf;
+    __syncthreads();
+    const uint32_t n = s_n;
+    const uint64_t W = w.n - static_cast<uint64_t>(t.m) + 1;
+    for (uint32_t q = 0; q < n; q++) {
+        const IxLeaf L = t.ix_leaves[s_leaf[q]];
+        for (uint32_t e = tid; e < L.count; e += kThreads) {
+            const uint32_t j = t.ix_ids[L.start + e];
+            if (j >= W || w.x[j] == kMissing)
+                continue;
+            if (!ix_pass(sax_lower_bound(bp, paa, t.lb_words + 16ull * j, t.m, 16), t.bound))
+                continue;
+            cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(d), j, NAN, -INFINITY, INFINITY);
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxIndex &ix, hipStream_t stream)
+{
+    if (W == 0 || W > 0x7fffffffull || leaf_size < 1)
+        return hipErrorInvalidValue;
+    const int n = static_cast<int>(W);
+    hipError_t e;
+    // Keys and ids, then sorted by key (ids along).
+    DevMem keys_in, ids_in, tmp;
+    if ((e = keys_in.reserve(8 * W)) || (e = ids_in.reserve(4 * W)) || (e = ix.keys.reserve(8 * W)) ||
+        (e = ix.ids.reserve(4 * W)))
+        return e;
+    const unsigned grid = static_cast<unsigned>((W + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_ix_keys, dim3(grid), dim3(kThreads), 0, stream, words, W, keys_in.as<uint64_t>(),
+                       ids_in.as<uint32_t>());
+    size_t bytes = 0;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in.as<uint64_t>(), ix.keys.as<uint64_t>(),
+                                                ids_in.as<uint32_t>(), ix.ids.as<uint32_t>(), n, 0, 64, stream)))
+        return e;
+    if ((e = tmp.reserve(bytes)))
+        return e;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, keys_in.as<uint64_t>(), ix.keys.as<uint64_t>(),
+                                                ids_in.as<uint32_t>(), ix.ids.as<uint32_t>(), n, 0, 64, stream)))
+        return e;
+    // Top-down split, one level per launch, from the root node (depth 0).
+    // The nodes of one level are disjoint and non-empty: at most 2^16 of them
+    // down to the root depth, then at most twice the nodes over leaf_size
+    // windows above; the leaves, at depth >= 16, at most 2^16 plus that many
+    // per depth.  Capacities from these counts (never above W), checked by the
+    // kernel.
+    const uint64_t per_level = std::min<uint64_t>(W, std::max<uint64_t>(1ull << 16, 2 * W / leaf_size)) + 1;
+    const uint64_t leaf_cap = std::min<uint64_t>(W, (1ull << 16) + 96 * (W / leaf_size) + 1) + 1;
+    DevMem nodes_a, nodes_b, leaves, counters;
+    if ((e = nodes_a.reserve(16 * per_level)) || (e = nodes_b.reserve(16 * per_level)) ||
+        (e = leaves.reserve(16 * leaf_cap)) || (e = counters.reserve(16)))
+        return e;
+    uint4 *cur = nodes_a.as<uint4>(), *nxt = nodes_b.as<uint4>();
+    const uint4 root = make_uint4(0u, static_cast<uint32_t>(W), 0u, 0u);
+    if ((e = hipMemcpyAsync(cur, &root, sizeof(root), hipMemcpyHostToDevice, stream)) ||
+        (e = hipMemsetAsync(counters.p, 0, 12, stream)))
+        return e;
+    uint32_t nn = 1, cnt[3] = {0, 0, 0};
+    for (int level = 0; nn > 0; level++) {
+        if (level > 64)
+            return hipErrorUnknown;  // a node's depth grows by one per level: at most 65 levels
+        if ((e = hipMemsetAsync(counters.as<uint32_t>(), 0, 4, stream)))
+            return e;
+        hipLaunchKernelGGL(k_ix_split, dim3((nn + kThreads - 1) / kThreads), dim3(kThreads), 0, stream,
+                           ix.keys.as<uint64_t>(), cur, nn, nxt, static_cast<uint32_t>(per_level), leaves.as<uint4>(),
+                           static_cast<uint32_t>(leaf_cap), counters.as<uint32_t>(),
+                           static_cast<uint32_t>(leaf_size));
+        if ((e = hipMemcpyAsync(cnt, counters.p, 12, hipMemcpyDeviceToHost, stream)) ||
+            (e = hipStreamSynchronize(stream)))
+            return e;
+        if (cnt[2])
+            return hipErrorUnknown;  // capacity exceeded: nothing past it was written
+        nn = cnt[0];
+        std::swap(cur, nxt);
+    }
+    const uint32_t nl = cnt[1];
+    // Leaves in key order (by their first sorted index).
+    DevMem ok_in, ov_in, ok_out, ov_out;
+    if ((e = ok_in.reserve(4 * nl)) || (e = ov_in.reserve(4 * nl)) || (e = ok_out.reserve(4 * nl)) ||
+        (e = ov_out.reserve(4 * nl)) || (e = ix.leaves.reserve(sizeof(IxLeaf) * nl)))
+        return e;
+    const unsigned lgrid = (nl + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_ix_leaf_keys, dim3(lgrid), dim3(kThreads), 0, stream, leaves.as<uint4>(), nl,
+                       ok_in.as<uint32_t>(), ov_in.as<uint32_t>());
+    bytes = 0;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ok_in.as<uint32_t>(), ok_out.as<uint32_t>(),
+                                                ov_in.as<uint32_t>(), ov_out.as<uint32_t>(), static_cast<int>(nl), 0,
+                                                32, stream)))
+        return e;
+    if ((e = tmp.reserve(bytes)))
+        return e;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, ok_in.as<uint32_t>(), ok_out.as<uint32_t>(),
+                                                ov_in.as<uint32_t>(), ov_out.as<uint32_t>(), static_cast<int>(nl), 0,
+                                                32, stream)))
+        return e;
+    hipLaunchKernelGGL(k_ix_leaf_gather, dim3(lgrid), dim3(kThreads), 0, stream, leaves.as<uint4>(), nl,
+                       ov_out.as<uint32_t>(), ix.keys.as<uint64_t>(), ix.leaves.as<IxLeaf>());
+    ix.nleaves = nl;
+    ix.leaf_size = leaf_size;
+    if ((e = hipGetLastError()))
+        return e;
+    return hipStreamSynchronize(stream);  // the scratch above is freed on return
+}
+
+hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves,
+                            const float *bp, const float *paa, unsigned long long *passed, bool emit,
+                            const CandOut &co, hipStream_t stream)
+{
+    if (count <= 0 || max_leaves <= 0)
+        return hipSuccess;
+    if (count > 65535 || max_leaves > 0x7fffffffll)
+        return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>((max_leaves + kThreads - 1) / kThreads), static_cast<unsigned>(count));
+    if (emit)
+        hipLaunchKernelGGL(k_ix_search<true>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa, passed, co);
+    else
+        hipLaunchKernelGGL(k_ix_search<false>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa, passed, co);
+    return hipGetLastError();
+}
+
+}  // namespace tmx

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kThreads) void k_sax_words(const float *__restrict_
 }
 
 // PAA of each template's z-normalised samples (paa_from_ts), one thread per
-// (template, segment).
+// (template, segment): the lower bound's segments, or 16 for the index.
 __global__ void k_tmpl_paa(const TmplDev *__restrict__ tmpl, int count, const float *__restrict__ qz,
                            float *__restrict__ paa)
 {
@@ -74,9 +74,10 @@ __global__ void k_tmpl_paa(const TmplDev *__restrict__ tmpl, int count, const fl
         return;
     const TmplDev &t = tmpl[i];
     const int s = threadIdx.x;
-    if (t.lb_seg <= 0 || s >= t.lb_seg)
+    const int seg = t.lb_seg > 0 ? t.lb_seg : (t.ix > 0 ? 16 : 0);  // the index's are 16
+    if (seg <= 0 || s >= seg)
         return;
-    const int len = t.m / t.lb_seg;
+    const int len = t.m / seg;
     const float *q = qz + t.q_off + static_cast<int64_t>(s) * len;
     float acc = 0.0f;
     for (int k = 0; k < len; k++)

@@ -88,6 +88,50 @@ __device__ inline float sax_lower_bound16_table(const float *gap, const uint8_t 
     return ((h[0] + h[2]) + (h[1] + h[3])) * ratio;
 }
 
+// ---- the iSAX index (index.hip) ------------------------------------------
+
+constexpr int kIxPlanes = 4;       // symbol bits per segment in the 64-bit key (16 segments)
+constexpr int kIxRootDepth = 16;   // the root level: one bit per segment
+
+// A leaf: the sorted windows [start, start + count) share the top `depth`
+// bits of their keys; key is its first window's.
+struct IxLeaf {
+    uint64_t key;
+    uint32_t start, count;
+    int32_t depth;
+    int32_t pad;
+};
+
+// The leaf's region bound (minidist_paa_to_isax, SAX.cpp:112-170, with the
+// segment cardinalities of the node): segment s known to b = ceil((depth - s)
+// / 16) bits (at most 4) is the symbol range [v 2^(8-b), (v + 1) 2^(8-b) - 1]
+// of its top bits v, whose breakpoints (MINVAL / MAXVAL at the ends) bound the
+// PAA value's distance; FP32 squares summed in segment order, times m / 16.
+__device__ inline float ix_leaf_bound(const float *bp, const float *paa, uint64_t key, int depth, int m)
+{
+    float dist = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const int b = depth > s ? min(kIxPlanes, (depth - s + 15) / 16) : 0;
+        uint32_t v = 0;
+        for (int p = 0; p < b; p++)
+            v = v << 1 | static_cast<uint32_t>((key >> (63 - (16 * p + s))) & 1ull);
+        const int lo_sym = static_cast<int>(v << (8 - b));
+        const int hi_sym = lo_sym + (1 << (8 - b)) - 1;
+        const float lo = lo_sym == 0 ? -2000000.0f : bp[lo_sym - 1];
+        const float hi = hi_sym == 255 ? 2000000.0f : bp[hi_sym];
+        const float d = lo > paa[s] ? lo - paa[s] : (hi < paa[s] ? hi - paa[s] : 0.0f);
+        dist += d * d;
+    }
+    return static_cast<float>(m) / 16.0f * dist;
+}
+
+// Whether a lower bound lets a window (or leaf) through against the
+// template's bound: pruned only when it exceeds the bound by more than 1e-5
+// relative (a margin over the FP32 roundings of the bound and of the exact
+// distance it bounds); NaN passes.
+__device__ inline bool ix_pass(float lb, float bound) { return !(lb * 0.99999f > bound); }
+
 // The 255 breakpoints of cardinality 256 (SAXBreakpoints.hpp, its
 // sub-table at (c - 1)(c - 2) / 2): the standard normal quantiles
 // Phi^-1(i / 256) rounded to float (host).
@@ -100,5 +144,13 @@ hipError_t launch_sax_words(const float *x, uint64_t n, int m, int segments, con
 // PAA of the templates [0, count) that carry a lower bound (lb_seg > 0)
 // into paa[tmpl.lb_paa ..].
 hipError_t launch_tmpl_paa(const TmplDev *tmpl, int count, const float *qz, float *paa, hipStream_t stream);
+
+// Templates ids[0, count) of the index: per leaf its bound against the
+// template's; emit = false adds the windows of the passing leaves to
+// passed[template]; emit = true makes each window of a passing leaf whose own
+// bound passes and whose first sample is present a candidate.
+hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves,
+                            const float *bp, const float *paa, unsigned long long *passed, bool emit,
+                            const CandOut &co, hipStream_t stream);
 
 }  // namespace tmx

@@ -98,7 +98,7 @@ hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const f
                           const int32_t *ids, int count, uint64_t max_windows, int sample, uint64_t sample_n,
                           const CandOut &co, unsigned long long *visited, const float *bp, const float *paa,
                           unsigned long long *skipped, hipStream_t stream);
-hipError_t launch_mark_ea(TmplDev *tmpl, const int32_t *ids, int count, hipStream_t stream);
+hipError_t launch_mark_ea(TmplDev *tmpl, const int32_t *ids, int count, int form, hipStream_t stream);
 hipError_t launch_rho_err(const Cand *cand, const float *est, const float *exact, uint64_t count,
                           const TmplDev *tmpl, unsigned int *out_bits, hipStream_t stream);
 }  // namespace tmx
@@ -147,6 +147,8 @@ struct CallPlan {
     int64_t out_slots = 0;          // sum of k over the kNN templates
     bool has_threshold = false;
     std::vector<int32_t> scan_ids;  // templates on the early-abandoning scan (set after seeding)
+    std::vector<int32_t> ix_ids;    // templates on the iSAX index (set after seeding)
+    int64_t ix_max_leaves = 0;      // most leaves among their indexes
     uint64_t scan_windows = 0;      // largest window count among them
     double scan_blocks = 0.0;       // their mean sampled blocks per window
 };
@@ -266,6 +268,26 @@ bool plan_has_lb(const CallPlan &plan)
     return false;
 }
 
+// A template length the iSAX index applies to: 16 PAA segments (the index's
+// keys; also the lower bound's segments when those are on), a whole number
+// of them, and fewer than 2^31 windows.
+bool ix_eligible(const tmx_cfg &cfg, int m, uint64_t n)
+{
+    return cfg.index_mode != TMX_INDEX_OFF && (!cfg.lower_bounds || cfg.paa_segments == 16) && m % 16 == 0 &&
+           static_cast<uint64_t>(m) <= n && n - static_cast<uint64_t>(m) + 1 < (1ull << 31);
+}
+
+bool plan_has_ix(const CallPlan &plan)
+{
+    for (const auto &t : plan.tmpl)
+        if (t.ix > 0)
+            return true;
+    return false;
+}
+
+// PAA segments of the words a template needs (0: none).
+int words_segments(const TmplDev &t) { return t.lb_seg > 0 ? t.lb_seg : (t.ix > 0 ? 16 : 0); }
+
 int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int count, CallPlan &plan,
                std::vector<float> &raw_packed)
 {
@@ -331,6 +353,10 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
         TmplDev &t = plan.tmpl[d];
         if (cfg.lower_bounds && lb_eligible(cfg, t.m, ctx->n)) {
             t.lb_seg = cfg.paa_segments;
+            t.lb_paa = static_cast<int64_t>(d) * kMaxSegments;
+        }
+        if (ix_eligible(cfg, t.m, ctx->n)) {
+            t.ix = 1;
             t.lb_paa = static_cast<int64_t>(d) * kMaxSegments;
         }
     }
@@ -403,17 +429,24 @@ SaxWords *find_sax(tmx_ctx *ctx, int m, int segments)
     return nullptr;
 }
 
-bool sax_ready(tmx_ctx *ctx, const CallPlan &plan)
+bool sax_ready(tmx_ctx *ctx, const tmx_cfg &cfg, const CallPlan &plan)
 {
-    for (const auto &t : plan.tmpl)
-        if (t.lb_seg > 0 && !find_sax(ctx, t.m, t.lb_seg))
+    for (const auto &t : plan.tmpl) {
+        const int seg = words_segments(t);
+        if (seg > 0 && !find_sax(ctx, t.m, seg))
             return false;
-    return !plan_has_lb(plan) || ctx->sax_bp.p;
+        if (t.ix > 0) {
+            const SaxWords *sw = find_sax(ctx, t.m, 16);
+            if (!sw || sw->ix.nleaves == 0 || sw->ix.leaf_size != cfg.leaf_size)
+                return false;
+        }
+    }
+    return !(plan_has_lb(plan) || plan_has_ix(plan)) || ctx->sax_bp.p;
 }
 
 bool resident_ready(tmx_ctx *ctx, const tmx_cfg &cfg, const CallPlan &plan)
 {
-    if (!sax_ready(ctx, plan))
+    if (!sax_ready(ctx, cfg, plan))
         return false;
     if (cfg.precision != TMX_PREC_FP32 && plan.Tg > 0 &&
         !(ctx->rb_n == ctx->n && ctx->rb_len >= block_prefix_len(plan)))
@@ -563,9 +596,9 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
 // kSaxKept lengths (the oldest are dropped first; no batch runs meanwhile).
 constexpr size_t kSaxKept = 8;
 
-int build_sax(tmx_ctx *ctx, Lane &L, const CallPlan &plan)
+int build_sax(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan)
 {
-    if (!plan_has_lb(plan))
+    if (!plan_has_lb(plan) && !plan_has_ix(plan))
         return 0;
     if (!ctx->sax_bp.p) {
         float bp[255];
@@ -574,27 +607,39 @@ int build_sax(tmx_ctx *ctx, Lane &L, const CallPlan &plan)
         LANE_HIP(hipMemcpy(ctx->sax_bp.p, bp, sizeof(bp), hipMemcpyHostToDevice), "copy breakpoints");
     }
     for (const auto &t : plan.tmpl) {
-        if (t.lb_seg <= 0 || find_sax(ctx, t.m, t.lb_seg))
+        const int seg = words_segments(t);
+        if (seg <= 0 || find_sax(ctx, t.m, seg))
             continue;
         while (ctx->sax.size() >= kSaxKept)
             ctx->sax.erase(ctx->sax.begin());
         auto w = std::make_unique<SaxWords>();
         w->m = t.m;
-        w->segments = t.lb_seg;
+        w->segments = seg;
         const uint64_t W = ctx->n - static_cast<uint64_t>(t.m) + 1;
-        LANE_HIP(w->words.reserve(W * static_cast<uint64_t>(t.lb_seg) + 16), "hipMalloc iSAX words");
-        LANE_HIP(launch_sax_words(ctx->x.as<float>(), ctx->n, t.m, t.lb_seg, ctx->sax_bp.as<float>(),
+        LANE_HIP(w->words.reserve(W * static_cast<uint64_t>(seg) + 16), "hipMalloc iSAX words");
+        LANE_HIP(launch_sax_words(ctx->x.as<float>(), ctx->n, t.m, seg, ctx->sax_bp.as<float>(),
                                   w->words.as<uint8_t>(), L.stream),
                  "iSAX words");
         LANE_HIP(hipStreamSynchronize(L.stream), "iSAX words");
         ctx->sax.push_back(std::move(w));
+    }
+    // The index of every index length (built once per length and leaf size).
+    for (const auto &t : plan.tmpl) {
+        if (t.ix <= 0)
+            continue;
+        SaxWords *sw = find_sax(ctx, t.m, 16);
+        if (sw->ix.nleaves > 0 && sw->ix.leaf_size == cfg.leaf_size)
+            continue;
+        const uint64_t W = ctx->n - static_cast<uint64_t>(t.m) + 1;
+        sw->ix.nleaves = 0;
+        LANE_HIP(build_isax_index(sw->words.as<uint8_t>(), W, cfg.leaf_size, sw->ix, L.stream), "iSAX index");
     }
     return 0;
 }
 
 int build_resident(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan)
 {
-    if (const int rc = build_sax(ctx, L, plan))
+    if (const int rc = build_sax(ctx, L, cfg, plan))
         return rc;
     if (cfg.precision != TMX_PREC_FP32 && plan.Tg > 0) {
         const int rc = build_block_prefix(ctx, L, block_prefix_len(plan));
@@ -731,7 +776,7 @@ int upload_and_prepare(Lane &L, const tmx_cfg &cfg, const CallPlan &plan, const 
     LANE_HIP(launch_prepare(L.raw.as<float>(), L.tmpl.as<TmplDev>(), T, L.qz.as<float>(), L.order.as<int32_t>(),
                             L.stream),
              "prepare kernel");
-    if (plan_has_lb(plan)) {
+    if (plan_has_lb(plan) || plan_has_ix(plan)) {
         LANE_HIP(L.lb_paa.reserve(sizeof(float) * kMaxSegments * static_cast<size_t>(T)), "hipMalloc template PAA");
         LANE_HIP(launch_tmpl_paa(L.tmpl.as<TmplDev>(), T, L.qz.as<float>(), L.lb_paa.as<float>(), L.stream),
                  "template PAA");
@@ -1139,6 +1184,11 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
                                 0, 0, cand_out(L), nullptr, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
                                 lb_count ? L.lb_skip.as<unsigned long long>() : nullptr, L.stream),
                  "early-abandoning scan");
+    if (mode == kModeMain && !plan.ix_ids.empty())
+        LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(plan.ix_ids.size()),
+                                  plan.ix_max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(), nullptr, true,
+                                  cand_out(L), L.stream),
+                 "iSAX index search");
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
     unsigned long long c = 0, nchunk = 0, nsurv = 0, nskip = 0;
     LANE_HIP(hipMemcpyAsync(&c, L.cand_count.p, sizeof(c), hipMemcpyDeviceToHost, L.stream), "read counter");
@@ -1253,6 +1303,68 @@ uint64_t waveform_blocks(tmx_ctx *ctx, const tmx_cfg &cfg)
     return (ctx->n + wb - 1) / wb;
 }
 
+// Which templates take the iSAX index (TREMOR's first choice,
+// Tremor.cpp:2236-2262): with the bounds known, every eligible template whose
+// index leaves the reference's fraction of its windows to visit, f = 0.01 (the
+// windows of the leaves whose region bound does not exceed the template's
+// bound, counted exactly over all leaves), or every eligible one
+// (TMX_INDEX_ALWAYS).  Marks them on the device (ea = 2) and in the plan.
+constexpr double kIndexFraction = 0.01;
+
+int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
+{
+    plan.ix_ids.clear();
+    plan.ix_max_leaves = 0;
+    if (cfg.index_mode == TMX_INDEX_OFF)
+        return 0;
+    std::vector<int32_t> ids;
+    int64_t max_leaves = 0;
+    for (int d = 0; d < plan.T; d++)
+        if (plan.tmpl[d].ix > 0) {
+            ids.push_back(d);
+            max_leaves = std::max<int64_t>(max_leaves, plan.tmpl[d].ix_nleaves);
+        }
+    if (ids.empty())
+        return 0;
+    LANE_HIP(L.ix_ids.reserve(sizeof(int32_t) * ids.size()), "hipMalloc index ids");
+    LANE_HIP(hipMemcpyAsync(L.ix_ids.p, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice, L.stream),
+             "copy index ids");
+    const size_t pb = sizeof(unsigned long long) * static_cast<size_t>(plan.T);
+    LANE_HIP(L.ix_passed.reserve(pb), "hipMalloc index counts");
+    LANE_HIP(launch_zero_f64(L.ix_passed.as<double>(), static_cast<uint64_t>(plan.T), L.stream), "clear index counts");
+    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
+    LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(ids.size()),
+                              max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
+                              L.ix_passed.as<unsigned long long>(), false, cand_out(L), L.stream),
+             "index leaf bounds");
+    std::vector<unsigned long long> passed(plan.T);
+    LANE_HIP(hipMemcpyAsync(passed.data(), L.ix_passed.p, pb, hipMemcpyDeviceToHost, L.stream), "read index counts");
+    LANE_HIP(hipStreamSynchronize(L.stream), "index leaf bounds");
+    std::vector<int32_t> chosen;
+    for (int d : ids) {
+        const double W = static_cast<double>(ctx->n - static_cast<uint64_t>(plan.tmpl[d].m) + 1);
+        if (cfg.index_mode == TMX_INDEX_ALWAYS || static_cast<double>(passed[d]) <= kIndexFraction * W) {
+            chosen.push_back(d);
+            st.index_windows += static_cast<int64_t>(passed[d]);
+        }
+    }
+    if (chosen.empty())
+        return 0;
+    LANE_HIP(hipMemcpyAsync(L.ix_ids.p, chosen.data(), sizeof(int32_t) * chosen.size(), hipMemcpyHostToDevice,
+                            L.stream),
+             "copy index ids");
+    LANE_HIP(launch_mark_ea(L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(chosen.size()), 2,
+                            L.stream),
+             "mark index templates");
+    for (int d : chosen) {
+        plan.tmpl[d].ea = 2;
+        plan.ix_max_leaves = std::max<int64_t>(plan.ix_max_leaves, plan.tmpl[d].ix_nleaves);
+    }
+    plan.ix_ids = chosen;
+    st.index_templates = static_cast<int32_t>(chosen.size());
+    return 0;
+}
+
 // Which templates take the early-abandoning scan: every one (TMX_SCAN_ALWAYS)
 // or, with the bounds known, those whose sampled scan estimate is not beaten
 // by 10% by the planned form (the reference's rule, Tremor.cpp:2286: the FFT
@@ -1266,7 +1378,7 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
         return 0;
     std::vector<int32_t> ids;
     for (int d = 0; d < plan.T; d++) {
-        if (static_cast<uint64_t>(plan.tmpl[d].m) > ctx->n)
+        if (static_cast<uint64_t>(plan.tmpl[d].m) > ctx->n || plan.tmpl[d].ea)
             continue;
         // The scan visits at least one block per window, unless a lower
         // bound skips windows before any block.
@@ -1313,7 +1425,7 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
                  "copy scan ids");
     }
     plan.scan_blocks = blocks_sum / static_cast<double>(chosen.size());
-    LANE_HIP(launch_mark_ea(L.tmpl.as<TmplDev>(), L.scan_ids.as<int32_t>(), static_cast<int>(chosen.size()),
+    LANE_HIP(launch_mark_ea(L.tmpl.as<TmplDev>(), L.scan_ids.as<int32_t>(), static_cast<int>(chosen.size()), 1,
                             L.stream),
              "mark scan templates");
     for (int d : chosen) {
@@ -1332,9 +1444,16 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     std::vector<const FftClass *> fcls;
     for (const auto &g : plan.fft)
         fcls.push_back(find_fft_class(ctx, g.shape));
-    for (auto &t : plan.tmpl)
-        if (t.lb_seg > 0)
-            t.lb_words = find_sax(ctx, t.m, t.lb_seg)->words.as<uint8_t>();
+    for (auto &t : plan.tmpl) {
+        if (words_segments(t) > 0)
+            t.lb_words = find_sax(ctx, t.m, words_segments(t))->words.as<uint8_t>();
+        if (t.ix > 0) {
+            const IxIndex &ix = find_sax(ctx, t.m, 16)->ix;
+            t.ix_leaves = ix.leaves.as<IxLeaf>();
+            t.ix_ids = ix.ids.as<uint32_t>();
+            t.ix_nleaves = ix.nleaves;
+        }
+    }
     int rc = upload_and_prepare(L, cfg, plan, raw_packed, fcls);
     if (rc)
         return rc;
@@ -1376,6 +1495,8 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
                  "seed bound");
         LANE_HIP(hipStreamSynchronize(L.stream), "seed");
     }
+    if ((rc = choose_index(ctx, L, cfg, plan, st)))
+        return rc;
     if ((rc = choose_scan(ctx, L, cfg, plan)))
         return rc;
     st.scan_templates = static_cast<int32_t>(plan.scan_ids.size());
@@ -1570,6 +1691,8 @@ void add_stats(tmx_stats &a, const tmx_stats &b)
     a.fft_lmax = std::max(a.fft_lmax, b.fft_lmax);
     a.splits += b.splits;
     a.lb_skipped += b.lb_skipped;
+    a.index_templates += b.index_templates;
+    a.index_windows += b.index_windows;
 }
 
 }  // namespace
@@ -1673,6 +1796,8 @@ int tmx_cfg_default(tmx_cfg *cfg)
     cfg->verify_seq = 0;
     cfg->autotune = 1;
     cfg->mfma_launch = TMX_LAUNCH_AUTO;
+    cfg->index_mode = TMX_INDEX_OFF;
+    cfg->leaf_size = 2000;
     cfg->max_candidates = 0;
     cfg->lower_bounds = 0;
     cfg->paa_segments = 16;
@@ -1692,6 +1817,8 @@ int tmx_create(tmx_ctx **out, const tmx_cfg *cfg)
     if (ctx->cfg.precision < TMX_PREC_FP32 || ctx->cfg.precision > TMX_PREC_FP16 ||
         ctx->cfg.fft_mode < TMX_FFT_AUTO || ctx->cfg.fft_mode > TMX_FFT_ALWAYS ||
         ctx->cfg.mfma_launch < TMX_LAUNCH_AUTO || ctx->cfg.mfma_launch > TMX_LAUNCH_QUEUE ||
+        ctx->cfg.index_mode < TMX_INDEX_OFF || ctx->cfg.index_mode > TMX_INDEX_ALWAYS ||
+        (ctx->cfg.index_mode != TMX_INDEX_OFF && ctx->cfg.leaf_size < 1) ||
         (ctx->cfg.lower_bounds && (ctx->cfg.paa_segments < 1 || ctx->cfg.paa_segments > kMaxSegments))) {
         delete ctx;
         return -1;
@@ -2044,6 +2171,59 @@ int tmx_window_stats(tmx_ctx *ctx, int32_t m, uint64_t first, uint64_t count, fl
         e = hipStreamSynchronize(L.stream);
     if (e != hipSuccess)
         return ctx->fail(std::string("window stats: ") + hipGetErrorString(e), -2);
+    return 0;
+}
+
+int tmx_index_probe(tmx_ctx *ctx, int32_t m, int32_t leaf_size, uint32_t *ids, uint64_t ids_cap,
+                    uint32_t *leaf_start, uint32_t *leaf_count, int32_t *leaf_depth, int64_t leaf_cap,
+                    int64_t *nleaves, uint8_t *words)
+{
+    if (!ctx)
+        return -1;
+    if (!nleaves || (leaf_cap > 0 && (!leaf_start || !leaf_count || !leaf_depth)) || leaf_size < 1)
+        return ctx->fail("index probe: bad arguments");
+    std::lock_guard<std::mutex> call(ctx->call_mu);
+    if (ctx->n == 0)
+        return ctx->fail("no waveform loaded");
+    tmx_cfg cfg = ctx->cfg;
+    cfg.index_mode = TMX_INDEX_ALWAYS;
+    cfg.leaf_size = leaf_size;
+    if (m <= 0 || !ix_eligible(cfg, m, ctx->n))
+        return ctx->fail("index probe: the length must be a multiple of 16, at most n, with fewer than 2^31 windows");
+    const uint64_t W = ctx->n - static_cast<uint64_t>(m) + 1;
+    if (ids && ids_cap < W)
+        return ctx->fail("index probe: ids holds fewer than n - m + 1 entries");
+    Lane &L = ctx->direct;
+    if (hipSetDevice(ctx->cfg.device) != hipSuccess)
+        return ctx->fail("hipSetDevice failed", -2);
+    CallPlan plan;
+    plan.T = 1;
+    plan.tmpl.assign(1, TmplDev{});
+    plan.tmpl[0].m = m;
+    plan.tmpl[0].ix = 1;
+    WriteResident wl(ctx);
+    if (const int rc = build_sax(ctx, L, cfg, plan))
+        return ctx->fail(L.err, rc);
+    const SaxWords *sw = find_sax(ctx, m, 16);
+    const IxIndex &ix = sw->ix;
+    *nleaves = ix.nleaves;
+    hipError_t e = hipSuccess;
+    if (ids)
+        e = hipMemcpy(ids, ix.ids.p, sizeof(uint32_t) * W, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && words)
+        e = hipMemcpy(words, sw->words.p, 16 * W, hipMemcpyDeviceToHost);
+    const int64_t nl = std::min<int64_t>(leaf_cap, ix.nleaves);
+    if (e == hipSuccess && nl > 0) {
+        std::vector<IxLeaf> lv(static_cast<size_t>(nl));
+        e = hipMemcpy(lv.data(), ix.leaves.p, sizeof(IxLeaf) * lv.size(), hipMemcpyDeviceToHost);
+        for (int64_t i = 0; e == hipSuccess && i < nl; i++) {
+            leaf_start[i] = lv[i].start;
+            leaf_count[i] = lv[i].count;
+            leaf_depth[i] = lv[i].depth;
+        }
+    }
+    if (e != hipSuccess)
+        return ctx->fail(std::string("index probe: ") + hipGetErrorString(e), -2);
     return 0;
 }
 

@@ -44,13 +44,17 @@ struct TmplDev {
     float bound;        // candidate cut: threshold distance, or the kNN seed bound
     int32_t k;          // kNN answers wanted (0: threshold template)
     int32_t merge;      // exclusion zone of the answers (merge offset)
-    int32_t ea;         // 1: the early-abandoning scan takes this template (the other forms skip it)
+    int32_t ea;         // 1: the early-abandoning scan takes this template, 2: the iSAX index does (the
+                        // other forms skip it either way)
     int64_t out_off;    // first kNN output slot of this template
     L1Consts l1;        // matrix-core level-1 constants (device-computed per pass)
     int32_t lb_seg;     // skip-sequential lower bound: PAA segments (0 = none)
-    int32_t lb_pad;
+    int32_t ix;         // 1: the iSAX index may take this template (16 segments; its PAA and words kept)
     int64_t lb_paa;     // offset of the template's PAA in the lane's PAA array
-    const uint8_t *lb_words;  // resident iSAX words of this length (window j at j * lb_seg)
+    const uint8_t *lb_words;  // resident iSAX words of this length (window j at j * lb_seg, or * 16)
+    const struct IxLeaf *ix_leaves;  // the index of this length (ix = 1): leaves in key order,
+    const uint32_t *ix_ids;          // the window ids in key order,
+    int64_t ix_nleaves;              // and the leaf count
 };
 
 // One (template, FFT piece) pair of the FFT form.
