@@ -268,23 +268,30 @@ def bench_skip_sequential(a):
     cut from events, kNN k = 10 and threshold rho >= 0.8.  The
     early-abandoning scan for every template, without and with the iSAX lower
     bound (the reference's sequential and skip-sequential methods), and the
-    cost model's own choice; all with exact verification and the same
-    answers.  The first bounded call builds the iSAX words of the length."""
+    cost model's own choice, and the iSAX index (the reference's tremor
+    method): every template through it at n <= 10^7, else by the 1% rule;
+    all with exact verification and the same answers.  The first bounded or
+    indexed call builds the iSAX words (and the index) of the length."""
     from tremor_amd import synth
     from tremor_amd.capi import Context
     w = synth.make("c0", n=a.n if a.n != 10**8 else None)
     k, rho = 10, 0.8
     out = {"config": {"workload": f"plain small workload: n={w.x.size}, {len(w.templates)} templates m=256, "
                                   f"kNN k={k}, threshold rho>={rho}", "segments": a.segments}}
-    legs = (("sequential", "always", False), ("skip_sequential", "always", True), ("cost_model", "auto", False))
+    index_mode = "always" if w.x.size <= 10**7 else "auto"
+    legs = (("sequential", "always", False, "off"), ("skip_sequential", "always", True, "off"),
+            ("cost_model", "auto", False, "off"), ("isax_index", "auto", False, index_mode))
     res, answers = {}, {}
-    for name, scan, lb in legs:
-        with Context(precision=a.prec, scan=scan, lower_bounds=lb, paa_segments=a.segments) as ctx:
+    for name, scan, lb, index in legs:
+        if index != "off" and a.segments != 16:
+            continue  # the index keys 16 segments
+        with Context(precision=a.prec, scan=scan, lower_bounds=lb, paa_segments=a.segments, index=index) as ctx:
             ctx.load_waveform(w.x)
+            row = {"index_mode": index}
             t0 = time.perf_counter()
             ctx.knn(w.templates, k, 0)  # first call: words of the length, tuning
             first = time.perf_counter() - t0
-            row = {"first_call_ms": 1e3 * first}
+            row["first_call_ms"] = 1e3 * first
             for kind in ("knn", "threshold"):
                 t0 = time.perf_counter()
                 for _ in range(a.steps):
@@ -300,10 +307,12 @@ def bench_skip_sequential(a):
                              "kernel_ms_main": st["kernel_ms_main"], "scan_templates": st["scan_templates"],
                              "scan_blocks_per_window": st["scan_blocks"], "lb_skipped": st["lb_skipped"],
                              "lb_skipped_frac": st["lb_skipped"] / max(1, st["comparisons"]),
+                             "index_templates": st["index_templates"], "index_windows": st["index_windows"],
                              "candidates": st["candidates"]}
                 answers[(name, kind)] = ans
                 print(f"[bench] c0 {name} {kind}: {1e3 * el:.2f} ms/step, main {st['kernel_ms_main']:.2f} ms, "
-                      f"scan templates {st['scan_templates']}, skipped by the bound {st['lb_skipped']}",
+                      f"scan templates {st['scan_templates']}, skipped by the bound {st['lb_skipped']}, "
+                      f"index templates {st['index_templates']} ({st['index_windows']} windows in passing leaves)",
                       file=sys.stderr, flush=True)
             res[name] = row
 
@@ -312,7 +321,7 @@ def bench_skip_sequential(a):
                                   v.view(np.uint32) if v.dtype == np.float32 else v) for u, v in zip(x, y))
 
     agree = all(same(answers[("sequential", kd)], answers[(nm, kd)])
-                for nm in ("skip_sequential", "cost_model") for kd in ("knn", "threshold"))
+                for nm in res if nm != "sequential" for kd in ("knn", "threshold"))
     out.update({"metric": METRIC, "unit": "comparisons/s", "n_gpus": 1, "forms": res, "same_answers": bool(agree),
                 "steps": a.steps, "higher_is_better": True,
                 "value": res["skip_sequential"]["knn"]["comparisons_per_s"],

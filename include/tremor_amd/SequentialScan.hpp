@@ -57,6 +57,9 @@ public:
 
     // Candidate forms (answers unchanged): enum tmx_fft_mode / tmx_scan_mode.
     void set_forms(int fft_mode, int scan_mode);
+    // The iSAX index (enum tmx_index_mode) with leaves of at most leaf_size
+    // windows: Tremor's method (tremor_main.cpp --method tremor, --leaf-size).
+    void set_index(int index_mode, int leaf_size);
 
     // Device bytes held beyond the raw samples (prefix statistics).
     double summary_memory_bytes() const { return summary_bytes_; }

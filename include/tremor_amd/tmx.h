@@ -32,6 +32,8 @@
  *   tmx_prepare_template znormalize_inplace + query_block_order ZnormSearch.hpp:33-90
  *   tmx_set_fft_mode,    Tremor::setRefinementMode          algos/htremor/Tremor.hpp:37-44,161
  *   tmx_set_scan_mode    (scan / FFT / adaptive refinement)
+ *   tmx_set_index_mode   Tremor's index search and leaf size  tremor_main.cpp:296-302 (--method tremor),
+ *                        (--refinement leaf / adaptive)      :312-314, :341-343 (--leaf-size)
  */
 #ifndef TREMOR_AMD_TMX_H
 #define TREMOR_AMD_TMX_H
@@ -170,6 +172,8 @@ const char *tmx_last_error(const tmx_ctx *ctx);
 int tmx_set_precision(tmx_ctx *ctx, int32_t precision);
 int tmx_set_fft_mode(tmx_ctx *ctx, int32_t mode);
 int tmx_set_scan_mode(tmx_ctx *ctx, int32_t mode);
+/* enum tmx_index_mode and the index's leaf size (>= 1) for later calls. */
+int tmx_set_index_mode(tmx_ctx *ctx, int32_t mode, int32_t leaf_size);
 
 /* Copies the waveform to the device and builds its FP64 prefix sums of x and
  * x^2 (missing samples as 0).  Replaces any waveform loaded before.  Every

@@ -42,6 +42,12 @@ def test_rejects_like_the_reference(cli, tmp_path):
     assert r.returncode == 1 and "Cannot open" in r.stderr
     r = _run(cli, "--dataset", x, "--queries", q, "--window-length", 64, "--dataset-size", 2000)
     assert r.returncode == 1
+    r = _run(cli, "--dataset", x, "--queries", q, "--window-length", 64, "--method", "index")
+    assert r.returncode == 2 and "Unknown --method" in r.stderr
+    r = _run(cli, "--dataset", x, "--queries", q, "--window-length", 64, "--refinement", "order")
+    assert r.returncode == 2 and "Unknown --refinement" in r.stderr
+    r = _run(cli, "--dataset", x, "--queries", q, "--window-length", 64, "--leaf-size", 0)
+    assert r.returncode == 2 and "--leaf-size must be greater than 0" in r.stderr
 
 
 def _corr(d, m):
@@ -68,6 +74,13 @@ def test_csv_outputs_match_reference_format(cli, oracle, tmp_path):
             for j in range(5):
                 exp.append("%d, %d, %d, %f" % (qi, j + 1, int(pos[qi, j]), float(_corr(d[qi, j], m))))
         assert out.read_text() == "\n".join(exp) + "\n"
+        # The tremor method (the iSAX index), leaf and adaptive refinement.
+        for extra in (("--method", "tremor"), ("--method", "tremor", "--refinement", "adaptive", "--leaf-size", 100)):
+            out_t = tmp_path / f"knn{merge}_tremor.csv"
+            r = _run(cli, "--dataset", x, "--queries", q, "--window-length", m, "--mode", "knn", "--k", 5,
+                     "--merge-offset", merge, "--output", out_t, *extra)
+            assert r.returncode == 0, r.stderr
+            assert out_t.read_text() == "\n".join(exp) + "\n"
         timings = (tmp_path / f"knn{merge}.csv.timings.csv").read_text().splitlines()
         assert timings[0] == "node_rank,index_time,query_time,index_gb" and timings[1].startswith("0,")
         prefix = tmp_path / f"thr{merge}"

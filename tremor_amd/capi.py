@@ -22,7 +22,7 @@ SCAN_MODES = {"auto": 0, "off": 1, "always": 2}
 
 EXPORTS = [
     "tmx_cfg_default", "tmx_create", "tmx_destroy", "tmx_last_error", "tmx_set_precision", "tmx_set_fft_mode",
-    "tmx_set_scan_mode",
+    "tmx_set_scan_mode", "tmx_set_index_mode",
     "tmx_load_waveform",
     "tmx_waveform_length", "tmx_knn", "tmx_threshold", "tmx_hits_free", "tmx_distance_batch",
     "tmx_distance_batch_noalloc", "tmx_window_stats",
@@ -103,6 +103,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.tmx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.tmx_set_fft_mode.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.tmx_set_scan_mode.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.tmx_set_index_mode.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
     L.tmx_load_waveform.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint64]
     L.tmx_waveform_length.argtypes = [ctypes.c_void_p]
     L.tmx_waveform_length.restype = ctypes.c_uint64
@@ -257,6 +258,11 @@ class Context:
     def set_scan_mode(self, mode: str):
         """Early-abandoning scan: "auto" (cost model), "off" or "always"."""
         self._check(self._L.tmx_set_scan_mode(self._h, SCAN_MODES[mode]))
+
+    def set_index_mode(self, mode: str, leaf_size: int = 2000):
+        """iSAX index: "off", "auto" (the 1% rule) or "always"; leaves of at
+        most leaf_size windows."""
+        self._check(self._L.tmx_set_index_mode(self._h, {"off": 0, "auto": 1, "always": 2}[mode], leaf_size))
 
     def load_waveform(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -33,8 +33,9 @@ struct Options {
     bool knn = true;
     idx_t k = 10;
     float rho = 0.93f;
-    int merge = 0, replication_groups = 0, paa = 16, workers = 4, index_threads = 4;
+    int merge = 0, replication_groups = 0, paa = 16, workers = 4, index_threads = 4, leaf_size = 2000;
     std::string method = "sequential";
+    bool adaptive = false;  // --refinement adaptive
     int precision = TMX_PREC_BF16X3;
     int fft_mode = TMX_FFT_AUTO;
     int scan_mode = TMX_SCAN_AUTO;
@@ -55,8 +56,13 @@ void usage(const char *prog)
         "  --output PATH              kNN CSV, or threshold prefix (<PATH>_0.csv)\n"
         "  --benchmark-output PATH    timings CSV (default <output>.timings.csv)\n"
         "  --no-benchmark-report      skip the timings CSV\n"
-        "  --method sequential|skip-sequential  (skip-sequential: every template on the\n"
-        "                             early-abandoning scan with the iSAX lower bound; same answers)\n"
+        "  --method sequential|skip-sequential|tremor  (skip-sequential: every template on the\n"
+        "                             early-abandoning scan with the iSAX lower bound; tremor: the iSAX\n"
+        "                             index for window lengths divisible by 16 with 16 PAA segments;\n"
+        "                             same answers)\n"
+        "  --refinement leaf|adaptive tremor: every such template through the index (leaf, default),\n"
+        "                             or where it leaves at most 1%% of the windows (adaptive)\n"
+        "  --leaf-size N              iSAX leaf size (default 2000)\n"
         "  --precision bf16x3|fp16|fp32  candidate pass on the GPU (answers identical)\n"
         "  --fft-mode auto|off|always     FFT form of the candidate pass (default auto: cost model)\n"
         "  --scan-mode auto|off|always    early-abandoning scan form (default auto: cost model)\n"
@@ -132,8 +138,16 @@ Options parse(int argc, char **argv)
                 throw std::invalid_argument("--mode must be knn or threshold");
         } else if (a == "--method") {
             o.method = value(i, a);
-            if (o.method != "sequential" && o.method != "skip-sequential")
-                throw std::invalid_argument("this GPU driver implements --method sequential / skip-sequential");
+            if (o.method != "sequential" && o.method != "skip-sequential" && o.method != "tremor")
+                throw std::invalid_argument("Unknown --method: " + o.method +
+                                            " (expected tremor, skip-sequential, or sequential)");
+        } else if (a == "--refinement") {
+            const std::string r = value(i, a);
+            if (r != "leaf" && r != "adaptive")
+                throw std::invalid_argument("Unknown --refinement: " + r + " (expected leaf or adaptive)");
+            o.adaptive = r == "adaptive";
+        } else if (a == "--leaf-size") {
+            o.leaf_size = parse_i(value(i, a), a);
         } else if (a == "--k") {
             o.k = parse_u(value(i, a), a);
         } else if (a == "--threshold") {
@@ -188,6 +202,8 @@ Options parse(int argc, char **argv)
         throw std::invalid_argument("--window-length must be divisible by 8");
     if (o.paa <= 0)
         throw std::invalid_argument("--paa-segments must be greater than 0");
+    if (o.leaf_size <= 0)
+        throw std::invalid_argument("--leaf-size must be greater than 0");
     if (o.window % static_cast<idx_t>(o.paa) != 0)
         throw std::invalid_argument("--window-length must be divisible by --paa-segments");
     if (o.dataset_size_set && o.dataset_size < o.window)
@@ -271,12 +287,20 @@ int run(const Options &o)
     const std::vector<float> queries = read_floats(o.queries, qcount * o.window);
     const double input_s = now_s() - t_in;
 
+    // Lower bounds for the methods other than sequential (tremor_main.cpp:302).
     tremor_amd::SequentialScan scan(o.workers, o.paa, o.merge, o.replication_groups, o.method != "sequential",
                                     o.precision, o.device);
     // The skip-sequential method is the bounded early-abandoning scan for
     // every template (unless --scan-mode says otherwise).
     const bool sss = o.method == "skip-sequential";
     scan.set_forms(o.fft_mode, sss && !o.scan_mode_given ? TMX_SCAN_ALWAYS : o.scan_mode);
+    // The tremor method: the iSAX index (16 PAA segments), for every eligible
+    // template with leaf refinement or by the 1% rule with adaptive.
+    if (o.method == "tremor") {
+        if (o.paa != 16)
+            throw std::invalid_argument("this GPU driver's index uses 16 PAA segments");
+        scan.set_index(o.adaptive ? TMX_INDEX_AUTO : TMX_INDEX_ALWAYS, o.leaf_size);
+    }
     const double t_build = now_s();
     scan.build(o.dataset, n, static_cast<int>(o.window));
     const double build_s = now_s() - t_build;

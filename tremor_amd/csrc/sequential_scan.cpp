@@ -62,6 +62,8 @@ void SequentialScan::set_forms(int fft_mode, int scan_mode)
     check(tmx_set_scan_mode(ctx_, scan_mode));
 }
 
+void SequentialScan::set_index(int index_mode, int leaf_size) { check(tmx_set_index_mode(ctx_, index_mode, leaf_size)); }
+
 void SequentialScan::build(const std::string &filename, idx_t total_samples, int window_length)
 {
     std::vector<float> raw(static_cast<size_t>(total_samples));

@@ -1877,6 +1877,20 @@ int tmx_set_scan_mode(tmx_ctx *ctx, int32_t mode)
     return 0;
 }
 
+int tmx_set_index_mode(tmx_ctx *ctx, int32_t mode, int32_t leaf_size)
+{
+    if (!ctx)
+        return -1;
+    if (mode < TMX_INDEX_OFF || mode > TMX_INDEX_ALWAYS)
+        return ctx->fail("unknown index mode");
+    if (leaf_size < 1)
+        return ctx->fail("the leaf size must be positive");
+    std::lock_guard<std::mutex> lk(ctx->call_mu);
+    ctx->cfg.index_mode = mode;
+    ctx->cfg.leaf_size = leaf_size;
+    return 0;
+}
+
 int tmx_set_precision(tmx_ctx *ctx, int32_t precision)
 {
     if (!ctx)
