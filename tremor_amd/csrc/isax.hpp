@@ -146,11 +146,16 @@ hipError_t launch_sax_words(const float *x, uint64_t n, int m, int segments, con
 hipError_t launch_tmpl_paa(const TmplDev *tmpl, int count, const float *qz, float *paa, hipStream_t stream);
 
 // Templates ids[0, count) of the index: per leaf its bound against the
-// template's; emit = false adds the windows of the passing leaves to
-// passed[template]; emit = true makes each window of a passing leaf whose own
-// bound passes and whose first sample is present a candidate.
+// template's (the windows of the passing leaves added to passed[template]);
+// emit: each window of a passing leaf whose own bound passes and whose first
+// sample is present becomes a candidate.
 hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves,
                             const float *bp, const float *paa, unsigned long long *passed, bool emit,
                             const CandOut &co, hipStream_t stream);
+
+// The routing sample: windows j * (W / samples), j < samples, whose word
+// bound does not exceed the template's bound, added to passed[template].
+hipError_t launch_ix_sample(const TmplDev *tmpl, const int32_t *ids, int count, const float *bp, const float *paa,
+                            uint64_t n, int samples, unsigned long long *passed, hipStream_t stream);
 
 }  // namespace tmx

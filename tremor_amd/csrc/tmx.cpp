@@ -1184,17 +1184,26 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
                                 0, 0, cand_out(L), nullptr, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
                                 lb_count ? L.lb_skip.as<unsigned long long>() : nullptr, L.stream),
                  "early-abandoning scan");
-    if (mode == kModeMain && !plan.ix_ids.empty())
+    const bool ix = mode == kModeMain && !plan.ix_ids.empty();
+    if (ix) {
+        LANE_HIP(launch_zero_f64(L.ix_passed.as<double>(), static_cast<uint64_t>(plan.T), L.stream),
+                 "clear index counts");
         LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(plan.ix_ids.size()),
-                                  plan.ix_max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(), nullptr, true,
-                                  cand_out(L), L.stream),
+                                  plan.ix_max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
+                                  L.ix_passed.as<unsigned long long>(), true, cand_out(L), L.stream),
                  "iSAX index search");
+    }
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
     unsigned long long c = 0, nchunk = 0, nsurv = 0, nskip = 0;
     LANE_HIP(hipMemcpyAsync(&c, L.cand_count.p, sizeof(c), hipMemcpyDeviceToHost, L.stream), "read counter");
     if (lb_count)
         LANE_HIP(hipMemcpyAsync(&nskip, L.lb_skip.p, sizeof(nskip), hipMemcpyDeviceToHost, L.stream),
                  "read skip counter");
+    std::vector<unsigned long long> ixp(ix ? plan.T : 0);
+    if (ix)
+        LANE_HIP(hipMemcpyAsync(ixp.data(), L.ix_passed.p, sizeof(unsigned long long) * plan.T, hipMemcpyDeviceToHost,
+                                L.stream),
+                 "read index counts");
     if (mfma && mode == kModeMain)
         LANE_HIP(hipMemcpyAsync(&nsurv, L.surv_n.p, sizeof(nsurv), hipMemcpyDeviceToHost, L.stream),
                  "read survivor counter");
@@ -1208,8 +1217,12 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     // Likewise a matrix-core pass whose survivor queue overflowed.
     L.surv_need = nsurv > L.surv_cap ? nsurv + (nsurv >> 3) + 1024 : 0;
     L.last_survivors = nsurv;
-    if (mode == kModeMain)
+    if (mode == kModeMain) {
         L.last_lb_skipped = nskip;
+        L.last_ix_windows = 0;
+        for (unsigned long long v : ixp)
+            L.last_ix_windows += v;
+    }
     LANE_HIP(hipEventElapsedTime(&L.last_pass_ms, L.ev0, L.ev1), "event time");
     *count = c;
     return 0;
@@ -1303,13 +1316,22 @@ uint64_t waveform_blocks(tmx_ctx *ctx, const tmx_cfg &cfg)
     return (ctx->n + wb - 1) / wb;
 }
 
-// Which templates take the iSAX index (TREMOR's first choice,
-// Tremor.cpp:2236-2262): with the bounds known, every eligible template whose
-// index leaves the reference's fraction of its windows to visit, f = 0.01 (the
-// windows of the leaves whose region bound does not exceed the template's
-// bound, counted exactly over all leaves), or every eligible one
-// (TMX_INDEX_ALWAYS).  Marks them on the device (ea = 2) and in the plan.
+// Which templates take the iSAX index (TREMOR's adaptive refinement,
+// Tremor.cpp:2236-2262): with the bounds known, every eligible template for
+// which at most the reference's fraction f = 0.01 of a strided sample of 2^16
+// windows have word bounds within the template's bound (sample_candidates,
+// Tremor.cpp:2045-2059; more than that and the reference scans instead), or
+// every eligible one (TMX_INDEX_ALWAYS).  Marks them on the device (ea = 2)
+// and in the plan.
 constexpr double kIndexFraction = 0.01;
+constexpr int kIndexSample = 1 << 16;
+// The index's own cost on this GPU, seconds per window of a passing leaf (its
+// word bound and the candidate write; measured 5.3e-11 at n = 10^8), and per
+// leaf (one thread's region bound, ~100 instructions).  Auto mode also asks
+// that the index beat 0.9 x the planned form, as the scan must (the
+// reference's rule is set for its CPU costs: on the GPU the dense forms cost
+// picoseconds per window).
+constexpr double kIndexPerWindow = 5.3e-11, kIndexPerLeaf = 1e-12;
 
 int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
 {
@@ -1318,12 +1340,9 @@ int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_
     if (cfg.index_mode == TMX_INDEX_OFF)
         return 0;
     std::vector<int32_t> ids;
-    int64_t max_leaves = 0;
     for (int d = 0; d < plan.T; d++)
-        if (plan.tmpl[d].ix > 0) {
+        if (plan.tmpl[d].ix > 0)
             ids.push_back(d);
-            max_leaves = std::max<int64_t>(max_leaves, plan.tmpl[d].ix_nleaves);
-        }
     if (ids.empty())
         return 0;
     LANE_HIP(L.ix_ids.reserve(sizeof(int32_t) * ids.size()), "hipMalloc index ids");
@@ -1332,21 +1351,50 @@ int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_
     const size_t pb = sizeof(unsigned long long) * static_cast<size_t>(plan.T);
     LANE_HIP(L.ix_passed.reserve(pb), "hipMalloc index counts");
     LANE_HIP(launch_zero_f64(L.ix_passed.as<double>(), static_cast<uint64_t>(plan.T), L.stream), "clear index counts");
-    Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
-    LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(ids.size()),
-                              max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
-                              L.ix_passed.as<unsigned long long>(), false, cand_out(L), L.stream),
-             "index leaf bounds");
-    std::vector<unsigned long long> passed(plan.T);
-    LANE_HIP(hipMemcpyAsync(passed.data(), L.ix_passed.p, pb, hipMemcpyDeviceToHost, L.stream), "read index counts");
-    LANE_HIP(hipStreamSynchronize(L.stream), "index leaf bounds");
+    std::vector<unsigned long long> passed(plan.T, 0ull);
+    if (cfg.index_mode != TMX_INDEX_ALWAYS) {
+        LANE_HIP(launch_ix_sample(L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(ids.size()),
+                                  ctx->sax_bp.as<float>(), L.lb_paa.as<float>(), ctx->n, kIndexSample,
+                                  L.ix_passed.as<unsigned long long>(), L.stream),
+                 "index sample");
+        LANE_HIP(hipMemcpyAsync(passed.data(), L.ix_passed.p, pb, hipMemcpyDeviceToHost, L.stream),
+                 "read index sample");
+        LANE_HIP(hipStreamSynchronize(L.stream), "index sample");
+    }
     std::vector<int32_t> chosen;
     for (int d : ids) {
-        const double W = static_cast<double>(ctx->n - static_cast<uint64_t>(plan.tmpl[d].m) + 1);
-        if (cfg.index_mode == TMX_INDEX_ALWAYS || static_cast<double>(passed[d]) <= kIndexFraction * W) {
+        const uint64_t W = ctx->n - static_cast<uint64_t>(plan.tmpl[d].m) + 1;
+        const double samples = static_cast<double>(std::min<uint64_t>(kIndexSample, W));
+        if (cfg.index_mode == TMX_INDEX_ALWAYS || static_cast<double>(passed[d]) <= kIndexFraction * samples)
             chosen.push_back(d);
-            st.index_windows += static_cast<int64_t>(passed[d]);
+    }
+    if (cfg.index_mode == TMX_INDEX_AUTO && !chosen.empty()) {
+        // The windows in the passing leaves of the templates the rule kept.
+        int64_t max_leaves = 0;
+        for (int d : chosen)
+            max_leaves = std::max<int64_t>(max_leaves, plan.tmpl[d].ix_nleaves);
+        LANE_HIP(hipMemcpyAsync(L.ix_ids.p, chosen.data(), sizeof(int32_t) * chosen.size(), hipMemcpyHostToDevice,
+                                L.stream),
+                 "copy index ids");
+        LANE_HIP(launch_zero_f64(L.ix_passed.as<double>(), static_cast<uint64_t>(plan.T), L.stream),
+                 "clear index counts");
+        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
+        LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(chosen.size()),
+                                  max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
+                                  L.ix_passed.as<unsigned long long>(), false, cand_out(L), L.stream),
+                 "index leaf count");
+        LANE_HIP(hipMemcpyAsync(passed.data(), L.ix_passed.p, pb, hipMemcpyDeviceToHost, L.stream),
+                 "read index counts");
+        LANE_HIP(hipStreamSynchronize(L.stream), "index leaf count");
+        std::vector<int32_t> kept;
+        for (int d : chosen) {
+            const double W = static_cast<double>(ctx->n - static_cast<uint64_t>(plan.tmpl[d].m) + 1);
+            const double ix_s = static_cast<double>(passed[d]) * kIndexPerWindow +
+                                static_cast<double>(plan.tmpl[d].ix_nleaves) * kIndexPerLeaf;
+            if (ix_s < 0.9 * planned_seconds_per_window(cfg, plan, d) * W)
+                kept.push_back(d);
         }
+        chosen.swap(kept);
     }
     if (chosen.empty())
         return 0;
@@ -1510,6 +1558,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
     if ((rc = run_main(ctx, L, cfg, plan, fcls, &ncand, &st.retries)))
         return rc;
     st.lb_skipped = static_cast<int64_t>(L.last_lb_skipped);
+    st.index_windows = static_cast<int64_t>(L.last_ix_windows);
     st.kernel_ms_main = L.last_pass_ms;
     st.flops_main = plan.flops;
     st.fft_templates = 0;
