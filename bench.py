@@ -47,6 +47,8 @@ def _args():
     ap.add_argument("--segments", type=int, default=16, help="c0: PAA segments of the lower bound")
     ap.add_argument("--c0-k", type=int, default=10, help="c0: kNN k")
     ap.add_argument("--c0-rho", type=float, default=0.8, help="c0: correlation threshold")
+    ap.add_argument("--c0-index", default="", choices=["", "auto", "always"],
+                    help="c0: iSAX index leg mode (default: always up to n = 10^7, else auto)")
     ap.add_argument("--fft", default="auto", choices=["auto", "off", "always"])
     ap.add_argument("--rate", type=float, default=200.0, help="c3: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="c3: seconds of arrivals (batched loop)")
@@ -280,7 +282,7 @@ def bench_skip_sequential(a):
     k, rho = a.c0_k, a.c0_rho
     out = {"config": {"workload": f"plain small workload: n={w.x.size}, {len(w.templates)} templates m=256, "
                                   f"kNN k={k}, threshold rho>={rho}", "segments": a.segments}}
-    index_mode = "always" if w.x.size <= 10**7 else "auto"
+    index_mode = a.c0_index or ("always" if w.x.size <= 10**7 else "auto")
     legs = (("sequential", "always", False, "off"), ("skip_sequential", "always", True, "off"),
             ("cost_model", "auto", False, "off"), ("isax_index", "auto", False, index_mode))
     res, answers = {}, {}

@@ -195,6 +195,57 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
     }
 }
 
+// The approximate search that seeds the reference's exact index search (the
+// query's own leaf first): for template ids[blockIdx.x], its word from its
+// PAA (symbols as the windows'), its key, and the `per` windows nearest that
+// key in key order (its leaf and the neighbouring ones) as candidates whose
+// exact distances tighten the kNN bound.  Windows starting with a missing
+// sample are left out.
+__global__ __launch_bounds__(kThreads) void k_ix_home(Waveform w, const TmplDev *__restrict__ tmpl,
+                                                      const int32_t *__restrict__ ids,
+                                                      const float *__restrict__ bp_g, const float *__restrict__ paa_g,
+                                                      int per, CandOut co)
+{
+    __shared__ float bp[256];
+    __shared__ uint32_t s_pos;
+    const int tid = threadIdx.x;
+    const int d = ids[blockIdx.x];
+    const TmplDev &t = tmpl[d];
+    for (int i = tid; i < 255; i += kThreads)
+        bp[i] = bp_g[i];
+    __syncthreads();
+    const uint64_t W = w.n - static_cast<uint64_t>(t.m) + 1;
+    const uint64_t *keys = t.ix_keys;
+    if (tid == 0) {
+        uint64_t key = 0;
+        for (int s = 0; s < 16; s++) {
+            const uint32_t sym = static_cast<uint32_t>(sax_symbol(bp, paa_g[t.lb_paa + s]));
+            for (int p = 0; p < kIxPlanes; p++)
+                key |= static_cast<uint64_t>((sym >> (7 - p)) & 1u) << (63 - (16 * p + s));
+        }
+        uint64_t lo = 0, hi = W;  // first sorted key >= the template's
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (keys[mid] < key)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        s_pos = static_cast<uint32_t>(lo);
+    }
+    __syncthreads();
+    const uint64_t n = static_cast<uint64_t>(per) < W ? static_cast<uint64_t>(per) : W;
+    const uint64_t half = n / 2;
+    uint64_t first = s_pos > half ? s_pos - half : 0;
+    if (first + n > W)
+        first = W - n;
+    for (uint64_t e = tid; e < n; e += kThreads) {
+        const uint32_t j = t.ix_ids[first + e];
+        if (w.x[j] != kMissing)
+            cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(d), j, NAN, -INFINITY, INFINITY);
+    }
+}
+
 }  // namespace
 
 hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxIndex &ix, hipStream_t stream)
@@ -297,6 +348,16 @@ hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_
         hipLaunchKernelGGL(k_ix_search<true>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa, passed, co);
     else
         hipLaunchKernelGGL(k_ix_search<false>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa, passed, co);
+    return hipGetLastError();
+}
+
+hipError_t launch_ix_home(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, const float *bp,
+                          const float *paa, int per, const CandOut &co, hipStream_t stream)
+{
+    if (count <= 0 || per <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_ix_home, dim3(static_cast<unsigned>(count)), dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa,
+                       per, co);
     return hipGetLastError();
 }
 

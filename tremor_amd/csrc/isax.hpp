@@ -153,6 +153,12 @@ hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_
                             const float *bp, const float *paa, unsigned long long *passed, bool emit,
                             const CandOut &co, hipStream_t stream);
 
+// The approximate search before the exact one: for each template ids[i],
+// the `per` windows nearest its own key in key order (its leaf and the
+// neighbouring ones) as candidates.
+hipError_t launch_ix_home(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, const float *bp,
+                          const float *paa, int per, const CandOut &co, hipStream_t stream);
+
 // The routing sample: windows j * (W / samples), j < samples, whose word
 // bound does not exceed the template's bound, added to passed[template].
 hipError_t launch_ix_sample(const TmplDev *tmpl, const int32_t *ids, int count, const float *bp, const float *paa,

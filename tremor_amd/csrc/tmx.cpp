@@ -1325,6 +1325,7 @@ uint64_t waveform_blocks(tmx_ctx *ctx, const tmx_cfg &cfg)
 // and in the plan.
 constexpr double kIndexFraction = 0.01;
 constexpr int kIndexSample = 1 << 16;
+constexpr uint64_t kIndexHome = 4096;  // windows of the approximate search per kNN template
 // The index's own cost on this GPU, seconds per window of a passing leaf (its
 // word bound and the candidate write; measured 5.3e-11 at n = 10^8), and per
 // leaf (one thread's region bound, ~100 instructions).  Auto mode also asks
@@ -1410,6 +1411,43 @@ int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_
     }
     plan.ix_ids = chosen;
     st.index_templates = static_cast<int32_t>(chosen.size());
+    // The approximate search first (the reference visits the query's own
+    // leaf before the others): the kNN templates' nearest windows in key
+    // order, exactly verified; the k-th (2k-1-th separated) smallest of their
+    // distances can only tighten each bound, and the exact search after it
+    // then visits fewer leaves and windows.
+    std::vector<int32_t> knn;
+    for (int d : chosen)
+        if (plan.tmpl[d].k > 0)
+            knn.push_back(d);
+    if (!knn.empty()) {
+        const int per = static_cast<int>(std::min<uint64_t>(kIndexHome, L.cand_cap / knn.size()));
+        LANE_HIP(L.scan_ids.reserve(sizeof(int32_t) * knn.size()), "hipMalloc index ids");
+        LANE_HIP(hipMemcpyAsync(L.scan_ids.p, knn.data(), sizeof(int32_t) * knn.size(), hipMemcpyHostToDevice,
+                                L.stream),
+                 "copy index ids");
+        LANE_HIP(launch_zero_u64(L.cand_count.as<unsigned long long>(), L.stream), "reset counter");
+        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
+        LANE_HIP(launch_ix_home(w, L.tmpl.as<TmplDev>(), L.scan_ids.as<int32_t>(), static_cast<int>(knn.size()),
+                                ctx->sax_bp.as<float>(), L.lb_paa.as<float>(), per, cand_out(L), L.stream),
+                 "index approximate search");
+        unsigned long long nh = 0;
+        LANE_HIP(hipMemcpyAsync(&nh, L.cand_count.p, sizeof(nh), hipMemcpyDeviceToHost, L.stream), "read counter");
+        LANE_HIP(hipStreamSynchronize(L.stream), "index approximate search");
+        nh = std::min<unsigned long long>(nh, L.cand_cap);
+        if (nh > 0) {
+            LANE_HIP(launch_verify(ctx->x.as<float>(), ctx->n, L.tmpl.as<TmplDev>(), L.qz.as<float>(),
+                                   L.order.as<int32_t>(), L.cand.as<Cand>(), nh, L.cand_d.as<float>(),
+                                   cfg.verify_seq != 0, L.stream),
+                     "verify kernel");
+            if (int rc = group_sort(L, L.cand_d.as<float>(), nh, plan.T, kPredFiniteH, kKeyValuePosH))
+                return rc;
+            LANE_HIP(launch_seed_bound(L.keys.as<uint64_t>(), L.offsets.as<uint32_t>(), L.tmpl.as<TmplDev>(), plan.T,
+                                       plan.max_want, L.stream),
+                     "index bound");
+            LANE_HIP(hipStreamSynchronize(L.stream), "index bound");
+        }
+    }
     return 0;
 }
 
@@ -1499,6 +1537,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
             const IxIndex &ix = find_sax(ctx, t.m, 16)->ix;
             t.ix_leaves = ix.leaves.as<IxLeaf>();
             t.ix_ids = ix.ids.as<uint32_t>();
+            t.ix_keys = ix.keys.as<uint64_t>();
             t.ix_nleaves = ix.nleaves;
         }
     }

@@ -53,7 +53,8 @@ struct TmplDev {
     int64_t lb_paa;     // offset of the template's PAA in the lane's PAA array
     const uint8_t *lb_words;  // resident iSAX words of this length (window j at j * lb_seg, or * 16)
     const struct IxLeaf *ix_leaves;  // the index of this length (ix = 1): leaves in key order,
-    const uint32_t *ix_ids;          // the window ids in key order,
+    const uint32_t *ix_ids;          // the window ids in key order, their sorted keys,
+    const uint64_t *ix_keys;
     int64_t ix_nleaves;              // and the leaf count
 };
 
