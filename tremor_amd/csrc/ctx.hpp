@@ -194,6 +194,7 @@ struct FftShape {
 struct IxIndex {
     DevMem keys;    // uint64 [W], sorted
     DevMem ids;     // uint32 [W], the windows in key order
+    DevMem words;   // uint8 [W][16], their words in key order
     DevMem leaves;  // IxLeaf [nleaves], in key order
     int64_t nleaves = 0;
     int leaf_size = 0;

@@ -88,6 +88,17 @@ __global__ __launch_bounds__(kThreads) void k_ix_split(const uint64_t *__restric
         put(next, next_cap, counters, make_uint4(lo, start + count - lo, depth + 1, 0));
 }
 
+// The words in key order (one gather at build): a leaf's words are then
+// consecutive for the search.
+__global__ __launch_bounds__(kThreads) void k_ix_gather_words(const uint8_t *__restrict__ words, uint64_t W,
+                                                              const uint32_t *__restrict__ ids,
+                                                              uint4 *__restrict__ out)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < W)
+        out[i] = *reinterpret_cast<const uint4 *>(words + 16ull * ids[i]);
+}
+
 __global__ __launch_bounds__(kThreads) void k_ix_leaf_keys(const uint4 *__restrict__ leaves, uint32_t n,
                                                            uint32_t *__restrict__ order_key,
                                                            uint32_t *__restrict__ order_val)
@@ -185,10 +196,12 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
     for (uint32_t q = 0; q < n; q++) {
         const IxLeaf L = t.ix_leaves[s_leaf[q]];
         for (uint32_t e = tid; e < L.count; e += kThreads) {
+            // The word first (consecutive in key order), the window's
+            // samples only for a word that passes.
+            if (!ix_pass(sax_lower_bound(bp, paa, t.ix_words + 16ull * (L.start + e), t.m, 16), t.bound))
+                continue;
             const uint32_t j = t.ix_ids[L.start + e];
             if (j >= W || w.x[j] == kMissing)
-                continue;
-            if (!ix_pass(sax_lower_bound(bp, paa, t.lb_words + 16ull * j, t.m, 16), t.bound))
                 continue;
             cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(d), j, NAN, -INFINITY, INFINITY);
         }
@@ -257,7 +270,7 @@ hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxI
     // Keys and ids, then sorted by key (ids along).
     DevMem keys_in, ids_in, tmp;
     if ((e = keys_in.reserve(8 * W)) || (e = ids_in.reserve(4 * W)) || (e = ix.keys.reserve(8 * W)) ||
-        (e = ix.ids.reserve(4 * W)))
+        (e = ix.ids.reserve(4 * W)) || (e = ix.words.reserve(16 * W)))
         return e;
     const unsigned grid = static_cast<unsigned>((W + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_ix_keys, dim3(grid), dim3(kThreads), 0, stream, words, W, keys_in.as<uint64_t>(),
@@ -271,6 +284,8 @@ hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxI
     if ((e = hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, keys_in.as<uint64_t>(), ix.keys.as<uint64_t>(),
                                                 ids_in.as<uint32_t>(), ix.ids.as<uint32_t>(), n, 0, 64, stream)))
         return e;
+    hipLaunchKernelGGL(k_ix_gather_words, dim3(grid), dim3(kThreads), 0, stream, words, W, ix.ids.as<uint32_t>(),
+                       ix.words.as<uint4>());
     // Top-down split, one level per launch, from the root node (depth 0).
     // The nodes of one level are disjoint and non-empty: at most 2^16 of them
     // down to the root depth, then at most twice the nodes over leaf_size

@@ -1538,6 +1538,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
             t.ix_leaves = ix.leaves.as<IxLeaf>();
             t.ix_ids = ix.ids.as<uint32_t>();
             t.ix_keys = ix.keys.as<uint64_t>();
+            t.ix_words = ix.words.as<uint8_t>();
             t.ix_nleaves = ix.nleaves;
         }
     }

@@ -55,6 +55,7 @@ struct TmplDev {
     const struct IxLeaf *ix_leaves;  // the index of this length (ix = 1): leaves in key order,
     const uint32_t *ix_ids;          // the window ids in key order, their sorted keys,
     const uint64_t *ix_keys;
+    const uint8_t *ix_words;         // and their words in key order,
     int64_t ix_nleaves;              // and the leaf count
 };
 
