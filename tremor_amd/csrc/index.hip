@@ -17,7 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "ctx.hpp"
-#include "isax.hpp"
+#include "ea.hpp"
 
 namespace tmx {
 namespace {
@@ -155,19 +155,15 @@ __global__ __launch_bounds__(kThreads) void k_ix_sample(const TmplDev *__restric
 }
 
 // Templates ids[blockIdx.y]: per leaf (one per thread), its bound against the
-// template's bound (the windows of the leaves that pass counted into
-// passed[template]); EMIT: each window of a passing leaf whose own word's
-// bound passes too and whose first sample is present becomes a candidate.
-template <bool EMIT>
-__global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDev *__restrict__ tmpl,
-                                                        const int32_t *__restrict__ ids,
-                                                        const float *__restrict__ bp_g, const float *__restrict__ paa_g,
-                                                        unsigned long long *__restrict__ passed, CandOut co)
+// template's bound; the windows of the leaves that pass counted into
+// passed[template] (the auto rule's cost of the exact search).
+__global__ __launch_bounds__(kThreads) void k_ix_count(const TmplDev *__restrict__ tmpl,
+                                                       const int32_t *__restrict__ ids,
+                                                       const float *__restrict__ bp_g, const float *__restrict__ paa_g,
+                                                       unsigned long long *__restrict__ passed)
 {
     __shared__ float bp[256];
     __shared__ float paa[16];
-    __shared__ uint32_t s_leaf[kThreads];
-    __shared__ uint32_t s_n;
     const int tid = threadIdx.x;
     const int d = ids[blockIdx.y];
     const TmplDev &t = tmpl[d];
@@ -175,35 +171,117 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
         bp[i] = bp_g[i];
     if (tid < 16)
         paa[tid] = paa_g[t.lb_paa + tid];
-    if (tid == 0)
-        s_n = 0;
     __syncthreads();
     const uint32_t leaf = blockIdx.x * kThreads + tid;
-    bool pass = false;
     if (leaf < t.ix_nleaves) {
         const IxLeaf L = t.ix_leaves[leaf];
-        pass = ix_pass(ix_leaf_bound(bp, paa, L.key, L.depth, t.m), t.bound);
-        if (pass)
+        if (ix_pass(ix_leaf_bound(bp, paa, L.key, L.depth, t.m), t.bound))
             atomicAdd(passed + d, static_cast<unsigned long long>(L.count));
     }
-    if (!EMIT)
-        return;  // the whole workgroup: no barrier follows
-    if (pass)
-        s_leaf[atomicAdd(&s_n, 1u)] = leaf;
-    __syncthreads();
-    const uint32_t n = s_n;
-    const uint64_t W = w.n - static_cast<uint64_t>(t.m) + 1;
-    for (uint32_t q = 0; q < n; q++) {
-        const IxLeaf L = t.ix_leaves[s_leaf[q]];
-        for (uint32_t e = tid; e < L.count; e += kThreads) {
-            // The word first (consecutive in key order), the window's
-            // samples only for a word that passes.
-            if (!ix_pass(sax_lower_bound(bp, paa, t.ix_words + 16ull * (L.start + e), t.m, 16), t.bound))
-                continue;
-            const uint32_t j = t.ix_ids[L.start + e];
-            if (j >= W || w.x[j] == kMissing)
-                continue;
-            cand_put(co, atomicAdd(co.count, 1ULL), static_cast<uint32_t>(d), j, NAN, -INFINITY, INFINITY);
+}
+
+constexpr int kIxPer = 8;  // windows of passing leaves per thread and step of the exact search
+
+// The exact search (Tremor.cpp:1195-1518 visits the leaves whose bound
+// passes and computes the distance of each of their windows whose word's
+// bound passes too, with early abandoning), template ids[blockIdx.y].
+// Workgroup b of G = gridDim.x takes the leaves b, b + G, b + 2G, ...: the
+// leaves that pass lie close together in key order (near the template's own
+// key), and the stride spreads them evenly over the workgroups.  Per round of
+// 256 leaves (one per thread) the passing ones' windows are numbered through
+// a workgroup-wide prefix sum, and the workgroup takes 256 kIxPer numbers per
+// step, so every lane has windows while any are left.  Their
+// counts go to passed[template]; each window whose word bound passes goes
+// through the early-abandoning window kernel of the scan pass (ea.hpp), which
+// drops it once its running sum proves the reference's distance above the
+// bound and otherwise makes it a candidate with an interval of that distance.
+__global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDev *__restrict__ tmpl,
+                                                        const int32_t *__restrict__ ids,
+                                                        const float *__restrict__ qz, const int32_t *__restrict__ order,
+                                                        const float *__restrict__ bp_g, const float *__restrict__ paa_g,
+                                                        unsigned long long *__restrict__ passed, CandOut co)
+{
+    __shared__ float bp[256];
+    __shared__ float paa[16];
+    __shared__ uint32_t s_first[kThreads];    // the round's leaves: first sorted index
+    __shared__ uint32_t s_off[kThreads + 1];  // exclusive prefix of the passing leaves' counts (0 if failing)
+    __shared__ uint32_t s_wave[kThreads / 64];
+    __shared__ uint32_t s_sorted[kIxPer][kThreads];  // each thread's windows of the step (sorted positions)
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int d = ids[blockIdx.y];
+    const TmplDev &t = tmpl[d];
+    for (int i = tid; i < 255; i += kThreads)
+        bp[i] = bp_g[i];
+    if (tid < 16)
+        paa[tid] = paa_g[t.lb_paa + tid];
+    const uint64_t G = gridDim.x;
+    const uint64_t nl = static_cast<uint64_t>(t.ix_nleaves);
+    for (uint64_t r = 0; r * G * kThreads < nl; r++) {  // uniform
+        __syncthreads();  // bp / paa written; the last round's s_first / s_off read
+        const uint64_t leaf = (r * kThreads + static_cast<uint64_t>(tid)) * G + blockIdx.x;
+        uint32_t cnt = 0, start = 0;
+        if (leaf < nl) {
+            const IxLeaf L = t.ix_leaves[leaf];
+            if (ix_pass(ix_leaf_bound(bp, paa, L.key, L.depth, t.m), t.bound)) {
+                cnt = L.count;
+                start = L.start;
+            }
+        }
+        // Workgroup-wide exclusive prefix of cnt (the leaves partition the W <
+        // 2^31 windows, so every sum of distinct leaves' counts fits 31 bits).
+        uint32_t incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off);
+            if (lane >= off)
+                incl += v;
+        }
+        if (lane == 63)
+            s_wave[wv] = incl;
+        s_first[tid] = start;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int i = 0; i < wv; i++)
+            before += s_wave[i];
+        s_off[tid] = before + incl - cnt;
+        uint32_t total = 0;
+        for (int i = 0; i < kThreads / 64; i++)
+            total += s_wave[i];
+        if (tid == kThreads - 1)
+            s_off[kThreads] = total;
+        if (tid == 0 && total > 0)
+            atomicAdd(passed + d, static_cast<unsigned long long>(total));
+        __syncthreads();
+        for (uint32_t base = 0; base < total; base += static_cast<uint32_t>(kThreads * kIxPer)) {
+            // Window numbers base + k 256 + tid: consecutive lanes take
+            // consecutive windows in key order (mostly neighbouring
+            // positions, so the window kernel's sample reads coalesce).  The
+            // leaf of number f is the last entry whose offset is <= f
+            // (entries of failing leaves share the next one's offset).
+            uint32_t keep = 0;
+#pragma unroll
+            for (int k = 0; k < kIxPer; k++) {
+                const uint32_t f = base + static_cast<uint32_t>(k * kThreads + tid);
+                int e = 0;
+#pragma unroll
+                for (int step = kThreads / 2; step > 0; step >>= 1)
+                    if (s_off[e + step] <= f)
+                        e += step;
+                const uint32_t sorted = s_first[e] + (f - s_off[e]);
+                s_sorted[k][tid] = sorted;
+                if (f < total && ix_pass(sax_lower_bound(bp, paa, t.ix_words + 16ull * sorted, t.m, 16), t.bound))
+                    keep |= 1u << k;
+            }
+            // One copy of the window kernel, the lanes on the same k at once.
+#pragma unroll 1
+            for (int k = 0; k < kIxPer; k++) {
+                if (keep & (1u << k)) {
+                    // Out-of-range ids and windows starting with a missing
+                    // sample are dropped inside (never searched, as in the scan).
+                    scan_window(w, tmpl, 0, d, t.m, qz, order, kModeMain, 0, co, nullptr, bp, paa, nullptr, 0,
+                                t.ix_ids[s_sorted[k][tid]]);
+                }
+            }
         }
     }
 }
@@ -350,19 +428,36 @@ hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxI
     return hipStreamSynchronize(stream);  // the scratch above is freed on return
 }
 
-hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves,
-                            const float *bp, const float *paa, unsigned long long *passed, bool emit,
-                            const CandOut &co, hipStream_t stream)
+hipError_t launch_ix_count(const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves, const float *bp,
+                           const float *paa, unsigned long long *passed, hipStream_t stream)
 {
     if (count <= 0 || max_leaves <= 0)
         return hipSuccess;
     if (count > 65535 || max_leaves > 0x7fffffffll || !passed)
         return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>((max_leaves + kThreads - 1) / kThreads), static_cast<unsigned>(count));
-    if (emit)
-        hipLaunchKernelGGL(k_ix_search<true>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa, passed, co);
-    else
-        hipLaunchKernelGGL(k_ix_search<false>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, bp, paa, passed, co);
+    hipLaunchKernelGGL(k_ix_count, grid, dim3(kThreads), 0, stream, tmpl, ids, bp, paa, passed);
+    return hipGetLastError();
+}
+
+hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves,
+                            const float *qz, const int32_t *order, const float *bp, const float *paa,
+                            unsigned long long *passed, const CandOut &co, int cus, hipStream_t stream)
+{
+    if (count <= 0 || max_leaves <= 0)
+        return hipSuccess;
+    if (count > 65535 || max_leaves > 0x7fffffffll || !passed)
+        return hipErrorInvalidValue;
+    // Workgroups per template: about 4 per compute unit (all resident at once
+    // at the window kernel's ~100 registers), and no more than a round of
+    // leaves each.  The workgroups run template after template (x before y):
+    // every template's search has the whole device, and the windows in
+    // flight are one template's (16 templates at once, each on 1/16 of the
+    // device, took 3.6x as long at n = 10^8 on their scattered reads).
+    const int64_t want = 4ll * std::max(cus, 1);
+    const int64_t G = std::max<int64_t>(1, std::min<int64_t>(want, (max_leaves + kThreads - 1) / kThreads));
+    hipLaunchKernelGGL(k_ix_search, dim3(static_cast<unsigned>(G), static_cast<unsigned>(count)), dim3(kThreads), 0,
+                       stream, w, tmpl, ids, qz, order, bp, paa, passed, co);
     return hipGetLastError();
 }
 

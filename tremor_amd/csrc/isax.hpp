@@ -146,12 +146,19 @@ hipError_t launch_sax_words(const float *x, uint64_t n, int m, int segments, con
 hipError_t launch_tmpl_paa(const TmplDev *tmpl, int count, const float *qz, float *paa, hipStream_t stream);
 
 // Templates ids[0, count) of the index: per leaf its bound against the
-// template's (the windows of the passing leaves added to passed[template]);
-// emit: each window of a passing leaf whose own bound passes and whose first
-// sample is present becomes a candidate.
+// template's, the windows of the passing leaves added to passed[template].
+hipError_t launch_ix_count(const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves, const float *bp,
+                           const float *paa, unsigned long long *passed, hipStream_t stream);
+
+// The exact index search of templates ids[0, count) (cus: compute units of
+// the device): the windows of the passing leaves added to passed[template]
+// as in launch_ix_count, and each of those windows whose own bound passes
+// goes through the early-abandoning window kernel (a candidate unless its
+// distance is proved above the bound; windows starting with a missing sample
+// are never searched).
 hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves,
-                            const float *bp, const float *paa, unsigned long long *passed, bool emit,
-                            const CandOut &co, hipStream_t stream);
+                            const float *qz, const int32_t *order, const float *bp, const float *paa,
+                            unsigned long long *passed, const CandOut &co, int cus, hipStream_t stream);
 
 // The approximate search before the exact one: for each template ids[i],
 // the `per` windows nearest its own key in key order (its leaf and the

@@ -1189,8 +1189,9 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
         LANE_HIP(launch_zero_f64(L.ix_passed.as<double>(), static_cast<uint64_t>(plan.T), L.stream),
                  "clear index counts");
         LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(plan.ix_ids.size()),
-                                  plan.ix_max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
-                                  L.ix_passed.as<unsigned long long>(), true, cand_out(L), L.stream),
+                                  plan.ix_max_leaves, L.qz.as<float>(), L.order.as<int32_t>(),
+                                  ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
+                                  L.ix_passed.as<unsigned long long>(), cand_out(L), L.cus, L.stream),
                  "iSAX index search");
     }
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
@@ -1326,13 +1327,17 @@ uint64_t waveform_blocks(tmx_ctx *ctx, const tmx_cfg &cfg)
 constexpr double kIndexFraction = 0.01;
 constexpr int kIndexSample = 1 << 16;
 constexpr uint64_t kIndexHome = 4096;  // windows of the approximate search per kNN template
-// The index's own cost on this GPU, seconds per window of a passing leaf (its
-// word bound and the candidate write; measured 5.3e-11 at n = 10^8), and per
-// leaf (one thread's region bound, ~100 instructions).  Auto mode also asks
-// that the index beat 0.9 x the planned form, as the scan must (the
-// reference's rule is set for its CPU costs: on the GPU the dense forms cost
-// picoseconds per window).
-constexpr double kIndexPerWindow = 5.3e-11, kIndexPerLeaf = 1e-12;
+// The index's own cost on this GPU: seconds per window of a passing leaf (its
+// word bound), per window whose word bound passes too (the early-abandoning
+// window kernel), and per leaf (one thread's region bound, ~100
+// instructions).  Measured on the exact search of 16 templates of 256 samples
+// at n = 10^8: threshold rho >= 0.95, 9.89e8 windows in passing leaves and
+// 6.0e7 passing their words, 12.5 ms (1.26e-11 per window all told); kNN k =
+// 1, 2.09e8 and 1.35e8, 18.3 ms (then 1.2e-10 per word that passes).  Auto
+// mode also asks that the index beat 0.9 x the planned form, as the scan must
+// (the reference's rule is set for its CPU costs: on the GPU the dense forms
+// cost picoseconds per window).
+constexpr double kIndexPerWindow = 1.2e-11, kIndexPerSurvivor = 1.2e-10, kIndexPerLeaf = 1e-12;
 
 int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
 {
@@ -1363,9 +1368,11 @@ int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_
         LANE_HIP(hipStreamSynchronize(L.stream), "index sample");
     }
     std::vector<int32_t> chosen;
+    std::vector<double> sampled(plan.T, 0.0);  // fraction of the sample whose word bounds pass
     for (int d : ids) {
         const uint64_t W = ctx->n - static_cast<uint64_t>(plan.tmpl[d].m) + 1;
         const double samples = static_cast<double>(std::min<uint64_t>(kIndexSample, W));
+        sampled[d] = static_cast<double>(passed[d]) / samples;
         if (cfg.index_mode == TMX_INDEX_ALWAYS || static_cast<double>(passed[d]) <= kIndexFraction * samples)
             chosen.push_back(d);
     }
@@ -1379,10 +1386,9 @@ int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_
                  "copy index ids");
         LANE_HIP(launch_zero_f64(L.ix_passed.as<double>(), static_cast<uint64_t>(plan.T), L.stream),
                  "clear index counts");
-        Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
-        LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(chosen.size()),
-                                  max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
-                                  L.ix_passed.as<unsigned long long>(), false, cand_out(L), L.stream),
+        LANE_HIP(launch_ix_count(L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(chosen.size()),
+                                 max_leaves, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
+                                 L.ix_passed.as<unsigned long long>(), L.stream),
                  "index leaf count");
         LANE_HIP(hipMemcpyAsync(passed.data(), L.ix_passed.p, pb, hipMemcpyDeviceToHost, L.stream),
                  "read index counts");
@@ -1390,7 +1396,7 @@ int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_
         std::vector<int32_t> kept;
         for (int d : chosen) {
             const double W = static_cast<double>(ctx->n - static_cast<uint64_t>(plan.tmpl[d].m) + 1);
-            const double ix_s = static_cast<double>(passed[d]) * kIndexPerWindow +
+            const double ix_s = static_cast<double>(passed[d]) * kIndexPerWindow + sampled[d] * W * kIndexPerSurvivor +
                                 static_cast<double>(plan.tmpl[d].ix_nleaves) * kIndexPerLeaf;
             if (ix_s < 0.9 * planned_seconds_per_window(cfg, plan, d) * W)
                 kept.push_back(d);
