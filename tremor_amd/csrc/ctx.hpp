@@ -195,6 +195,7 @@ struct IxIndex {
     DevMem keys;    // uint64 [W], sorted
     DevMem ids;     // uint32 [W], the windows in key order
     DevMem words;   // uint8 [W][16], their words in key order
+    DevMem sums;    // WindowSums [W], their sums in key order (eS1 < 0: first sample missing)
     DevMem leaves;  // IxLeaf [nleaves], in key order
     int64_t nleaves = 0;
     int leaf_size = 0;
@@ -208,8 +209,10 @@ struct SaxWords {
     IxIndex ix;
 };
 
-// Builds the index of W windows from their 16-segment words.
-hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxIndex &ix, hipStream_t stream);
+// Builds the index of the W windows of length m of w from their 16-segment
+// words.
+hipError_t build_isax_index(const Waveform &w, int m, const uint8_t *words, uint64_t W, int leaf_size, IxIndex &ix,
+                            hipStream_t stream);
 
 // Piece spectra of the resident waveform for one shape (B, Lmax).
 struct FftClass {

@@ -55,46 +55,20 @@ __device__ inline double gamma32(double k)
 
 __device__ inline double sq(double v) { return v * v; }
 
-// One window of the early-abandoning scan (index i of the template's
-// windows, or of its sample); true when the lower bound skipped it.
-__device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__restrict__ tmpl, int T, int t, int m,
-                                            const float *__restrict__ qz, const int32_t *__restrict__ order, int mode,
-                                            uint64_t sample_n, const CandOut &co,
-                                            unsigned long long *__restrict__ visited, const float *bp,
-                                            const float *paa, const float *gap, int lb_seg, uint64_t i)
+// The window kernel proper for window j (present, not skipped by a lower
+// bound), from its sums and their error bounds (filter.hpp's, with products
+// by the rounded 1 / m, whose extra roundings the 8 u64 terms cover).
+__device__ __forceinline__ void scan_window_sums(const Waveform &w, const TmplDev *__restrict__ tmpl, int T, int t,
+                                                 int m, const float *__restrict__ qz,
+                                                 const int32_t *__restrict__ order, int mode, const CandOut &co,
+                                                 unsigned long long *__restrict__ visited, uint64_t j,
+                                                 const WindowSums &ws)
 {
     constexpr double u = 5.9604644775390625e-08;
     constexpr double u64 = 1.1102230246251565e-16;
-    const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
-    uint64_t j;
-    if (mode == kModeSample) {
-        if (i >= sample_n)
-            return false;
-        j = (i * W) / sample_n;
-    } else {
-        if (i >= W)
-            return false;
-        j = i;
-    }
-    if (w.x[j] == kMissing)
-        return false;  // never searched (SkipSequential.cpp:256)
-    if (lb_seg > 0) {
-        const uint8_t *word = tmpl[t].lb_words + j * static_cast<uint64_t>(lb_seg);
-        const float lb = lb_seg == 16 ? sax_lower_bound16_table(gap, word, m)
-                                      : sax_lower_bound(bp, paa, word, m, lb_seg);
-        if (lb > tmpl[t].bound) {
-            if (mode == kModeSample)
-                atomicAdd(&visited[T + t], 1ull);  // sampled, no block visited
-            return true;
-        }
-    }
     const int nb = m / kBlock;
     const double md = static_cast<double>(m);
     const double invm = 1.0 / md;
-    // Statistics from the prefix sums with their error bounds (filter.hpp's,
-    // with products by the rounded 1 / m, whose extra roundings the 8 u64
-    // terms cover).
-    const WindowSums ws = window_sums_fast(w, j, j + static_cast<uint64_t>(m));
     const double S1 = ws.S1, S2 = ws.S2, eS1 = ws.eS1, eS2 = ws.eS2;
     const double S1sq = S1 * S1 * invm;
     const double V = S2 - S1sq;
@@ -109,7 +83,7 @@ __device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__
             cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN, -INFINITY,
                      INFINITY);
         }
-        return false;
+        return;
     }
     // sigma = sqrt(V_exact / m) with |V_exact - V| <= eV, rV = eV / V <= 1/4:
     // |sigma / sig - 1| <= rV (sqrt(1 + x) - 1 <= x / 2, 1 - sqrt(1 - x) <= x).
@@ -162,10 +136,10 @@ __device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__
     if (mode == kModeSample) {
         atomicAdd(&visited[t], static_cast<unsigned long long>(abandoned ? b : nb + 1));
         atomicAdd(&visited[T + t], 1ull);
-        return false;
+        return;
     }
     if (abandoned)
-        return false;
+        return;
     float ts = 0.0f;
     for (int k = nb * kBlock; k < m; k++) {
         float rv = r[k];
@@ -179,6 +153,44 @@ __device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__
     const double hi = (1.0 + g_ref) * sq((rs * (1.0 + g_acc + 1.01 * u) + Tm) / (1.0 - u));
     if (!(lo > bound))
         cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), s, lo, hi);
+    return;
+}
+
+
+// One window of the early-abandoning scan (index i of the template's
+// windows, or of its sample); true when the lower bound skipped it.
+__device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__restrict__ tmpl, int T, int t, int m,
+                                            const float *__restrict__ qz, const int32_t *__restrict__ order, int mode,
+                                            uint64_t sample_n, const CandOut &co,
+                                            unsigned long long *__restrict__ visited, const float *bp,
+                                            const float *paa, const float *gap, int lb_seg, uint64_t i)
+{
+    const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
+    uint64_t j;
+    if (mode == kModeSample) {
+        if (i >= sample_n)
+            return false;
+        j = (i * W) / sample_n;
+    } else {
+        if (i >= W)
+            return false;
+        j = i;
+    }
+    if (w.x[j] == kMissing)
+        return false;  // never searched (SkipSequential.cpp:256)
+    if (lb_seg > 0) {
+        const uint8_t *word = tmpl[t].lb_words + j * static_cast<uint64_t>(lb_seg);
+        const float lb = lb_seg == 16 ? sax_lower_bound16_table(gap, word, m)
+                                      : sax_lower_bound(bp, paa, word, m, lb_seg);
+        if (lb > tmpl[t].bound) {
+            if (mode == kModeSample)
+                atomicAdd(&visited[T + t], 1ull);  // sampled, no block visited
+            return true;
+        }
+    }
+    // Statistics from the prefix sums with their error bounds.
+    scan_window_sums(w, tmpl, T, t, m, qz, order, mode, co, visited, j,
+                     window_sums_fast(w, j, j + static_cast<uint64_t>(m)));
     return false;
 }
 

@@ -88,15 +88,28 @@ __global__ __launch_bounds__(kThreads) void k_ix_split(const uint64_t *__restric
         put(next, next_cap, counters, make_uint4(lo, start + count - lo, depth + 1, 0));
 }
 
-// The words in key order (one gather at build): a leaf's words are then
-// consecutive for the search.
-__global__ __launch_bounds__(kThreads) void k_ix_gather_words(const uint8_t *__restrict__ words, uint64_t W,
-                                                              const uint32_t *__restrict__ ids,
-                                                              uint4 *__restrict__ out)
+// The words and the window sums in key order (one gather at build): a
+// leaf's are then consecutive for the search, which reads them coalesced
+// instead of two prefix-sum lines and the first sample per window.  The sums
+// are window_sums_fast's, exactly as the scan computes them; eS1 = -1 marks
+// a window whose first sample is missing.
+__global__ __launch_bounds__(kThreads) void k_ix_gather(Waveform w, int m, const uint8_t *__restrict__ words,
+                                                        uint64_t W, const uint32_t *__restrict__ ids,
+                                                        uint4 *__restrict__ out, WindowSums *__restrict__ sums)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (i < W)
-        out[i] = *reinterpret_cast<const uint4 *>(words + 16ull * ids[i]);
+    if (i >= W)
+        return;
+    const uint64_t j = ids[i];
+    out[i] = *reinterpret_cast<const uint4 *>(words + 16ull * j);
+    WindowSums ws;
+    if (w.x[j] == kMissing) {
+        ws.S1 = ws.S2 = ws.eS2 = 0.0;
+        ws.eS1 = -1.0;
+    } else {
+        ws = window_sums_fast(w, j, j + static_cast<uint64_t>(m));
+    }
+    sums[i] = ws;
 }
 
 __global__ __launch_bounds__(kThreads) void k_ix_leaf_keys(const uint4 *__restrict__ leaves, uint32_t n,
@@ -272,14 +285,15 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
                 if (f < total && ix_pass(sax_lower_bound(bp, paa, t.ix_words + 16ull * sorted, t.m, 16), t.bound))
                     keep |= 1u << k;
             }
-            // One copy of the window kernel, the lanes on the same k at once.
+            // One copy of the window kernel, the lanes on the same k at once,
+            // from the window's sums in key order (coalesced like the words).
 #pragma unroll 1
             for (int k = 0; k < kIxPer; k++) {
                 if (keep & (1u << k)) {
-                    // Out-of-range ids and windows starting with a missing
-                    // sample are dropped inside (never searched, as in the scan).
-                    scan_window(w, tmpl, 0, d, t.m, qz, order, kModeMain, 0, co, nullptr, bp, paa, nullptr, 0,
-                                t.ix_ids[s_sorted[k][tid]]);
+                    const uint32_t sorted = s_sorted[k][tid];
+                    const WindowSums ws = t.ix_sums[sorted];
+                    if (ws.eS1 >= 0.0)  // else its first sample is missing: never searched (as in the scan)
+                        scan_window_sums(w, tmpl, 0, d, t.m, qz, order, kModeMain, co, nullptr, t.ix_ids[sorted], ws);
                 }
             }
         }
@@ -339,16 +353,17 @@ __global__ __launch_bounds__(kThreads) void k_ix_home(Waveform w, const TmplDev 
 
 }  // namespace
 
-hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxIndex &ix, hipStream_t stream)
+hipError_t build_isax_index(const Waveform &w, int m, const uint8_t *words, uint64_t W, int leaf_size, IxIndex &ix,
+                            hipStream_t stream)
 {
-    if (W == 0 || W > 0x7fffffffull || leaf_size < 1)
+    if (W == 0 || W > 0x7fffffffull || leaf_size < 1 || m < 1 || W != w.n - static_cast<uint64_t>(m) + 1)
         return hipErrorInvalidValue;
     const int n = static_cast<int>(W);
     hipError_t e;
     // Keys and ids, then sorted by key (ids along).
     DevMem keys_in, ids_in, tmp;
     if ((e = keys_in.reserve(8 * W)) || (e = ids_in.reserve(4 * W)) || (e = ix.keys.reserve(8 * W)) ||
-        (e = ix.ids.reserve(4 * W)) || (e = ix.words.reserve(16 * W)))
+        (e = ix.ids.reserve(4 * W)) || (e = ix.words.reserve(16 * W)) || (e = ix.sums.reserve(sizeof(WindowSums) * W)))
         return e;
     const unsigned grid = static_cast<unsigned>((W + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_ix_keys, dim3(grid), dim3(kThreads), 0, stream, words, W, keys_in.as<uint64_t>(),
@@ -362,8 +377,8 @@ hipError_t build_isax_index(const uint8_t *words, uint64_t W, int leaf_size, IxI
     if ((e = hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, keys_in.as<uint64_t>(), ix.keys.as<uint64_t>(),
                                                 ids_in.as<uint32_t>(), ix.ids.as<uint32_t>(), n, 0, 64, stream)))
         return e;
-    hipLaunchKernelGGL(k_ix_gather_words, dim3(grid), dim3(kThreads), 0, stream, words, W, ix.ids.as<uint32_t>(),
-                       ix.words.as<uint4>());
+    hipLaunchKernelGGL(k_ix_gather, dim3(grid), dim3(kThreads), 0, stream, w, m, words, W, ix.ids.as<uint32_t>(),
+                       ix.words.as<uint4>(), ix.sums.as<WindowSums>());
     // Top-down split, one level per launch, from the root node (depth 0).
     // The nodes of one level are disjoint and non-empty: at most 2^16 of them
     // down to the root depth, then at most twice the nodes over leaf_size

@@ -632,7 +632,8 @@ int build_sax(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan)
             continue;
         const uint64_t W = ctx->n - static_cast<uint64_t>(t.m) + 1;
         sw->ix.nleaves = 0;
-        LANE_HIP(build_isax_index(sw->words.as<uint8_t>(), W, cfg.leaf_size, sw->ix, L.stream), "iSAX index");
+        const Waveform w{ctx->x.as<float>(), ctx->p.as<double2>(), ctx->pl.as<double2>(), ctx->n};
+        LANE_HIP(build_isax_index(w, t.m, sw->words.as<uint8_t>(), W, cfg.leaf_size, sw->ix, L.stream), "iSAX index");
     }
     return 0;
 }
@@ -1332,12 +1333,12 @@ constexpr uint64_t kIndexHome = 4096;  // windows of the approximate search per 
 // window kernel), and per leaf (one thread's region bound, ~100
 // instructions).  Measured on the exact search of 16 templates of 256 samples
 // at n = 10^8: threshold rho >= 0.95, 9.89e8 windows in passing leaves and
-// 6.0e7 passing their words, 12.5 ms (1.26e-11 per window all told); kNN k =
-// 1, 2.09e8 and 1.35e8, 18.3 ms (then 1.2e-10 per word that passes).  Auto
+// 6.0e7 passing their words, 10.65 ms (1.08e-11 per window all told); kNN k =
+// 1, 2.09e8 and 1.35e8, 12.1 ms (then 7.3e-11 per word that passes).  Auto
 // mode also asks that the index beat 0.9 x the planned form, as the scan must
 // (the reference's rule is set for its CPU costs: on the GPU the dense forms
 // cost picoseconds per window).
-constexpr double kIndexPerWindow = 1.2e-11, kIndexPerSurvivor = 1.2e-10, kIndexPerLeaf = 1e-12;
+constexpr double kIndexPerWindow = 1.1e-11, kIndexPerSurvivor = 7.5e-11, kIndexPerLeaf = 1e-12;
 
 int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
 {
@@ -1545,6 +1546,7 @@ int batch_locked(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan,
             t.ix_ids = ix.ids.as<uint32_t>();
             t.ix_keys = ix.keys.as<uint64_t>();
             t.ix_words = ix.words.as<uint8_t>();
+            t.ix_sums = ix.sums.as<WindowSums>();
             t.ix_nleaves = ix.nleaves;
         }
     }

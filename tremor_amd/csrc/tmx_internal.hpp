@@ -55,7 +55,8 @@ struct TmplDev {
     const struct IxLeaf *ix_leaves;  // the index of this length (ix = 1): leaves in key order,
     const uint32_t *ix_ids;          // the window ids in key order, their sorted keys,
     const uint64_t *ix_keys;
-    const uint8_t *ix_words;         // and their words in key order,
+    const uint8_t *ix_words;         // their words in key order,
+    const struct WindowSums *ix_sums;  // their sums in key order (eS1 < 0: first sample missing),
     int64_t ix_nleaves;              // and the leaf count
 };
 
