@@ -4,9 +4,10 @@ Tremor.cpp:1195-1518 leaf-order search, SAX.cpp:112-170 node mindist).
 Bar: the device build equals the oracle's restatement exactly (the words, the
 window ids in key order, every leaf's start, count and depth), and searches
 through the index return the oracle scan's answers bit for bit - kNN and
-threshold, merge 0 and > 0, missing samples, lengths the index cannot take
-beside ones it does - with the reference's 1% rule choosing the index where
-its leaves prune.
+threshold, merge 0 and > 0, missing samples, flat stretches (one leaf at full
+depth larger than the leaf size), lengths the index cannot take beside ones it
+does - with the reference's 1% rule choosing the index where its leaves
+prune.
 """
 import numpy as np
 import pytest
@@ -33,6 +34,7 @@ def test_index_build_equals_oracle(oracle, m, leaf_size):
     from tremor_amd.capi import Context
     x, _, _ = _wave(1 << 17, 71)
     x[5000:5100] = MISSING
+    x[20000:23000] = 0.25  # a flat stretch: thousands of windows with one word, a leaf at full depth
     _log("index build", m=m, leaf_size=leaf_size)
     with Context() as ctx:
         ctx.load_waveform(x)
@@ -52,6 +54,10 @@ def test_index_answers_equal_scan(oracle, prec):
     from tremor_amd.capi import Context
     x, onsets, rng = _wave(1 << 18, 73)
     x[70000:70400] = MISSING
+    # A flat stretch: near-constant windows are undecidable for the early
+    # abandon (always candidates), and over 2,048 windows of length 64 share
+    # one word, so one leaf at full depth spans several steps of the search.
+    x[120000:123000] = -0.5
     lengths = [64, 96, 100, 128, 250, 256, 512]
     tm = synth.cut_templates(x, onsets, lengths, rng)
     eligible = sum(1 for m in lengths if m % 16 == 0)
