@@ -120,3 +120,29 @@ def test_index_rebuilt_for_another_leaf_size(oracle):
             pos, d, st = ctx.knn(tm, 5, 0)
             assert st["index_templates"] == 3
             assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
+
+
+def test_more_lengths_than_the_words_cache(oracle):
+    """A batch with more distinct lengths than the resident words keep (8):
+    the words and indexes of every length the batch uses stay resident while
+    the others are built, through the index and through the skip-sequential
+    bound alike."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(1 << 16, 89, events=20)
+    lengths = [64 + 16 * i for i in range(11)]
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    po, do, _ = oracle.scan_knn(x, tm, 3, 0)
+    tt, tp, td = oracle.scan_threshold(x, tm, 0.7, 0)
+    for kw in ({"index": "always", "leaf_size": 200}, {"scan": "always", "lower_bounds": True}):
+        with Context(**kw) as ctx:
+            ctx.load_waveform(x)
+            for rep in range(2):  # the second call finds everything resident
+                _log("many lengths", rep=rep, **kw)
+                pos, d, st = ctx.knn(tm, 3, 0)
+                assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
+                r = ctx.threshold(tm, 0.7, 0)
+                assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+                assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+            if "index" in kw:
+                assert st["index_templates"] == len(lengths)

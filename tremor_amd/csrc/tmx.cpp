@@ -593,7 +593,9 @@ int build_fft_class(tmx_ctx *ctx, Lane &L, int Lt)
 
 // iSAX words for every lower-bound template length of the plan that has
 // none yet (SkipSequential.cpp:155-190, on the device), keeping at most
-// kSaxKept lengths (the oldest are dropped first; no batch runs meanwhile).
+// kSaxKept lengths: the oldest that the plan does not use are dropped first
+// (no batch runs meanwhile), and a plan that uses more lengths keeps them
+// all.
 constexpr size_t kSaxKept = 8;
 
 int build_sax(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan)
@@ -610,8 +612,16 @@ int build_sax(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan)
         const int seg = words_segments(t);
         if (seg <= 0 || find_sax(ctx, t.m, seg))
             continue;
-        while (ctx->sax.size() >= kSaxKept)
-            ctx->sax.erase(ctx->sax.begin());
+        while (ctx->sax.size() >= kSaxKept) {
+            const auto unused = std::find_if(ctx->sax.begin(), ctx->sax.end(), [&](const auto &sw) {
+                return std::none_of(plan.tmpl.begin(), plan.tmpl.end(), [&](const TmplDev &u) {
+                    return u.m == sw->m && words_segments(u) == sw->segments;
+                });
+            });
+            if (unused == ctx->sax.end())
+                break;
+            ctx->sax.erase(unused);
+        }
         auto w = std::make_unique<SaxWords>();
         w->m = t.m;
         w->segments = seg;
