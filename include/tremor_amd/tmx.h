@@ -252,7 +252,8 @@ int tmx_fft_probe(tmx_ctx *ctx, int32_t B, int32_t kind, const float *in, int32_
  * window count and depth (key bits its windows share), in key order, into the
  * leaf arrays (leaf_cap entries; the leaves beyond are not copied); *nleaves
  * gets the leaf count.  Also copies the 16 symbols of every window into words
- * when words is not NULL (W x 16 bytes). */
+ * when words is not NULL (ids_cap x 16 bytes, ids_cap >= W, also when ids is
+ * NULL). */
 int tmx_index_probe(tmx_ctx *ctx, int32_t m, int32_t leaf_size, uint32_t *ids, uint64_t ids_cap,
                     uint32_t *leaf_start, uint32_t *leaf_count, int32_t *leaf_depth, int64_t leaf_cap,
                     int64_t *nleaves, uint8_t *words);

@@ -2312,8 +2312,8 @@ int tmx_index_probe(tmx_ctx *ctx, int32_t m, int32_t leaf_size, uint32_t *ids, u
     if (m <= 0 || !ix_eligible(cfg, m, ctx->n))
         return ctx->fail("index probe: the length must be a multiple of 16, at most n, with fewer than 2^31 windows");
     const uint64_t W = ctx->n - static_cast<uint64_t>(m) + 1;
-    if (ids && ids_cap < W)
-        return ctx->fail("index probe: ids holds fewer than n - m + 1 entries");
+    if ((ids || words) && ids_cap < W)
+        return ctx->fail("index probe: ids / words hold fewer than n - m + 1 entries");
     Lane &L = ctx->direct;
     if (hipSetDevice(ctx->cfg.device) != hipSuccess)
         return ctx->fail("hipSetDevice failed", -2);
