@@ -7,7 +7,8 @@ through the index return the oracle scan's answers bit for bit - kNN and
 threshold, merge 0 and > 0, missing samples, flat stretches (one leaf at full
 depth larger than the leaf size), lengths the index cannot take beside ones it
 does - with the reference's 1% rule choosing the index where its leaves
-prune.
+prune; and, on the 16-segment fixture cases written from the reference build
+(the plain workload at full size among them), the reference's own answers.
 """
 import numpy as np
 import pytest
@@ -146,3 +147,48 @@ def test_more_lengths_than_the_words_cache(oracle):
                 assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
             if "index" in kw:
                 assert st["index_templates"] == len(lengths)
+
+
+@pytest.mark.parametrize("name", ["validate16", "c0"])
+def test_index_equals_reference_fixtures(oracle, name):
+    """The index search against the reference's own answers on the
+    16-segment fixture cases (tests/golden/sss_lb_vectors.npz, written from
+    the reference build: its exact skip-sequential search, whose answers the
+    reference's index search returns too) - the plain workload at its full
+    size among them - and bit for bit against the oracle."""
+    import os
+    import sys
+    from tremor_amd.capi import Context
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_sss_lb_golden as G
+    lbg = np.load(os.path.join(here, "golden", "sss_lb_vectors.npz"), allow_pickle=False)
+    _, x, tm, m, seg, k, rhos, merges, _full = {c[0]: c for c in G.cases()}[name]
+    assert seg == 16
+
+    def fmt(v):
+        return "%f" % float(v)
+
+    with Context(precision="bf16x3", index="always", leaf_size=2000) as ctx:
+        ctx.load_waveform(x)
+        for merge in merges:
+            _log("index-knn-ref", case=name, merge=merge)
+            pos, d, st = ctx.knn(tm, k, merge)
+            assert st["index_templates"] == len(tm)
+            kk = f"{name}/knn{merge}"
+            keep = d.ravel() < np.finfo(np.float32).max
+            assert np.array_equal(np.repeat(np.arange(len(tm)), k)[keep], lbg[kk + "/tid"])
+            assert np.array_equal(pos.ravel()[keep], lbg[kk + "/pos"])
+            assert [fmt(oracle.ed_to_cc(v, m)) for v in d.ravel()[keep]] == list(lbg[kk + "/corr"])
+            po, do, _ = oracle.scan_knn(x, tm, k, merge)
+            assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
+            for i, rho in enumerate(rhos):
+                _log("index-threshold-ref", case=name, rho=rho, merge=merge)
+                r = ctx.threshold(tm, rho, merge)
+                assert r.stats["index_templates"] == len(tm)
+                key = f"{name}/thr{merge}_{i}"
+                assert np.array_equal(r.tmpl, lbg[key + "/q"]) and np.array_equal(r.pos, lbg[key + "/pos"])
+                assert [fmt(oracle.ed_to_cc(v, m)) for v in r.d] == list(lbg[key + "/corr"])
+                tt, tp, td = oracle.scan_threshold(x, tm, rho, merge)
+                assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+                assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
