@@ -182,3 +182,42 @@ def test_serving_errors():
         assert r2 is not None and r2["status"] == 0
         with pytest.raises(TmxError):
             ctx.submit(x[:100], "knn", 10)        # stopped
+
+
+def test_serving_index_many_lengths(oracle):
+    """Serving with every template on the iSAX index, requests of 24
+    distinct lengths over two lanes: the lanes' batches need more lengths
+    between them than the resident words keep, so they drop each other's
+    words and indexes; every request still finishes with the answer it gets
+    alone and the oracle's."""
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(1 << 16, 13, 20)
+    reqs = []
+    for i in range(48):
+        m = 64 + 16 * (i % 24)
+        s = int(min(onsets[rng.integers(0, onsets.size)], x.size - m))
+        t = (x[s:s + m].astype(np.float64) + 0.1 * rng.standard_normal(m)).astype(np.float32)
+        if i % 3:
+            reqs.append(dict(template=t, kind="knn", k=int(rng.choice([1, 5])), merge=0, rho=0.0))
+        else:
+            reqs.append(dict(template=t, kind="threshold", k=0, merge=0, rho=0.8))
+    with Context(index="always", leaf_size=300) as ctx:
+        ctx.load_waveform(x)
+        _log("serving_index", requests=len(reqs))
+        ctx.serve_start(lanes=2, max_batch=6)
+        tickets = [ctx.submit(r["template"], r["kind"], r["k"], r["rho"], r["merge"]) for r in reqs]
+        res = [ctx.wait(t, timeout=120.0) for t in tickets]
+        ctx.serve_stop()
+        assert all(r is not None and r["status"] == 0 for r in res), [r and r["error"] for r in res]
+        for i, (r, got) in enumerate(zip(reqs, res)):
+            if i % 4:
+                continue
+            _log("serving_index_alone", i=i, kind=r["kind"], m=r["template"].size)
+            assert _same((got["pos"], got["d"]), _alone(ctx, r)), i
+    for i, (r, got) in enumerate(zip(reqs, res)):
+        if r["kind"] == "knn":
+            po, do, _ = oracle.scan_knn(x, [r["template"]], r["k"], r["merge"])
+            assert _same((got["pos"], got["d"]), (po[0], do[0])), i
+        else:
+            tt, tp, td = oracle.scan_threshold(x, [r["template"]], r["rho"], r["merge"])
+            assert _same((got["pos"], got["d"]), (tp, td)), i

@@ -1749,28 +1749,41 @@ int run_batch_once(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, in
     int rc = build_plan(ctx, L, cfg, q, count, plan, raw_packed);
     if (rc)
         return rc;
+    // The batch itself, with the resident structures held (shared, or
+    // exclusively after repeated builds).
+    const auto run_held = [&]() {
+        st.comparisons = plan.comparisons;
+        const int brc = batch_locked(ctx, L, cfg, plan, raw_packed, out, st);
+        if (brc)
+            return brc;
+        st.seconds_total = now_s() - t0;
+        if (cfg.verbose)
+            std::fprintf(stderr,
+                         "tmx batch: T=%d seed %.3f ms main %.3f ms (%lld cand) verify %.3f ms select %.3f ms\n",
+                         count, 1e3 * st.seconds_seed, 1e3 * st.seconds_main, static_cast<long long>(st.candidates),
+                         1e3 * st.seconds_verify, 1e3 * st.seconds_select);
+        if (stats)
+            *stats = st;
+        return 0;
+    };
+    int builds = 0;
     while (!resident_ready(ctx, cfg, plan)) {
         rl.unlock();
         {
             WriteResident wl(ctx);
             rc = build_resident(ctx, L, cfg, plan);
+            if (rc)
+                return rc;
+            // Lanes whose batches use more template lengths between them
+            // than the words cache keeps can drop each other's words between
+            // a build and the next shared hold.  After a second build the
+            // batch runs under this exclusive hold, so every batch finishes.
+            if (++builds >= 2 && resident_ready(ctx, cfg, plan))
+                return run_held();
         }
-        if (rc)
-            return rc;
         rl = read_resident(ctx);
     }
-    st.comparisons = plan.comparisons;
-    rc = batch_locked(ctx, L, cfg, plan, raw_packed, out, st);
-    if (rc)
-        return rc;
-    st.seconds_total = now_s() - t0;
-    if (cfg.verbose)
-        std::fprintf(stderr, "tmx batch: T=%d seed %.3f ms main %.3f ms (%lld cand) verify %.3f ms select %.3f ms\n",
-                     count, 1e3 * st.seconds_seed, 1e3 * st.seconds_main, static_cast<long long>(st.candidates),
-                     1e3 * st.seconds_verify, 1e3 * st.seconds_select);
-    if (stats)
-        *stats = st;
-    return 0;
+    return run_held();
 }
 
 void add_stats(tmx_stats &a, const tmx_stats &b)
