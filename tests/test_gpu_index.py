@@ -192,3 +192,25 @@ def test_index_equals_reference_fixtures(oracle, name):
                 tt, tp, td = oracle.scan_threshold(x, tm, rho, merge)
                 assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
                 assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
+
+
+def test_auto_stays_dense_where_the_index_does_not_fit(oracle):
+    """At 2^31 - 64 samples the index of one length would need about 200 GB
+    while it is built: auto mode keeps the template on the dense forms (the
+    call answers instead of failing on device memory)."""
+    from tremor_amd.capi import Context
+    n = (1 << 31) - 64
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n, dtype=np.float32)
+    j = 123457
+    t = x[j:j + 256].copy()
+    with Context(index="auto") as ctx:
+        ctx.load_waveform(x)
+        _log("index auto at 2^31")
+        pos, d, st = ctx.knn([t], 1, 0)
+    assert st["index_templates"] == 0
+    assert pos[0][0] == j  # the template's own window, distance ~0
+    # the oracle on the neighbourhood: the same distance as the reference's kernel there
+    lo = j - 1000
+    po, do, _ = oracle.scan_knn(x[lo:j + 1256], [t], 1, 0)
+    assert po[0][0] + lo == j and np.array_equal(d[0].view(np.uint32), do[0].view(np.uint32))

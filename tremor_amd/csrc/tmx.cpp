@@ -271,6 +271,13 @@ bool plan_has_lb(const CallPlan &plan)
 // A template length the iSAX index applies to: 16 PAA segments (the index's
 // keys; also the lower bound's segments when those are on), a whole number
 // of them, and fewer than 2^31 windows.
+// Device bytes per window while the index of one length is built (its words
+// 16, keys 8, ids 4, words 16 and sums 32 in key order, and the sort's
+// unsorted keys and ids with their scratch), an upper estimate.
+constexpr uint64_t kIxBuildBytes = 100;
+
+SaxWords *find_sax(tmx_ctx *ctx, int m, int segments);
+
 bool ix_eligible(const tmx_cfg &cfg, int m, uint64_t n)
 {
     return cfg.index_mode != TMX_INDEX_OFF && (!cfg.lower_bounds || cfg.paa_segments == 16) && m % 16 == 0 &&
@@ -348,14 +355,44 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
     plan.afrag_len = aoff;
     // Skip-sequential lower bounds: templates whose length is a whole number
     // of segments; their PAA at d * kMaxSegments in the lane's array, their
-    // words bound once the resident words exist (bind_sax).
+    // words bound once the resident words exist (bind_sax).  The index:
+    // eligible lengths, in auto mode only those whose index is built or fits
+    // (a length without one needs about kIxBuildBytes per window while it is
+    // built: its words, keys, ids, words and sums in key order, and the
+    // sort's scratch) within 80% of the device's free memory, so that a call
+    // at a size the index cannot take runs on the dense forms instead of
+    // failing.
+    std::vector<std::pair<int, bool>> ix_fit;  // length -> whether auto mode takes it
+    size_t free_b = 0;
+    bool have_free = false;
+    uint64_t ix_planned = 0;
+    const auto ix_takes = [&](int m) {
+        if (cfg.index_mode != TMX_INDEX_AUTO)
+            return true;
+        for (const auto &f : ix_fit)
+            if (f.first == m)
+                return f.second;
+        const SaxWords *sw = find_sax(ctx, m, 16);
+        bool fits = sw && sw->ix.nleaves > 0 && sw->ix.leaf_size == cfg.leaf_size;
+        if (!fits) {
+            size_t total_b = 0;
+            if (!have_free)
+                have_free = hipMemGetInfo(&free_b, &total_b) == hipSuccess;
+            const uint64_t need = kIxBuildBytes * (ctx->n - static_cast<uint64_t>(m) + 1);
+            fits = have_free && static_cast<double>(ix_planned + need) <= 0.8 * static_cast<double>(free_b);
+            if (fits)
+                ix_planned += need;
+        }
+        ix_fit.emplace_back(m, fits);
+        return fits;
+    };
     for (int d = 0; d < count; d++) {
         TmplDev &t = plan.tmpl[d];
         if (cfg.lower_bounds && lb_eligible(cfg, t.m, ctx->n)) {
             t.lb_seg = cfg.paa_segments;
             t.lb_paa = static_cast<int64_t>(d) * kMaxSegments;
         }
-        if (ix_eligible(cfg, t.m, ctx->n)) {
+        if (ix_eligible(cfg, t.m, ctx->n) && ix_takes(t.m)) {
             t.ix = 1;
             t.lb_paa = static_cast<int64_t>(d) * kMaxSegments;
         }
