@@ -220,6 +220,7 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
     __shared__ uint32_t s_off[kThreads + 1];  // exclusive prefix of the passing leaves' counts (0 if failing)
     __shared__ uint32_t s_wave[kThreads / 64];
     __shared__ uint32_t s_sorted[kIxPer][kThreads];  // each thread's windows of the step (sorted positions)
+    __shared__ float gap[16 * 256];  // gap[256 s + v] = sax_gap(bp, paa[s], v): a word's bound is 16 reads
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int d = ids[blockIdx.y];
@@ -228,10 +229,13 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
         bp[i] = bp_g[i];
     if (tid < 16)
         paa[tid] = paa_g[t.lb_paa + tid];
+    __syncthreads();
+    for (int e = tid; e < 16 * 256; e += kThreads)
+        gap[e] = sax_gap(bp, paa[e >> 8], e & 255);
     const uint64_t G = gridDim.x;
     const uint64_t nl = static_cast<uint64_t>(t.ix_nleaves);
     for (uint64_t r = 0; r * G * kThreads < nl; r++) {  // uniform
-        __syncthreads();  // bp / paa written; the last round's s_first / s_off read
+        __syncthreads();  // the gap table written; the last round's s_first / s_off read
         const uint64_t leaf = (r * kThreads + static_cast<uint64_t>(tid)) * G + blockIdx.x;
         uint32_t cnt = 0, start = 0;
         if (leaf < nl) {
@@ -282,7 +286,7 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
                         e += step;
                 const uint32_t sorted = s_first[e] + (f - s_off[e]);
                 s_sorted[k][tid] = sorted;
-                if (f < total && ix_pass(sax_lower_bound(bp, paa, t.ix_words + 16ull * sorted, t.m, 16), t.bound))
+                if (f < total && ix_pass(sax_lower_bound16_table(gap, t.ix_words + 16ull * sorted, t.m), t.bound))
                     keep |= 1u << k;
             }
             // One copy of the window kernel, the lanes on the same k at once,

@@ -1380,12 +1380,12 @@ constexpr uint64_t kIndexHome = 4096;  // windows of the approximate search per 
 // window kernel), and per leaf (one thread's region bound, ~100
 // instructions).  Measured on the exact search of 16 templates of 256 samples
 // at n = 10^8: threshold rho >= 0.95, 9.89e8 windows in passing leaves and
-// 6.0e7 passing their words, 10.65 ms (1.08e-11 per window all told); kNN k =
-// 1, 2.09e8 and 1.35e8, 12.1 ms (then 7.3e-11 per word that passes).  Auto
+// 6.0e7 passing their words, 8.3 ms (8.4e-12 per window all told); kNN k = 1,
+// 2.09e8 and 1.35e8, 12.05 ms (then 7.6e-11 per word that passes).  Auto
 // mode also asks that the index beat 0.9 x the planned form, as the scan must
 // (the reference's rule is set for its CPU costs: on the GPU the dense forms
 // cost picoseconds per window).
-constexpr double kIndexPerWindow = 1.1e-11, kIndexPerSurvivor = 7.5e-11, kIndexPerLeaf = 1e-12;
+constexpr double kIndexPerWindow = 8.5e-12, kIndexPerSurvivor = 7.6e-11, kIndexPerLeaf = 1e-12;
 
 int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
 {
