@@ -214,3 +214,28 @@ def test_auto_stays_dense_where_the_index_does_not_fit(oracle):
     lo = j - 1000
     po, do, _ = oracle.scan_knn(x[lo:j + 1256], [t], 1, 0)
     assert po[0][0] + lo == j and np.array_equal(d[0].view(np.uint32), do[0].view(np.uint32))
+
+
+def test_index_with_lower_bounds_and_forms_mixed(oracle):
+    """The index beside the skip-sequential bound (both on the 16-segment
+    words of a length) and beside the FFT form: lengths the index takes go
+    through it, the others through the bounded scan or the FFT form; the
+    answers are the oracle's."""
+    from tremor_amd import synth
+    from tremor_amd.capi import Context
+    x, onsets, rng = _wave(1 << 17, 97)
+    lengths = [64, 100, 160, 250, 256, 1000]
+    tm = synth.cut_templates(x, onsets, lengths, rng)
+    po, do, _ = oracle.scan_knn(x, tm, 4, 8)
+    tt, tp, td = oracle.scan_threshold(x, tm, 0.75, 0)
+    for kw in ({"scan": "always"}, {"fft": "always"}):
+        with Context(index="always", lower_bounds=True, paa_segments=16, leaf_size=400, **kw) as ctx:
+            ctx.load_waveform(x)
+            _log("index mixed", **kw)
+            pos, d, st = ctx.knn(tm, 4, 8)
+            assert st["index_templates"] == 3  # 64, 160, 256
+            assert np.array_equal(pos, po) and np.array_equal(d.view(np.uint32), do.view(np.uint32))
+            r = ctx.threshold(tm, 0.75, 0)
+            assert r.stats["index_templates"] == 3
+            assert np.array_equal(r.tmpl, tt) and np.array_equal(r.pos, tp)
+            assert np.array_equal(r.d.view(np.uint32), td.view(np.uint32))
