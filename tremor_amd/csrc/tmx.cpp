@@ -361,11 +361,22 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
     // built: its words, keys, ids, words and sums in key order, and the
     // sort's scratch) within 80% of the device's free memory, so that a call
     // at a size the index cannot take runs on the dense forms instead of
-    // failing.
-    std::vector<std::pair<int, bool>> ix_fit;  // length -> whether auto mode takes it
+    // failing.  The same for the skip-sequential words when the scan is only
+    // a candidate form (scan auto): lengths whose words are resident or fit;
+    // none when the scan is off, since nothing would read them.
+    std::vector<std::pair<int, bool>> ix_fit, lb_fit;  // length -> whether it is taken
     size_t free_b = 0;
     bool have_free = false;
-    uint64_t ix_planned = 0;
+    uint64_t planned = 0;  // device bytes the new structures of this call need
+    const auto fits_memory = [&](uint64_t need) {
+        size_t total_b = 0;
+        if (!have_free)
+            have_free = hipMemGetInfo(&free_b, &total_b) == hipSuccess;
+        const bool ok = have_free && static_cast<double>(planned + need) <= 0.8 * static_cast<double>(free_b);
+        if (ok)
+            planned += need;
+        return ok;
+    };
     const auto ix_takes = [&](int m) {
         if (cfg.index_mode != TMX_INDEX_AUTO)
             return true;
@@ -373,22 +384,28 @@ int build_plan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const Query *q, int co
             if (f.first == m)
                 return f.second;
         const SaxWords *sw = find_sax(ctx, m, 16);
-        bool fits = sw && sw->ix.nleaves > 0 && sw->ix.leaf_size == cfg.leaf_size;
-        if (!fits) {
-            size_t total_b = 0;
-            if (!have_free)
-                have_free = hipMemGetInfo(&free_b, &total_b) == hipSuccess;
-            const uint64_t need = kIxBuildBytes * (ctx->n - static_cast<uint64_t>(m) + 1);
-            fits = have_free && static_cast<double>(ix_planned + need) <= 0.8 * static_cast<double>(free_b);
-            if (fits)
-                ix_planned += need;
-        }
+        const bool fits = (sw && sw->ix.nleaves > 0 && sw->ix.leaf_size == cfg.leaf_size) ||
+                          fits_memory(kIxBuildBytes * (ctx->n - static_cast<uint64_t>(m) + 1));
         ix_fit.emplace_back(m, fits);
+        return fits;
+    };
+    const auto lb_takes = [&](int m) {
+        if (cfg.scan_mode == TMX_SCAN_OFF)
+            return false;
+        if (cfg.scan_mode == TMX_SCAN_ALWAYS)
+            return true;
+        for (const auto &f : lb_fit)
+            if (f.first == m)
+                return f.second;
+        const uint64_t seg = static_cast<uint64_t>(cfg.paa_segments);
+        const bool fits = find_sax(ctx, m, cfg.paa_segments) ||
+                          fits_memory(seg * (ctx->n - static_cast<uint64_t>(m) + 1));
+        lb_fit.emplace_back(m, fits);
         return fits;
     };
     for (int d = 0; d < count; d++) {
         TmplDev &t = plan.tmpl[d];
-        if (cfg.lower_bounds && lb_eligible(cfg, t.m, ctx->n)) {
+        if (cfg.lower_bounds && lb_eligible(cfg, t.m, ctx->n) && lb_takes(t.m)) {
             t.lb_seg = cfg.paa_segments;
             t.lb_paa = static_cast<int64_t>(d) * kMaxSegments;
         }
