@@ -169,6 +169,9 @@ int tmx_cfg_default(tmx_cfg *cfg);
 int tmx_create(tmx_ctx **out, const tmx_cfg *cfg);
 void tmx_destroy(tmx_ctx *ctx);
 const char *tmx_last_error(const tmx_ctx *ctx);
+/* The setters below change the context's settings for later synchronous
+ * calls and for serving loops started later; a running serving loop keeps
+ * the settings it started with. */
 int tmx_set_precision(tmx_ctx *ctx, int32_t precision);
 int tmx_set_fft_mode(tmx_ctx *ctx, int32_t mode);
 int tmx_set_scan_mode(tmx_ctx *ctx, int32_t mode);
