@@ -95,3 +95,9 @@ def test_csv_outputs_match_reference_format(cli, oracle, tmp_path):
             counter[int(t)] = c + 1
             exp.append("%d, %d, %d, %f" % (int(t), c, int(p), float(_corr(dd, m))))
         assert (tmp_path / f"thr{merge}_0.csv").read_text() == "\n".join(exp) + "\n"
+        # The same rows through the tremor method.
+        prefix_t = tmp_path / f"thr{merge}_tremor"
+        r = _run(cli, "--dataset", x, "--queries", q, "--window-length", m, "--mode", "threshold",
+                 "--threshold", "0.4", "--merge-offset", merge, "--output", prefix_t, "--method", "tremor")
+        assert r.returncode == 0, r.stderr
+        assert (tmp_path / f"thr{merge}_tremor_0.csv").read_text() == "\n".join(exp) + "\n"
