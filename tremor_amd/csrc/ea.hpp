@@ -57,12 +57,15 @@ __device__ inline double sq(double v) { return v * v; }
 
 // The window kernel proper for window j (present, not skipped by a lower
 // bound), from its sums and their error bounds (filter.hpp's, with products
-// by the rounded 1 / m, whose extra roundings the 8 u64 terms cover).
-__device__ __forceinline__ void scan_window_sums(const Waveform &w, const TmplDev *__restrict__ tmpl, int T, int t,
-                                                 int m, const float *__restrict__ qz,
-                                                 const int32_t *__restrict__ order, int mode, const CandOut &co,
-                                                 unsigned long long *__restrict__ visited, uint64_t j,
-                                                 const WindowSums &ws)
+// by the rounded 1 / m, whose extra roundings the 8 u64 terms cover), against
+// bound_f (the template's bound, or a tighter valid one).  Returns an upper
+// bound of the reference's distance, rounded up to float, for a window that
+// became a candidate with a finite interval, else +inf.
+__device__ __forceinline__ float scan_window_sums(const Waveform &w, const TmplDev *__restrict__ tmpl, int T, int t,
+                                                  int m, const float *__restrict__ qz,
+                                                  const int32_t *__restrict__ order, int mode, const CandOut &co,
+                                                  unsigned long long *__restrict__ visited, uint64_t j,
+                                                  const WindowSums &ws, float bound_f)
 {
     constexpr double u = 5.9604644775390625e-08;
     constexpr double u64 = 1.1102230246251565e-16;
@@ -83,7 +86,7 @@ __device__ __forceinline__ void scan_window_sums(const Waveform &w, const TmplDe
             cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), NAN, -INFINITY,
                      INFINITY);
         }
-        return;
+        return INFINITY;
     }
     // sigma = sqrt(V_exact / m) with |V_exact - V| <= eV, rV = eV / V <= 1/4:
     // |sigma / sig - 1| <= rV (sqrt(1 + x) - 1 <= x / 2, 1 - sqrt(1 - x) <= x).
@@ -102,7 +105,7 @@ __device__ __forceinline__ void scan_window_sums(const Waveform &w, const TmplDe
     const double Tm = (e_sig + 5.0 * u + 1e-9) * rm + rm * (dmu + dmu_ref) * 1.001 * inv_lo;
     const double g_acc = gamma32(static_cast<double>(nb + m % kBlock + 34));
     const double g_ref = gamma32(floor(md / 16.0) + 40.0);
-    const double bound = static_cast<double>(tmpl[t].bound);
+    const double bound = static_cast<double>(bound_f);
     const double shrink = 1.0 - g_acc - 1.01 * u;
     float thr = INFINITY;
     if (bound < 3.0e38) {
@@ -136,10 +139,10 @@ __device__ __forceinline__ void scan_window_sums(const Waveform &w, const TmplDe
     if (mode == kModeSample) {
         atomicAdd(&visited[t], static_cast<unsigned long long>(abandoned ? b : nb + 1));
         atomicAdd(&visited[T + t], 1ull);
-        return;
+        return INFINITY;
     }
     if (abandoned)
-        return;
+        return INFINITY;
     float ts = 0.0f;
     for (int k = nb * kBlock; k < m; k++) {
         float rv = r[k];
@@ -151,9 +154,14 @@ __device__ __forceinline__ void scan_window_sums(const Waveform &w, const TmplDe
     const double rs = sqrt(static_cast<double>(s));
     const double lo = (1.0 - g_ref) * sq(fmax(0.0, rs * shrink - Tm) / (1.0 + u));
     const double hi = (1.0 + g_ref) * sq((rs * (1.0 + g_acc + 1.01 * u) + Tm) / (1.0 - u));
-    if (!(lo > bound))
-        cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), s, lo, hi);
-    return;
+    if (lo > bound)
+        return INFINITY;
+    cand_put(co, wave_claim(co.count), static_cast<uint32_t>(t), static_cast<uint32_t>(j), s, lo, hi);
+    // hi >= 0: the next float up is the next bit pattern.
+    float hf = static_cast<float>(hi);
+    if (static_cast<double>(hf) < hi)
+        hf = __uint_as_float(__float_as_uint(hf) + 1u);
+    return hf;
 }
 
 
@@ -190,7 +198,7 @@ __device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__
     }
     // Statistics from the prefix sums with their error bounds.
     scan_window_sums(w, tmpl, T, t, m, qz, order, mode, co, visited, j,
-                     window_sums_fast(w, j, j + static_cast<uint64_t>(m)));
+                     window_sums_fast(w, j, j + static_cast<uint64_t>(m)), tmpl[t].bound);
     return false;
 }
 

@@ -195,6 +195,16 @@ __global__ __launch_bounds__(kThreads) void k_ix_count(const TmplDev *__restrict
 
 constexpr int kIxPer = 8;  // windows of passing leaves per thread and step of the exact search
 
+// The running bounds of the exact search start at the templates' bounds.
+__global__ __launch_bounds__(kThreads) void k_ix_bound_init(const TmplDev *__restrict__ tmpl,
+                                                            const int32_t *__restrict__ ids, int count,
+                                                            unsigned int *__restrict__ dyn)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < count)
+        dyn[ids[i]] = __float_as_uint(tmpl[ids[i]].bound);
+}
+
 // The exact search (Tremor.cpp:1195-1518 visits the leaves whose bound
 // passes and computes the distance of each of their windows whose word's
 // bound passes too, with early abandoning), template ids[blockIdx.y].
@@ -208,11 +218,16 @@ constexpr int kIxPer = 8;  // windows of passing leaves per thread and step of t
 // through the early-abandoning window kernel of the scan pass (ea.hpp), which
 // drops it once its running sum proves the reference's distance above the
 // bound and otherwise makes it a candidate with an interval of that distance.
+// kNN templates with k = 1 tighten their bound as they go, as the reference's
+// best-so-far does: dyn[template] (float bits; non-negative floats order as
+// their bits) holds the smallest upper bound of a candidate's reference
+// distance found so far, which bounds the nearest window's distance too.
 __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDev *__restrict__ tmpl,
                                                         const int32_t *__restrict__ ids,
                                                         const float *__restrict__ qz, const int32_t *__restrict__ order,
                                                         const float *__restrict__ bp_g, const float *__restrict__ paa_g,
-                                                        unsigned long long *__restrict__ passed, CandOut co)
+                                                        unsigned long long *__restrict__ passed,
+                                                        unsigned int *__restrict__ dyn, CandOut co)
 {
     __shared__ float bp[256];
     __shared__ float paa[16];
@@ -234,13 +249,19 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
         gap[e] = sax_gap(bp, paa[e >> 8], e & 255);
     const uint64_t G = gridDim.x;
     const uint64_t nl = static_cast<uint64_t>(t.ix_nleaves);
+    const bool dynamic = t.k == 1;
+    const auto current = [&]() {
+        return dynamic ? fminf(t.bound, __uint_as_float(__hip_atomic_load(dyn + d, __ATOMIC_RELAXED,
+                                                                           __HIP_MEMORY_SCOPE_AGENT)))
+                       : t.bound;
+    };
     for (uint64_t r = 0; r * G * kThreads < nl; r++) {  // uniform
         __syncthreads();  // the gap table written; the last round's s_first / s_off read
         const uint64_t leaf = (r * kThreads + static_cast<uint64_t>(tid)) * G + blockIdx.x;
         uint32_t cnt = 0, start = 0;
         if (leaf < nl) {
             const IxLeaf L = t.ix_leaves[leaf];
-            if (ix_pass(ix_leaf_bound(bp, paa, L.key, L.depth, t.m), t.bound)) {
+            if (ix_pass(ix_leaf_bound(bp, paa, L.key, L.depth, t.m), current())) {
                 cnt = L.count;
                 start = L.start;
             }
@@ -275,6 +296,7 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
             // positions, so the window kernel's sample reads coalesce).  The
             // leaf of number f is the last entry whose offset is <= f
             // (entries of failing leaves share the next one's offset).
+            float bnd = current();
             uint32_t keep = 0;
 #pragma unroll
             for (int k = 0; k < kIxPer; k++) {
@@ -286,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
                         e += step;
                 const uint32_t sorted = s_first[e] + (f - s_off[e]);
                 s_sorted[k][tid] = sorted;
-                if (f < total && ix_pass(sax_lower_bound16_table(gap, t.ix_words + 16ull * sorted, t.m), t.bound))
+                if (f < total && ix_pass(sax_lower_bound16_table(gap, t.ix_words + 16ull * sorted, t.m), bnd))
                     keep |= 1u << k;
             }
             // One copy of the window kernel, the lanes on the same k at once,
@@ -296,8 +318,14 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
                 if (keep & (1u << k)) {
                     const uint32_t sorted = s_sorted[k][tid];
                     const WindowSums ws = t.ix_sums[sorted];
-                    if (ws.eS1 >= 0.0)  // else its first sample is missing: never searched (as in the scan)
-                        scan_window_sums(w, tmpl, 0, d, t.m, qz, order, kModeMain, co, nullptr, t.ix_ids[sorted], ws);
+                    if (ws.eS1 < 0.0)
+                        continue;  // its first sample is missing: never searched (as in the scan)
+                    const float hi =
+                        scan_window_sums(w, tmpl, 0, d, t.m, qz, order, kModeMain, co, nullptr, t.ix_ids[sorted], ws, bnd);
+                    if (dynamic && hi < bnd) {
+                        atomicMin(dyn + d, __float_as_uint(hi));
+                        bnd = hi;
+                    }
                 }
             }
         }
@@ -461,12 +489,15 @@ hipError_t launch_ix_count(const TmplDev *tmpl, const int32_t *ids, int count, i
 
 hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves,
                             const float *qz, const int32_t *order, const float *bp, const float *paa,
-                            unsigned long long *passed, const CandOut &co, int cus, hipStream_t stream)
+                            unsigned long long *passed, unsigned int *dyn, const CandOut &co, int cus,
+                            hipStream_t stream)
 {
     if (count <= 0 || max_leaves <= 0)
         return hipSuccess;
-    if (count > 65535 || max_leaves > 0x7fffffffll || !passed)
+    if (count > 65535 || max_leaves > 0x7fffffffll || !passed || !dyn)
         return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ix_bound_init, dim3((count + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, tmpl, ids,
+                       count, dyn);
     // Workgroups per template: about 4 per compute unit (all resident at once
     // at the window kernel's ~100 registers), and no more than a round of
     // leaves each.  The workgroups run template after template (x before y):
@@ -476,7 +507,7 @@ hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_
     const int64_t want = 4ll * std::max(cus, 1);
     const int64_t G = std::max<int64_t>(1, std::min<int64_t>(want, (max_leaves + kThreads - 1) / kThreads));
     hipLaunchKernelGGL(k_ix_search, dim3(static_cast<unsigned>(G), static_cast<unsigned>(count)), dim3(kThreads), 0,
-                       stream, w, tmpl, ids, qz, order, bp, paa, passed, co);
+                       stream, w, tmpl, ids, qz, order, bp, paa, passed, dyn, co);
     return hipGetLastError();
 }
 

@@ -1253,10 +1253,12 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     if (ix) {
         LANE_HIP(launch_zero_f64(L.ix_passed.as<double>(), static_cast<uint64_t>(plan.T), L.stream),
                  "clear index counts");
+        LANE_HIP(L.ix_dyn.reserve(sizeof(unsigned int) * static_cast<size_t>(plan.T)), "hipMalloc index bounds");
         LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(plan.ix_ids.size()),
                                   plan.ix_max_leaves, L.qz.as<float>(), L.order.as<int32_t>(),
                                   ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
-                                  L.ix_passed.as<unsigned long long>(), cand_out(L), L.cus, L.stream),
+                                  L.ix_passed.as<unsigned long long>(), L.ix_dyn.as<unsigned int>(), cand_out(L), L.cus,
+                                  L.stream),
                  "iSAX index search");
     }
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
@@ -1397,12 +1399,14 @@ constexpr uint64_t kIndexHome = 4096;  // windows of the approximate search per 
 // window kernel), and per leaf (one thread's region bound, ~100
 // instructions).  Measured on the exact search of 16 templates of 256 samples
 // at n = 10^8: threshold rho >= 0.95, 9.89e8 windows in passing leaves and
-// 6.0e7 passing their words, 8.3 ms (8.4e-12 per window all told); kNN k = 1,
-// 2.09e8 and 1.35e8, 12.05 ms (then 7.6e-11 per word that passes).  Auto
+// 6.0e7 passing their words, 8.2 ms (8.3e-12 per window all told); kNN k = 1
+// with its running bound, 1.74e8 windows in passing leaves, 6.6 ms (then
+// 3.8e-11 per word that passes, counting the 1.35e8 that pass the seeded
+// bound).  Auto
 // mode also asks that the index beat 0.9 x the planned form, as the scan must
 // (the reference's rule is set for its CPU costs: on the GPU the dense forms
 // cost picoseconds per window).
-constexpr double kIndexPerWindow = 8.5e-12, kIndexPerSurvivor = 7.6e-11, kIndexPerLeaf = 1e-12;
+constexpr double kIndexPerWindow = 8.5e-12, kIndexPerSurvivor = 4.0e-11, kIndexPerLeaf = 1e-12;
 
 int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
 {
