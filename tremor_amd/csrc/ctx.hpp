@@ -99,7 +99,7 @@ struct Lane {
     DevMem surv, surv_n;
     DevMem queue;  // the persistent matrix-core launch's per-XCD work counters
     DevMem ix_ids, ix_passed;  // templates on the iSAX index; windows of their passing leaves
-    DevMem ix_dyn;             // the index search's running bounds (k = 1), one per template
+    DevMem ix_dyn;             // running bounds (k = 1) of the scan and index searches, one per template
     uint64_t surv_cap = 0;
     uint64_t surv_need = 0;  // > surv_cap after a main pass that overflowed the queue
     uint64_t last_survivors = 0;  // level-1 survivors of the last main pass

@@ -15,12 +15,18 @@ constexpr uint32_t kBlocksPerTemplate = 2048;  // main mode: workgroups per temp
 // visited[T + t] += windows sampled.  A template with a lower bound (lb_seg >
 // 0) skips every window whose bound exceeds its own bound before reading a
 // sample (SkipSequential.cpp:256, :321); main mode counts them in *skipped.
+// Main mode with dyn: kNN templates with k = 1 tighten their bound as they
+// go, as the reference's bound set does (SkipSequential.cpp:299-330):
+// dyn[template] (float bits; non-negative floats order as their bits) holds
+// the smallest upper bound of a candidate's reference distance found so far,
+// which bounds the nearest window's distance too.
 __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev *__restrict__ tmpl, int T,
                                                       const float *__restrict__ qz, const int32_t *__restrict__ order,
                                                       const int32_t *__restrict__ ids, int mode, uint64_t sample_n,
                                                       CandOut co, unsigned long long *__restrict__ visited,
                                                       const float *__restrict__ bp_g, const float *__restrict__ paa_g,
-                                                      unsigned long long *__restrict__ skipped)
+                                                      unsigned long long *__restrict__ skipped,
+                                                      unsigned int *__restrict__ dyn)
 {
     const int t = ids[blockIdx.y];
     const int m = tmpl[t].m;
@@ -46,9 +52,19 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
         }
     }
     const uint64_t stride = mode == kModeSample ? 0 : static_cast<uint64_t>(gridDim.x) * kThreads;
+    const bool dynamic = dyn && mode != kModeSample && tmpl[t].k == 1;
     unsigned long long nskip = 0;
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;;) {
-        nskip += scan_window(w, tmpl, T, t, m, qz, order, mode, sample_n, co, visited, bp, paa, gap, lb_seg, i) ? 1 : 0;
+        const float bnd = dynamic ? fminf(tmpl[t].bound, __uint_as_float(__hip_atomic_load(
+                                                               dyn + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+                                  : tmpl[t].bound;
+        float hi;
+        nskip += scan_window(w, tmpl, T, t, m, qz, order, mode, sample_n, co, visited, bp, paa, gap, lb_seg, i, bnd,
+                             &hi)
+                     ? 1
+                     : 0;
+        if (dynamic && hi < bnd)
+            atomicMin(dyn + t, __float_as_uint(hi));
         if (mode == kModeSample)
             break;
         i += stride;
@@ -64,6 +80,15 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
     }
 }
 
+// The running bounds start at the templates' bounds.
+__global__ void k_ea_bound_init(const TmplDev *__restrict__ tmpl, const int32_t *__restrict__ ids, int count,
+                                unsigned int *__restrict__ dyn)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < count)
+        dyn[ids[i]] = __float_as_uint(tmpl[ids[i]].bound);
+}
+
 __global__ void k_mark_ea(TmplDev *__restrict__ tmpl, const int32_t *__restrict__ ids, int count, int form)
 {
     const int i = blockIdx.x * kThreads + threadIdx.x;
@@ -75,10 +100,12 @@ __global__ void k_mark_ea(TmplDev *__restrict__ tmpl, const int32_t *__restrict_
 
 // Main pass (mode kModeMain) over the templates ids[0..count) (all windows),
 // or a sample pass (sample_n windows each) accumulating the blocks visited.
+// dyn (main pass, may be null): one word per template of the batch, the
+// running bound of k = 1 templates.
 hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const float *qz, const int32_t *order,
                           const int32_t *ids, int count, uint64_t max_windows, int sample, uint64_t sample_n,
                           const CandOut &co, unsigned long long *visited, const float *bp, const float *paa,
-                          unsigned long long *skipped, hipStream_t stream)
+                          unsigned long long *skipped, unsigned int *dyn, hipStream_t stream)
 {
     if (count <= 0)
         return hipSuccess;
@@ -90,9 +117,14 @@ hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const f
         return hipSuccess;
     if (gx > 0x7fffffffull || count > 65535)
         return hipErrorInvalidValue;
+    if (sample)
+        dyn = nullptr;
+    if (dyn)
+        hipLaunchKernelGGL(k_ea_bound_init, dim3((count + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, tmpl,
+                           ids, count, dyn);
     hipLaunchKernelGGL(k_scan_ea, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(count)), dim3(kThreads), 0,
                        stream, w, tmpl, T, qz, order, ids, sample ? kModeSample : static_cast<int>(kModeMain),
-                       sample_n, co, visited, bp, paa, skipped);
+                       sample_n, co, visited, bp, paa, skipped, dyn);
     return hipGetLastError();
 }
 

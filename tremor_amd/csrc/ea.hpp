@@ -166,13 +166,17 @@ __device__ __forceinline__ float scan_window_sums(const Waveform &w, const TmplD
 
 
 // One window of the early-abandoning scan (index i of the template's
-// windows, or of its sample); true when the lower bound skipped it.
+// windows, or of its sample) against bound_f (the template's bound or a
+// tighter valid one); true when the lower bound skipped it.  *hi_out gets
+// the candidate's upper bound as scan_window_sums returns it (+inf if none).
 __device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__restrict__ tmpl, int T, int t, int m,
                                             const float *__restrict__ qz, const int32_t *__restrict__ order, int mode,
                                             uint64_t sample_n, const CandOut &co,
                                             unsigned long long *__restrict__ visited, const float *bp,
-                                            const float *paa, const float *gap, int lb_seg, uint64_t i)
+                                            const float *paa, const float *gap, int lb_seg, uint64_t i,
+                                            float bound_f, float *hi_out)
 {
+    *hi_out = INFINITY;
     const uint64_t W = w.n - static_cast<uint64_t>(m) + 1;
     uint64_t j;
     if (mode == kModeSample) {
@@ -190,15 +194,15 @@ __device__ __forceinline__ bool scan_window(const Waveform &w, const TmplDev *__
         const uint8_t *word = tmpl[t].lb_words + j * static_cast<uint64_t>(lb_seg);
         const float lb = lb_seg == 16 ? sax_lower_bound16_table(gap, word, m)
                                       : sax_lower_bound(bp, paa, word, m, lb_seg);
-        if (lb > tmpl[t].bound) {
+        if (lb > bound_f) {
             if (mode == kModeSample)
                 atomicAdd(&visited[T + t], 1ull);  // sampled, no block visited
             return true;
         }
     }
     // Statistics from the prefix sums with their error bounds.
-    scan_window_sums(w, tmpl, T, t, m, qz, order, mode, co, visited, j,
-                     window_sums_fast(w, j, j + static_cast<uint64_t>(m)), tmpl[t].bound);
+    *hi_out = scan_window_sums(w, tmpl, T, t, m, qz, order, mode, co, visited, j,
+                               window_sums_fast(w, j, j + static_cast<uint64_t>(m)), bound_f);
     return false;
 }
 

@@ -97,7 +97,7 @@ hipError_t launch_zero_u64(unsigned long long *p, hipStream_t stream);
 hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const float *qz, const int32_t *order,
                           const int32_t *ids, int count, uint64_t max_windows, int sample, uint64_t sample_n,
                           const CandOut &co, unsigned long long *visited, const float *bp, const float *paa,
-                          unsigned long long *skipped, hipStream_t stream);
+                          unsigned long long *skipped, unsigned int *dyn, hipStream_t stream);
 hipError_t launch_mark_ea(TmplDev *tmpl, const int32_t *ids, int count, int form, hipStream_t stream);
 hipError_t launch_rho_err(const Cand *cand, const float *est, const float *exact, uint64_t count,
                           const TmplDev *tmpl, unsigned int *out_bits, hipStream_t stream);
@@ -1243,17 +1243,19 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
         LANE_HIP(L.lb_skip.reserve(sizeof(unsigned long long)), "hipMalloc skip counter");
         LANE_HIP(launch_zero_u64(L.lb_skip.as<unsigned long long>(), L.stream), "reset skip counter");
     }
+    if (mode == kModeMain && (!plan.scan_ids.empty() || !plan.ix_ids.empty()))
+        LANE_HIP(L.ix_dyn.reserve(sizeof(unsigned int) * static_cast<size_t>(plan.T)), "hipMalloc running bounds");
     if (mode == kModeMain && !plan.scan_ids.empty())
         LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
                                 L.scan_ids.as<int32_t>(), static_cast<int>(plan.scan_ids.size()), plan.scan_windows,
                                 0, 0, cand_out(L), nullptr, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
-                                lb_count ? L.lb_skip.as<unsigned long long>() : nullptr, L.stream),
+                                lb_count ? L.lb_skip.as<unsigned long long>() : nullptr, L.ix_dyn.as<unsigned int>(),
+                                L.stream),
                  "early-abandoning scan");
     const bool ix = mode == kModeMain && !plan.ix_ids.empty();
     if (ix) {
         LANE_HIP(launch_zero_f64(L.ix_passed.as<double>(), static_cast<uint64_t>(plan.T), L.stream),
                  "clear index counts");
-        LANE_HIP(L.ix_dyn.reserve(sizeof(unsigned int) * static_cast<size_t>(plan.T)), "hipMalloc index bounds");
         LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(plan.ix_ids.size()),
                                   plan.ix_max_leaves, L.qz.as<float>(), L.order.as<int32_t>(),
                                   ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
@@ -1563,7 +1565,7 @@ int choose_scan(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan)
         LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
                                 L.scan_ids.as<int32_t>(), static_cast<int>(ids.size()), 0, 1, kScanSample,
                                 cand_out(L), L.scan_visit.as<unsigned long long>(), ctx->sax_bp.as<float>(),
-                                L.lb_paa.as<float>(), nullptr, L.stream),
+                                L.lb_paa.as<float>(), nullptr, nullptr, L.stream),
                  "scan sample");
         std::vector<unsigned long long> visit(2 * static_cast<size_t>(plan.T));
         LANE_HIP(hipMemcpyAsync(visit.data(), L.scan_visit.p, vb, hipMemcpyDeviceToHost, L.stream), "read scan sample");
