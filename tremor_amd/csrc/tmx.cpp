@@ -1404,11 +1404,13 @@ constexpr uint64_t kIndexHome = 4096;  // windows of the approximate search per 
 // 6.0e7 passing their words, 8.2 ms (8.3e-12 per window all told); kNN k = 1
 // with its running bound, 1.74e8 windows in passing leaves, 6.6 ms (then
 // 3.8e-11 per word that passes, counting the 1.35e8 that pass the seeded
-// bound).  Auto
+// bound; 7.6e-11 before the running bound, the figure kept for other
+// templates, whose bounds stay as seeded).  Auto
 // mode also asks that the index beat 0.9 x the planned form, as the scan must
 // (the reference's rule is set for its CPU costs: on the GPU the dense forms
 // cost picoseconds per window).
-constexpr double kIndexPerWindow = 8.5e-12, kIndexPerSurvivor = 4.0e-11, kIndexPerLeaf = 1e-12;
+constexpr double kIndexPerWindow = 8.5e-12, kIndexPerLeaf = 1e-12;
+constexpr double kIndexPerSurvivorK1 = 4.0e-11, kIndexPerSurvivor = 7.6e-11;
 
 int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_stats &st)
 {
@@ -1467,7 +1469,8 @@ int choose_index(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, CallPlan &plan, tmx_
         std::vector<int32_t> kept;
         for (int d : chosen) {
             const double W = static_cast<double>(ctx->n - static_cast<uint64_t>(plan.tmpl[d].m) + 1);
-            const double ix_s = static_cast<double>(passed[d]) * kIndexPerWindow + sampled[d] * W * kIndexPerSurvivor +
+            const double per_survivor = plan.tmpl[d].k == 1 ? kIndexPerSurvivorK1 : kIndexPerSurvivor;
+            const double ix_s = static_cast<double>(passed[d]) * kIndexPerWindow + sampled[d] * W * per_survivor +
                                 static_cast<double>(plan.tmpl[d].ix_nleaves) * kIndexPerLeaf;
             if (ix_s < 0.9 * planned_seconds_per_window(cfg, plan, d) * W)
                 kept.push_back(d);
