@@ -20,6 +20,7 @@ constexpr uint32_t kBlocksPerTemplate = 2048;  // main mode: workgroups per temp
 // dyn[template] (float bits; non-negative floats order as their bits) holds
 // the smallest upper bound of a candidate's reference distance found so far,
 // which bounds the nearest window's distance too.
+template <bool DYN>
 __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev *__restrict__ tmpl, int T,
                                                       const float *__restrict__ qz, const int32_t *__restrict__ order,
                                                       const int32_t *__restrict__ ids, int mode, uint64_t sample_n,
@@ -52,7 +53,9 @@ __global__ __launch_bounds__(kThreads) void k_scan_ea(Waveform w, const TmplDev 
         }
     }
     const uint64_t stride = mode == kModeSample ? 0 : static_cast<uint64_t>(gridDim.x) * kThreads;
-    const bool dynamic = dyn && mode != kModeSample && tmpl[t].k == 1;
+    // DYN: a separate instance, so that launches without k = 1 templates
+    // keep the plain loop (the running bound's code cost 15% there).
+    const bool dynamic = DYN && mode != kModeSample && tmpl[t].k == 1;
     unsigned long long nskip = 0;
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;;) {
         const float bnd = dynamic ? fminf(tmpl[t].bound, __uint_as_float(__hip_atomic_load(
@@ -101,7 +104,7 @@ __global__ void k_mark_ea(TmplDev *__restrict__ tmpl, const int32_t *__restrict_
 // Main pass (mode kModeMain) over the templates ids[0..count) (all windows),
 // or a sample pass (sample_n windows each) accumulating the blocks visited.
 // dyn (main pass, may be null): one word per template of the batch, the
-// running bound of k = 1 templates.
+// running bound of k = 1 templates (pass null when none is among ids).
 hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const float *qz, const int32_t *order,
                           const int32_t *ids, int count, uint64_t max_windows, int sample, uint64_t sample_n,
                           const CandOut &co, unsigned long long *visited, const float *bp, const float *paa,
@@ -119,12 +122,17 @@ hipError_t launch_scan_ea(const Waveform &w, const TmplDev *tmpl, int T, const f
         return hipErrorInvalidValue;
     if (sample)
         dyn = nullptr;
-    if (dyn)
+    const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(count));
+    const int mode = sample ? kModeSample : static_cast<int>(kModeMain);
+    if (dyn) {
         hipLaunchKernelGGL(k_ea_bound_init, dim3((count + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, tmpl,
                            ids, count, dyn);
-    hipLaunchKernelGGL(k_scan_ea, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(count)), dim3(kThreads), 0,
-                       stream, w, tmpl, T, qz, order, ids, sample ? kModeSample : static_cast<int>(kModeMain),
-                       sample_n, co, visited, bp, paa, skipped, dyn);
+        hipLaunchKernelGGL(k_scan_ea<true>, grid, dim3(kThreads), 0, stream, w, tmpl, T, qz, order, ids, mode,
+                           sample_n, co, visited, bp, paa, skipped, dyn);
+    } else {
+        hipLaunchKernelGGL(k_scan_ea<false>, grid, dim3(kThreads), 0, stream, w, tmpl, T, qz, order, ids, mode,
+                           sample_n, co, visited, bp, paa, skipped, dyn);
+    }
     return hipGetLastError();
 }
 

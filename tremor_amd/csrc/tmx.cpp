@@ -1245,12 +1245,14 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
     }
     if (mode == kModeMain && (!plan.scan_ids.empty() || !plan.ix_ids.empty()))
         LANE_HIP(L.ix_dyn.reserve(sizeof(unsigned int) * static_cast<size_t>(plan.T)), "hipMalloc running bounds");
+    const bool scan_k1 = std::any_of(plan.scan_ids.begin(), plan.scan_ids.end(),
+                                     [&](int32_t d) { return plan.tmpl[d].k == 1; });
     if (mode == kModeMain && !plan.scan_ids.empty())
         LANE_HIP(launch_scan_ea(w, L.tmpl.as<TmplDev>(), plan.T, L.qz.as<float>(), L.order.as<int32_t>(),
                                 L.scan_ids.as<int32_t>(), static_cast<int>(plan.scan_ids.size()), plan.scan_windows,
                                 0, 0, cand_out(L), nullptr, ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
-                                lb_count ? L.lb_skip.as<unsigned long long>() : nullptr, L.ix_dyn.as<unsigned int>(),
-                                L.stream),
+                                lb_count ? L.lb_skip.as<unsigned long long>() : nullptr,
+                                scan_k1 ? L.ix_dyn.as<unsigned int>() : nullptr, L.stream),
                  "early-abandoning scan");
     const bool ix = mode == kModeMain && !plan.ix_ids.empty();
     if (ix) {
