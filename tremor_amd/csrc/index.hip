@@ -222,6 +222,7 @@ __global__ __launch_bounds__(kThreads) void k_ix_bound_init(const TmplDev *__res
 // best-so-far does: dyn[template] (float bits; non-negative floats order as
 // their bits) holds the smallest upper bound of a candidate's reference
 // distance found so far, which bounds the nearest window's distance too.
+template <bool DYN>
 __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDev *__restrict__ tmpl,
                                                         const int32_t *__restrict__ ids,
                                                         const float *__restrict__ qz, const int32_t *__restrict__ order,
@@ -249,7 +250,9 @@ __global__ __launch_bounds__(kThreads) void k_ix_search(Waveform w, const TmplDe
         gap[e] = sax_gap(bp, paa[e >> 8], e & 255);
     const uint64_t G = gridDim.x;
     const uint64_t nl = static_cast<uint64_t>(t.ix_nleaves);
-    const bool dynamic = t.k == 1;
+    // DYN: a separate instance for batches with k = 1 templates, so that the
+    // others keep the plain loop.
+    const bool dynamic = DYN && t.k == 1;
     const auto current = [&]() {
         return dynamic ? fminf(t.bound, __uint_as_float(__hip_atomic_load(dyn + d, __ATOMIC_RELAXED,
                                                                            __HIP_MEMORY_SCOPE_AGENT)))
@@ -494,10 +497,11 @@ hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_
 {
     if (count <= 0 || max_leaves <= 0)
         return hipSuccess;
-    if (count > 65535 || max_leaves > 0x7fffffffll || !passed || !dyn)
+    if (count > 65535 || max_leaves > 0x7fffffffll || !passed)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_ix_bound_init, dim3((count + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, tmpl, ids,
-                       count, dyn);
+    if (dyn)
+        hipLaunchKernelGGL(k_ix_bound_init, dim3((count + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, tmpl,
+                           ids, count, dyn);
     // Workgroups per template: about 4 per compute unit (all resident at once
     // at the window kernel's ~100 registers), and no more than a round of
     // leaves each.  The workgroups run template after template (x before y):
@@ -506,8 +510,13 @@ hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_
     // device, took 3.6x as long at n = 10^8 on their scattered reads).
     const int64_t want = 4ll * std::max(cus, 1);
     const int64_t G = std::max<int64_t>(1, std::min<int64_t>(want, (max_leaves + kThreads - 1) / kThreads));
-    hipLaunchKernelGGL(k_ix_search, dim3(static_cast<unsigned>(G), static_cast<unsigned>(count)), dim3(kThreads), 0,
-                       stream, w, tmpl, ids, qz, order, bp, paa, passed, dyn, co);
+    const dim3 grid(static_cast<unsigned>(G), static_cast<unsigned>(count));
+    if (dyn)
+        hipLaunchKernelGGL(k_ix_search<true>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, qz, order, bp, paa, passed,
+                           dyn, co);
+    else
+        hipLaunchKernelGGL(k_ix_search<false>, grid, dim3(kThreads), 0, stream, w, tmpl, ids, qz, order, bp, paa, passed,
+                           dyn, co);
     return hipGetLastError();
 }
 

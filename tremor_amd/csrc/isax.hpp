@@ -156,7 +156,7 @@ hipError_t launch_ix_count(const TmplDev *tmpl, const int32_t *ids, int count, i
 // goes through the early-abandoning window kernel (a candidate unless its
 // distance is proved above the bound; windows starting with a missing sample
 // are never searched).  dyn: one word per template of the batch (ids index
-// it), the running bound of k = 1 templates.
+// it), the running bound of k = 1 templates; null when none is among ids.
 hipError_t launch_ix_search(const Waveform &w, const TmplDev *tmpl, const int32_t *ids, int count, int64_t max_leaves,
                             const float *qz, const int32_t *order, const float *bp, const float *paa,
                             unsigned long long *passed, unsigned int *dyn, const CandOut &co, int cus,

@@ -1261,8 +1261,12 @@ int run_pass(tmx_ctx *ctx, Lane &L, const tmx_cfg &cfg, const CallPlan &plan, co
         LANE_HIP(launch_ix_search(w, L.tmpl.as<TmplDev>(), L.ix_ids.as<int32_t>(), static_cast<int>(plan.ix_ids.size()),
                                   plan.ix_max_leaves, L.qz.as<float>(), L.order.as<int32_t>(),
                                   ctx->sax_bp.as<float>(), L.lb_paa.as<float>(),
-                                  L.ix_passed.as<unsigned long long>(), L.ix_dyn.as<unsigned int>(), cand_out(L), L.cus,
-                                  L.stream),
+                                  L.ix_passed.as<unsigned long long>(),
+                                  std::any_of(plan.ix_ids.begin(), plan.ix_ids.end(),
+                                              [&](int32_t d) { return plan.tmpl[d].k == 1; })
+                                      ? L.ix_dyn.as<unsigned int>()
+                                      : nullptr,
+                                  cand_out(L), L.cus, L.stream),
                  "iSAX index search");
     }
     LANE_HIP(hipEventRecord(L.ev1, L.stream), "event");
